@@ -1,0 +1,842 @@
+// dadi_hip.hip -- MI355X (gfx950) kernels and C ABI for dadi's ADI time
+// stepping of the allele-frequency density phi (1-5 populations) and the
+// phi -> spectrum sampling integral.  See include/dadi_hip.h for the ABI.
+//
+// Design (CDNA4, FP64 VALU, no MFMA):
+//  * One lane solves one tridiagonal line with the Numerical-Recipes Thomas
+//    recurrence (dadi/tridiag.c:14-34).  Lines of the strided axes are taken
+//    by consecutive lanes at consecutive addresses, so every row load/store of
+//    a wave is one coalesced 512-byte access.
+//  * Coefficients are never materialised: the per-axis 1D vectors
+//    (dfactor, xInt, V/(2dx) at both ends of each interval, selection term)
+//    live in LDS, are identical for every lane (broadcast reads), and each
+//    lane adds only its own migration term from the other axes' coordinates
+//    (dadi/integration_shared.c:4-93 folded into the forward sweep).
+//  * The forward elimination writes u' over phi in place and keeps the
+//    gamma_j column of its lane in LDS (stride 64 doubles: conflict free).
+//  * The whole time loop of an instance runs on the device.  The nD kernel is
+//    persistent: each workgroup pulls whole instances from a device-side
+//    queue; the 1D kernel does the same per lane.
+//  * Compiled with -ffp-contract=off and IEEE division, in the reference's
+//    operation order: results are bit-identical to the reference C.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+
+#include "dadi_hip.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define DH_HIP(expr)                                                            \
+    do {                                                                        \
+        hipError_t e_ = (expr);                                                 \
+        if (e_ != hipSuccess)                                                   \
+            return fail(DH_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int g_num_cu = 0;
+size_t g_max_lds = 0;
+bool g_inited = false;
+
+constexpr int LDS_TABLES_PER_AXIS = 3;  // W0, W1, S
+constexpr int LDS_GRID_TABLES = 4;      // x, xInt, dfactor, dx
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Device arithmetic, each in the reference's association order.
+// ---------------------------------------------------------------------------
+
+// V(x) = 1/nu * x * (1-x), dadi/integration_shared.c:4-6
+__device__ __forceinline__ double d_vfunc(double x, double nu)
+{
+    return 1. / nu * x * (1. - x);
+}
+
+// selection part of M, dadi/integration_shared.c:12-30
+__device__ __forceinline__ double d_sel(double x, double gamma, double h)
+{
+    return gamma * 2 * (h + (1. - 2 * h) * x) * x * (1. - x);
+}
+
+// M = m_0 (c_0 - x) + m_1 (c_1 - x) + ... + sel, summed left to right
+template <int NO>
+__device__ __forceinline__ double d_M(const double *c, const double *m, double x, double sel)
+{
+    if constexpr (NO == 0) {
+        return sel;
+    } else {
+        double acc = m[0] * (c[0] - x);
+#pragma unroll
+        for (int q = 1; q < NO; ++q) acc = acc + m[q] * (c[q] - x);
+        return acc + sel;
+    }
+}
+
+// One line: inject (optional), build a/b/c on the fly, Thomas forward with u'
+// over phi in place and gamma_j in `gam`, then back substitution.
+//   ts: stride of the 1D tables; gs: stride of the gamma column.
+template <int NO, int ORDER>
+__device__ __forceinline__ void solve_line(
+    double *__restrict__ p, long long s, int n,
+    const double *__restrict__ xInt, const double *__restrict__ df,
+    const double *__restrict__ W0, const double *__restrict__ W1,
+    const double *__restrict__ S, long long ts,
+    const double *c, const double *m,
+    double dt, double inv_dt,
+    bool bcf_on, double bcf, bool bcl_on, double bcl,
+    int inj_row, double inj,
+    double *__restrict__ gam, long long gs)
+{
+    if (inj_row >= 0) p[inj_row * s] = p[inj_row * s] + inj;
+
+    // row 0 (compute_abc_nobc, dadi/integration_shared.c:72-93)
+    double M = d_M<NO>(c, m, xInt[0], S[0]);
+    double at = M * 0.5 + W0[0];
+    double ct = -M * (1 - 0.5) + W1[0];
+    double b;
+    if constexpr (ORDER == DH_ORDER_TV) {
+        b = inv_dt + df[0] * at;
+        if (bcf_on) b += bcf;
+    } else {
+        b = df[0] * at;
+        if (bcf_on) b += bcf;
+        b = b + inv_dt;
+    }
+    double p0 = p[0];
+    double r = (ORDER == DH_ORDER_PRE) ? inv_dt * p0 : p0 / dt;
+    double bet = b;
+    double u = r / bet;
+    p[0] = u;
+    double pa = at, pc = ct;
+
+    // forward elimination (dadi/tridiag.c:25-29)
+    for (int j = 1; j < n; ++j) {
+        const double a = -df[j * ts] * pa;
+        const double cm = -df[(j - 1) * ts] * pc;
+        double na = 0., nc = 0.;
+        if (j < n - 1) {
+            const double Mj = d_M<NO>(c, m, xInt[j * ts], S[j * ts]);
+            na = Mj * 0.5 + W0[j * ts];
+            nc = -Mj * (1 - 0.5) + W1[j * ts];
+            if constexpr (ORDER == DH_ORDER_TV)
+                b = (inv_dt + df[j * ts] * pc) + df[j * ts] * na;
+            else
+                b = (df[j * ts] * na + df[j * ts] * pc) + inv_dt;
+        } else {
+            if constexpr (ORDER == DH_ORDER_TV) {
+                b = inv_dt + df[j * ts] * pc;
+                if (bcl_on) b += bcl;
+            } else {
+                b = df[j * ts] * pc;
+                if (bcl_on) b += bcl;
+                b = b + inv_dt;
+            }
+        }
+        const double g = cm / bet;
+        bet = b - a * g;
+        const double pj = p[j * s];
+        r = (ORDER == DH_ORDER_PRE) ? inv_dt * pj : pj / dt;
+        u = (r - a * u) / bet;
+        gam[j * gs] = g;
+        p[j * s] = u;
+        pa = na;
+        pc = nc;
+    }
+    // back substitution (dadi/tridiag.c:31-33)
+    for (int j = n - 2; j >= 0; --j) {
+        const double v = p[j * s] - gam[(j + 1) * gs] * u;
+        p[j * s] = v;
+        u = v;
+    }
+}
+
+// Boundary terms and injection row of one line of axis `a`.
+template <int D, int ORDER>
+struct LineSetup {
+    double c[D > 1 ? D - 1 : 1];
+    bool bcf_on, bcl_on;
+    double bcf, bcl;
+    int inj_row;
+};
+
+template <int D, int ORDER>
+__device__ __forceinline__ void line_setup(LineSetup<D, ORDER> &L, long long line, int n, int a,
+                                           const double *__restrict__ sx, const double *mig,
+                                           double nu, double gam_a, double h_a, double dx0,
+                                           double dxl, bool first_axis, int injm)
+{
+    constexpr int NO = D - 1;
+    int idx[NO > 0 ? NO : 1];
+    long long rem = line;
+#pragma unroll
+    for (int q = NO - 1; q >= 0; --q) {
+        idx[q] = (int)(rem % n);
+        rem /= n;
+    }
+    bool all0 = true, all1 = true, first = true, last = true;
+    int nones = 0, onepos = -1, nzero = 0;
+#pragma unroll
+    for (int q = 0; q < NO; ++q) {
+        L.c[q] = sx[idx[q]];
+        all0 = all0 && (L.c[q] == 0);
+        all1 = all1 && (L.c[q] == 1);
+        first = first && (idx[q] == 0);
+        last = last && (idx[q] == n - 1);
+        if (idx[q] == 1) { ++nones; onepos = q; }
+        if (idx[q] == 0) ++nzero;
+    }
+    L.bcf_on = false;
+    L.bcl_on = false;
+    L.bcf = 0.;
+    L.bcl = 0.;
+    if constexpr (ORDER == DH_ORDER_TV) {
+        // dadi/integration2D.c:67-70 and its 3-5D/1D twins
+        if (all0) {
+            const double Mf = d_M<NO>(L.c, mig, sx[0], d_sel(sx[0], gam_a, h_a));
+            if (Mf <= 0) { L.bcf_on = true; L.bcf = (0.5 / nu - Mf) * 2. / dx0; }
+        }
+        if (all1) {
+            const double Ml = d_M<NO>(L.c, mig, sx[n - 1], d_sel(sx[n - 1], gam_a, h_a));
+            if (Ml >= 0) { L.bcl_on = true; L.bcl = -(-0.5 / nu - Ml) * 2. / dxl; }
+        }
+    } else {
+        // dadi/Integration.py:1089-1092: corner lines only, M at x0 / x_{n-1}
+        if (first) {
+            double cc[NO > 0 ? NO : 1];
+#pragma unroll
+            for (int q = 0; q < NO; ++q) cc[q] = sx[0];
+            const double M0 = d_M<NO>(cc, mig, sx[0], d_sel(sx[0], gam_a, h_a));
+            if (M0 <= 0) { L.bcf_on = true; L.bcf = (0.5 / nu - M0) * 2 / dx0; }
+        }
+        if (last) {
+            double cc[NO > 0 ? NO : 1];
+#pragma unroll
+            for (int q = 0; q < NO; ++q) cc[q] = sx[n - 1];
+            const double M1 = d_M<NO>(cc, mig, sx[n - 1], d_sel(sx[n - 1], gam_a, h_a));
+            if (M1 >= 0) { L.bcl_on = true; L.bcl = -(-0.5 / nu - M1) * 2 / dxl; }
+        }
+    }
+    // mutation injection (dadi/Integration.py:70-145) happens before the
+    // first sweep of a step; each injected point lies on its own line.
+    L.inj_row = -1;
+    if (first_axis) {
+        if (nzero == NO && ((injm >> a) & 1)) {
+            L.inj_row = 1;
+        } else if (nones == 1 && nzero == NO - 1) {
+            const int pop = onepos < a ? onepos : onepos + 1;
+            if ((injm >> pop) & 1) L.inj_row = 0;
+        }
+    }
+}
+
+// Per-axis 1D coefficient vectors for one segment:
+// W0[i] = V_i/(2 dx_i), W1[i] = V_{i+1}/(2 dx_i), S[i] = sel(xInt_i).
+template <int D>
+__device__ __forceinline__ void build_axis_tables(const dh_segment *sp, int a, int n,
+                                                  const double *sx, const double *sxi,
+                                                  const double *sdx, double *W0, double *W1,
+                                                  double *S, int tid, int nthr)
+{
+    const double nu = sp->nu[a], g = sp->gamma[a], h = sp->h[a];
+    const double bp = sp->beta_p, bq = sp->beta_q;
+    for (int i = tid; i < n - 1; i += nthr) {
+        double vi = d_vfunc(sx[i], nu), vj = d_vfunc(sx[i + 1], nu);
+        if (D == 1) {  // Vfunc_beta, dadi/integration_shared.c:8-10
+            vi = vi * bp / bq;
+            vj = vj * bp / bq;
+        }
+        W0[i] = vi / (2 * sdx[i]);
+        W1[i] = vj / (2 * sdx[i]);
+        S[i] = d_sel(sxi[i], g, h);
+    }
+}
+
+// dx, xInt, dfactor: dadi/integration_shared.c:32-50
+__device__ __forceinline__ void build_grid_tables(const double *__restrict__ xx, int n,
+                                                  double *sx, double *sxi, double *sdf,
+                                                  double *sdx, int tid, int nthr)
+{
+    for (int i = tid; i < n; i += nthr) sx[i] = xx[i];
+    __syncthreads();
+    for (int i = tid; i < n - 1; i += nthr) {
+        sdx[i] = sx[i + 1] - sx[i];
+        sxi[i] = 0.5 * (sx[i + 1] + sx[i]);
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += nthr) {
+        double v;
+        if (i == 0) v = 2. / sdx[0];
+        else if (i == n - 1) v = 2. / sdx[n - 2];
+        else v = 2. / (sdx[i] + sdx[i - 1]);
+        sdf[i] = v;
+    }
+}
+
+// All lines of axis `a`, tiles of 64 lines per wave.
+template <int D, int ORDER>
+__device__ __forceinline__ void sweep_tiles(double *__restrict__ phi, int n, int a, bool first_axis,
+                                            int injm, const dh_segment *sp,
+                                            const double *sx, const double *sxi,
+                                            const double *sdf, const double *sdx,
+                                            const double *W0, const double *W1, const double *S,
+                                            double dt, double inv_dt, double *gamcol,
+                                            long long tile0, long long tstep, int lane)
+{
+    constexpr int NO = D - 1;
+    long long s = 1, nsys = 1;
+    for (int q = a + 1; q < D; ++q) s *= n;
+    for (int q = 0; q < NO; ++q) nsys *= n;
+    double mig[NO > 0 ? NO : 1];
+#pragma unroll
+    for (int q = 0; q < NO; ++q) mig[q] = sp->mig[a][q];
+    const double nu = sp->nu[a], ga = sp->gamma[a], ha = sp->h[a];
+    const double inj = sp->inject;
+    for (long long t = tile0; t * 64 < nsys; t += tstep) {
+        const long long line = t * 64 + lane;
+        if (line < nsys) {
+            LineSetup<D, ORDER> L;
+            line_setup<D, ORDER>(L, line, n, a, sx, mig, nu, ga, ha, sdx[0], sdx[n - 2],
+                                 first_axis, injm);
+            const long long hi = line / s, lo = line - hi * s;
+            double *p = phi + hi * (long long)n * s + lo;
+            solve_line<NO, ORDER>(p, s, n, sxi, sdf, W0, W1, S, 1, L.c, mig, dt, inv_dt,
+                                  L.bcf_on, L.bcf, L.bcl_on, L.bcl, L.inj_row, inj, gamcol, 64);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Kernel 1: persistent, one workgroup per instance (any D), whole schedule.
+// LDS: x, xInt, dfactor, dx (4n) | per axis W0,W1,S (3Dn) | gamma (waves*64*n)
+// ---------------------------------------------------------------------------
+template <int D, int ORDER>
+__global__ void __launch_bounds__(512) k_integrate_nd(dh_instance *__restrict__ tab, int ninst,
+                                                      int *__restrict__ counter,
+                                                      long long max_steps)
+{
+    extern __shared__ double lds[];
+    __shared__ int s_next;
+    const int nthr = blockDim.x, tid = threadIdx.x;
+    const int nw = nthr >> 6, wave = tid >> 6, lane = tid & 63;
+    for (;;) {
+        __syncthreads();
+        if (tid == 0) s_next = atomicAdd(counter, 1);
+        __syncthreads();
+        const int id = __builtin_amdgcn_readfirstlane(s_next);
+        if (id >= ninst) break;
+        dh_instance *ip = tab + id;
+        const int n = ip->n;
+        double *sx = lds, *sxi = sx + n, *sdf = sxi + n, *sdx = sdf + n;
+        double *tabs = sdx + n;
+        double *gamcol = tabs + LDS_TABLES_PER_AXIS * D * n + (long long)wave * 64 * n + lane;
+        build_grid_tables(ip->xx_dev, n, sx, sxi, sdf, sdx, tid, nthr);
+        long long seg = ip->cur_seg, step = ip->cur_step;
+        const long long nseg = ip->nseg;
+        long long budget = max_steps > 0 ? max_steps : LLONG_MAX;
+        const int frozen = ip->frozen, injm = ip->inject;
+        double *phi = ip->phi_dev;
+        while (seg < nseg && budget > 0) {
+            const dh_segment *sp = ip->seg_dev + seg;
+            __syncthreads();
+            for (int a = 0; a < D; ++a) {
+                if ((frozen >> a) & 1) continue;
+                double *W0 = tabs + LDS_TABLES_PER_AXIS * a * n;
+                build_axis_tables<D>(sp, a, n, sx, sxi, sdx, W0, W0 + n, W0 + 2 * n, tid, nthr);
+            }
+            __syncthreads();
+            const double dt = sp->dt, inv_dt = 1. / dt;
+            long long nst = sp->nsteps - step;
+            if (nst > budget) nst = budget;
+            for (long long k = 0; k < nst; ++k) {
+                bool first = true;
+                for (int a = 0; a < D; ++a) {
+                    if ((frozen >> a) & 1) continue;
+                    const double *W0 = tabs + LDS_TABLES_PER_AXIS * a * n;
+                    sweep_tiles<D, ORDER>(phi, n, a, first, injm, sp, sx, sxi, sdf, sdx, W0,
+                                          W0 + n, W0 + 2 * n, dt, inv_dt, gamcol, wave, nw, lane);
+                    __syncthreads();
+                    first = false;
+                }
+            }
+            budget -= nst;
+            step += nst;
+            if (step >= sp->nsteps) {
+                ++seg;
+                step = 0;
+            }
+        }
+        if (tid == 0) {
+            ip->cur_seg = seg;
+            ip->cur_step = step;
+            ip->status = seg >= nseg ? 1 : 0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Kernel 2: 1D instances, one lane each, lanes pull instances independently.
+// Scratch layout (coalesced over lanes): [q][i][slot], q = xInt, df, W0, W1,
+// S, gamma.
+// ---------------------------------------------------------------------------
+template <int ORDER>
+__global__ void __launch_bounds__(256) k_integrate_1d(dh_instance *__restrict__ tab, int ninst,
+                                                      int *__restrict__ counter,
+                                                      double *__restrict__ scratch,
+                                                      long long nlanes, int nmax,
+                                                      long long max_steps)
+{
+    const long long slot = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= nlanes) return;
+    double *T = scratch + slot;
+    const long long qs = (long long)nmax * nlanes;  // stride between tables
+    double *txi = T, *tdf = T + qs, *tW0 = T + 2 * qs, *tW1 = T + 3 * qs, *tS = T + 4 * qs;
+    double *tg = T + 5 * qs;
+    const long long ts = nlanes;
+    for (;;) {
+        const int id = atomicAdd(counter, 1);
+        if (id >= ninst) break;
+        dh_instance *ip = tab + id;
+        const int n = ip->n;
+        const double *xx = ip->xx_dev;
+        for (int i = 0; i < n - 1; ++i) txi[i * ts] = 0.5 * (xx[i + 1] + xx[i]);
+        for (int i = 0; i < n; ++i) {
+            double v;
+            if (i == 0) v = 2. / (xx[1] - xx[0]);
+            else if (i == n - 1) v = 2. / (xx[n - 1] - xx[n - 2]);
+            else v = 2. / ((xx[i + 1] - xx[i]) + (xx[i] - xx[i - 1]));
+            tdf[i * ts] = v;
+        }
+        const double dx0 = xx[1] - xx[0], dxl = xx[n - 1] - xx[n - 2];
+        long long seg = ip->cur_seg, step = ip->cur_step;
+        const long long nseg = ip->nseg;
+        long long budget = max_steps > 0 ? max_steps : LLONG_MAX;
+        double *phi = ip->phi_dev;
+        const bool injon = ip->inject & 1;
+        const bool frz = ip->frozen & 1;
+        while (seg < nseg && budget > 0) {
+            const dh_segment *sp = ip->seg_dev + seg;
+            const double nu = sp->nu[0], g = sp->gamma[0], h = sp->h[0];
+            const double bp = sp->beta_p, bq = sp->beta_q;
+            for (int i = 0; i < n - 1; ++i) {
+                const double dxi = xx[i + 1] - xx[i];
+                const double vi = d_vfunc(xx[i], nu) * bp / bq;
+                const double vj = d_vfunc(xx[i + 1], nu) * bp / bq;
+                tW0[i * ts] = vi / (2 * dxi);
+                tW1[i * ts] = vj / (2 * dxi);
+                tS[i * ts] = d_sel(txi[i * ts], g, h);
+            }
+            const double sel0 = d_sel(xx[0], g, h), sel1 = d_sel(xx[n - 1], g, h);
+            // 1D: the line is the whole axis, both boundary terms always apply
+            // when M has the right sign (dadi/integration1D.c:50-53).
+            const bool bf = sel0 <= 0, bl = sel1 >= 0;
+            const double bcf = (0.5 / nu - sel0) * 2. / dx0;
+            const double bcl = -(-0.5 / nu - sel1) * 2. / dxl;
+            const double dt = sp->dt, inv_dt = 1. / dt;
+            long long nst = sp->nsteps - step;
+            if (nst > budget) nst = budget;
+            if (!frz) {
+                for (long long k = 0; k < nst; ++k)
+                    solve_line<0, ORDER>(phi, 1, n, txi, tdf, tW0, tW1, tS, ts, nullptr, nullptr,
+                                         dt, inv_dt, bf, bcf, bl, bcl, injon ? 1 : -1,
+                                         sp->inject, tg, ts);
+            }
+            budget -= nst;
+            step += nst;
+            if (step >= sp->nsteps) {
+                ++seg;
+                step = 0;
+            }
+        }
+        ip->cur_seg = seg;
+        ip->cur_step = step;
+        ip->status = seg >= nseg ? 1 : 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Kernel 3: one axis of one step of one instance over many workgroups.
+// ---------------------------------------------------------------------------
+template <int D, int ORDER>
+__global__ void __launch_bounds__(256) k_sweep_axis(dh_instance inst, int a, int first_axis,
+                                                    const long long *__restrict__ base,
+                                                    long long off,
+                                                    const int *__restrict__ seg_of_step,
+                                                    long long total)
+{
+    const long long st = *base + off;
+    if (st >= total) return;
+    extern __shared__ double lds[];
+    const int n = inst.n;
+    const int nthr = blockDim.x, tid = threadIdx.x;
+    const int nw = nthr >> 6, wave = tid >> 6, lane = tid & 63;
+    const dh_segment *sp = inst.seg_dev + seg_of_step[st];
+    double *sx = lds, *sxi = sx + n, *sdf = sxi + n, *sdx = sdf + n;
+    double *W0 = sdx + n, *W1 = W0 + n, *S = W1 + n;
+    double *gamcol = S + n + (long long)wave * 64 * n + lane;
+    build_grid_tables(inst.xx_dev, n, sx, sxi, sdf, sdx, tid, nthr);
+    __syncthreads();
+    build_axis_tables<D>(sp, a, n, sx, sxi, sdx, W0, W1, S, tid, nthr);
+    __syncthreads();
+    const double dt = sp->dt, inv_dt = 1. / dt;
+    sweep_tiles<D, ORDER>(inst.phi_dev, n, a, first_axis != 0, inst.inject, sp, sx, sxi, sdf,
+                          sdx, W0, W1, S, dt, inv_dt, gamcol, (long long)blockIdx.x * nw + wave,
+                          (long long)gridDim.x * nw, lane);
+}
+
+__global__ void k_advance(long long *base, long long by)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *base += by;
+}
+
+__global__ void k_set_int(int *p, int v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+__global__ void k_set_ll(long long *p, long long v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+// ---------------------------------------------------------------------------
+// Sampling: s = dphi/dx and c1 = (phi - s x)/(m+1) per interval, stored
+// interval-major so the contraction reads them coalesced.
+// ---------------------------------------------------------------------------
+__global__ void k_sample_prep(const double *__restrict__ in, long long P, int n,
+                              const double *__restrict__ xx, int m, double *__restrict__ sT,
+                              double *__restrict__ cT)
+{
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long tot = P * (n - 1);
+    if (e >= tot) return;
+    const long long i = e / P, p = e - i * P;
+    const double f0 = in[p * n + i], f1 = in[p * n + i + 1];
+    const double s = (f1 - f0) / (xx[i + 1] - xx[i]);   // dadi/Spectrum_mod.py:1527
+    const double c1 = (f0 - s * xx[i]) / (m + 1);        // dadi/Spectrum_mod.py:1528
+    sT[e] = s;
+    cT[e] = c1;
+}
+
+__global__ void k_sample_contract(const double *__restrict__ sT, const double *__restrict__ cT,
+                                  long long P, int n, int m, const double *__restrict__ db1,
+                                  const double *__restrict__ db2, double *__restrict__ out)
+{
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long long)(m + 1) * P) return;
+    const long long k = e / P, p = e - k * P;
+    const double *d1 = db1 + k * (n - 1), *d2 = db2 + k * (n - 1);
+    double t1 = 0., t2 = 0.;
+    for (int i = 0; i < n - 1; ++i) {
+        t1 = t1 + cT[(long long)i * P + p] * d1[i];
+        t2 = t2 + sT[(long long)i * P + p] * d2[i];
+    }
+    // dadi/Spectrum_mod.py:1531-1532: term2 *= (k+1)/((m+1)(m+2)); term1 + term2
+    const double fac = (double)(k + 1) / (double)((m + 1) * (m + 2));
+    out[e] = t1 + t2 * fac;
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+namespace {
+
+template <int D, int ORDER>
+const void *nd_kernel() { return reinterpret_cast<const void *>(&k_integrate_nd<D, ORDER>); }
+
+const void *pick_nd(int D, int order)
+{
+    switch (D * 4 + order) {
+#define DH_CASE(d, o) case d * 4 + o: return nd_kernel<d, o>();
+        DH_CASE(1, 0) DH_CASE(1, 1) DH_CASE(1, 2)
+        DH_CASE(2, 0) DH_CASE(2, 1) DH_CASE(2, 2)
+        DH_CASE(3, 0) DH_CASE(3, 1) DH_CASE(3, 2)
+        DH_CASE(4, 0) DH_CASE(4, 1) DH_CASE(4, 2)
+        DH_CASE(5, 0) DH_CASE(5, 1) DH_CASE(5, 2)
+#undef DH_CASE
+    }
+    return nullptr;
+}
+
+template <int D, int ORDER>
+const void *sw_kernel() { return reinterpret_cast<const void *>(&k_sweep_axis<D, ORDER>); }
+
+const void *pick_sweep(int D, int order)
+{
+    switch (D * 4 + order) {
+#define DH_CASE(d, o) case d * 4 + o: return sw_kernel<d, o>();
+        DH_CASE(1, 0) DH_CASE(1, 1) DH_CASE(1, 2)
+        DH_CASE(2, 0) DH_CASE(2, 1) DH_CASE(2, 2)
+        DH_CASE(3, 0) DH_CASE(3, 1) DH_CASE(3, 2)
+        DH_CASE(4, 0) DH_CASE(4, 1) DH_CASE(4, 2)
+        DH_CASE(5, 0) DH_CASE(5, 1) DH_CASE(5, 2)
+#undef DH_CASE
+    }
+    return nullptr;
+}
+
+const void *pick_1d(int order)
+{
+    switch (order) {
+    case 0: return reinterpret_cast<const void *>(&k_integrate_1d<0>);
+    case 1: return reinterpret_cast<const void *>(&k_integrate_1d<1>);
+    case 2: return reinterpret_cast<const void *>(&k_integrate_1d<2>);
+    }
+    return nullptr;
+}
+
+size_t nd_lds(int D, int n, int waves)
+{
+    return sizeof(double) * (size_t)n *
+           (LDS_GRID_TABLES + LDS_TABLES_PER_AXIS * (size_t)D + 64 * (size_t)waves);
+}
+
+size_t sweep_lds(int n, int waves)
+{
+    return sizeof(double) * (size_t)n * (LDS_GRID_TABLES + LDS_TABLES_PER_AXIS + 64 * (size_t)waves);
+}
+
+int check_init()
+{
+    if (!g_inited) return fail(DH_EINVAL, "dh_init() has not been called");
+    return DH_OK;
+}
+
+int set_lds(const void *fn, size_t bytes)
+{
+    if (bytes > g_max_lds)
+        return fail(DH_ERANGE, "LDS request of " + std::to_string(bytes) + " bytes exceeds " +
+                                   std::to_string(g_max_lds));
+    DH_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    return DH_OK;
+}
+
+std::mutex g_graph_mu;
+std::map<std::string, hipGraphExec_t> g_graphs;
+
+}  // namespace
+
+extern "C" {
+
+int dh_abi_version(void) { return 1; }
+size_t dh_sizeof_segment(void) { return sizeof(dh_segment); }
+size_t dh_sizeof_instance(void) { return sizeof(dh_instance); }
+size_t dh_lds_bytes(int D, int n, int waves) { return nd_lds(D, n, waves); }
+size_t dh_max_lds(void) { return g_max_lds; }
+
+int dh_last_error(char *buf, int len)
+{
+    if (buf && len > 0) {
+        std::strncpy(buf, g_err.c_str(), (size_t)len - 1);
+        buf[len - 1] = 0;
+    }
+    return (int)g_err.size();
+}
+
+int dh_init(int device)
+{
+    DH_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    DH_HIP(hipGetDeviceProperties(&prop, device));
+    g_num_cu = prop.multiProcessorCount;
+    int maxsh = 0;
+    if (hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, device) !=
+            hipSuccess ||
+        maxsh <= 0)
+        maxsh = (int)prop.sharedMemPerBlock;
+    g_max_lds = (size_t)maxsh;
+    g_inited = true;
+    return DH_OK;
+}
+
+int dh_integrate_nd(dh_instance *table_dev, int ninst, int D, int order, int nmax,
+                    int *counter_dev, const dh_launch *cfg, void *stream)
+{
+    if (int e = check_init()) return e;
+    if (D < 1 || D > DH_MAXD) return fail(DH_EINVAL, "D must be 1..5");
+    if (order < 0 || order > 2) return fail(DH_EINVAL, "bad order");
+    if (ninst <= 0) return DH_OK;
+    if (!table_dev || !counter_dev || !cfg) return fail(DH_EINVAL, "null pointer");
+    if (nmax < 3) return fail(DH_EINVAL, "n must be >= 3");
+    const int waves = cfg->waves;
+    if (waves < 1 || waves > 8) return fail(DH_EINVAL, "waves must be 1..8");
+    const void *fn = pick_nd(D, order);
+    const size_t lds = nd_lds(D, nmax, waves);
+    if (int e = set_lds(fn, lds)) return e;
+    int groups = cfg->groups;
+    if (groups <= 0) {
+        int per_cu = (int)(g_max_lds / lds);
+        if (per_cu < 1) per_cu = 1;
+        if (per_cu > 16 / waves + 1) per_cu = 16 / waves + 1;
+        groups = g_num_cu * per_cu;
+    }
+    if (groups > ninst) groups = ninst;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, counter_dev, 0);
+    long long ms = cfg->max_steps;
+    void *args[] = {&table_dev, &ninst, &counter_dev, &ms};
+    DH_HIP(hipLaunchKernel(fn, dim3(groups), dim3(64 * waves), args, lds, st));
+    return DH_OK;
+}
+
+int dh_integrate_1d(dh_instance *table_dev, int ninst, int order, int nmax, int *counter_dev,
+                    double *scratch_dev, long long scratch_doubles, const dh_launch *cfg,
+                    void *stream)
+{
+    if (int e = check_init()) return e;
+    if (order < 0 || order > 2) return fail(DH_EINVAL, "bad order");
+    if (ninst <= 0) return DH_OK;
+    if (!table_dev || !counter_dev || !cfg || !scratch_dev) return fail(DH_EINVAL, "null pointer");
+    if (nmax < 3) return fail(DH_EINVAL, "n must be >= 3");
+    const int waves = cfg->waves;
+    if (waves < 1 || waves > 4) return fail(DH_EINVAL, "waves must be 1..4 for the 1D kernel");
+    const int thr = 64 * waves;
+    long long groups = cfg->groups;
+    if (groups <= 0) groups = (ninst + thr - 1) / thr;
+    const long long nlanes = groups * thr;
+    if (6LL * nmax * nlanes > scratch_doubles)
+        return fail(DH_ERANGE, "1D scratch too small: need " + std::to_string(6LL * nmax * nlanes) +
+                                   " doubles");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, counter_dev, 0);
+    long long ms = cfg->max_steps;
+    void *args[] = {&table_dev, &ninst, &counter_dev, &scratch_dev, (void *)&nlanes, &nmax, &ms};
+    DH_HIP(hipLaunchKernel(pick_1d(order), dim3((unsigned)groups), dim3(thr), args, 0, st));
+    return DH_OK;
+}
+
+int dh_sweep_axis(const dh_instance *inst_host, int axis, int first_axis,
+                  const long long *step_base_dev, long long step_offset,
+                  const int *seg_of_step_dev, long long total_steps, int waves, void *stream)
+{
+    if (int e = check_init()) return e;
+    if (!inst_host) return fail(DH_EINVAL, "null instance");
+    const dh_instance I = *inst_host;
+    if (I.D < 1 || I.D > DH_MAXD || I.order < 0 || I.order > 2 || axis < 0 || axis >= I.D)
+        return fail(DH_EINVAL, "bad instance/axis");
+    if (I.n < 3) return fail(DH_EINVAL, "n must be >= 3");
+    if (waves < 1 || waves > 4) return fail(DH_EINVAL, "waves must be 1..4");
+    long long nsys = 1;
+    for (int q = 0; q < I.D - 1; ++q) nsys *= I.n;
+    const long long tiles = (nsys + 63) / 64;
+    long long groups = (tiles + waves - 1) / waves;
+    const void *fn = pick_sweep(I.D, I.order);
+    const size_t lds = sweep_lds(I.n, waves);
+    if (int e = set_lds(fn, lds)) return e;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    dh_instance Ic = I;
+    int a = axis, fa = first_axis;
+    void *args[] = {&Ic, &a, &fa, (void *)&step_base_dev, &step_offset, (void *)&seg_of_step_dev,
+                    &total_steps};
+    DH_HIP(hipLaunchKernel(fn, dim3((unsigned)groups), dim3(64 * waves), args, lds, st));
+    return DH_OK;
+}
+
+int dh_advance_steps(long long *step_base_dev, long long by, void *stream)
+{
+    if (int e = check_init()) return e;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, st, step_base_dev, by);
+    DH_HIP(hipGetLastError());
+    return DH_OK;
+}
+
+int dh_integrate_graph(const dh_instance *inst_host, long long *step_base_dev,
+                       const int *seg_of_step_dev, long long total_steps, int nsteps_per_graph,
+                       int waves, void *stream)
+{
+    if (int e = check_init()) return e;
+    if (!inst_host || !step_base_dev || !seg_of_step_dev) return fail(DH_EINVAL, "null pointer");
+    if (nsteps_per_graph < 1) return fail(DH_EINVAL, "nsteps_per_graph must be >= 1");
+    if (total_steps <= 0) return DH_OK;
+    const dh_instance I = *inst_host;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    // key: everything baked into the captured launches
+    std::string key((const char *)&I, sizeof(I));
+    key.append((const char *)&step_base_dev, sizeof(step_base_dev));
+    key.append((const char *)&seg_of_step_dev, sizeof(seg_of_step_dev));
+    key.append((const char *)&total_steps, sizeof(total_steps));
+    key.append((const char *)&nsteps_per_graph, sizeof(int));
+    key.append((const char *)&waves, sizeof(int));
+    hipGraphExec_t exec = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_graph_mu);
+        auto it = g_graphs.find(key);
+        if (it != g_graphs.end()) exec = it->second;
+    }
+    if (!exec) {
+        hipStream_t cap;
+        DH_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+        DH_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+        int rc = DH_OK;
+        for (int k = 0; k < nsteps_per_graph && rc == DH_OK; ++k) {
+            int first = 1;
+            for (int a = 0; a < I.D && rc == DH_OK; ++a) {
+                if ((I.frozen >> a) & 1) continue;
+                rc = dh_sweep_axis(&I, a, first, step_base_dev, k, seg_of_step_dev, total_steps,
+                                   waves, cap);
+                first = 0;
+            }
+        }
+        if (rc == DH_OK) hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, cap, step_base_dev,
+                                            (long long)nsteps_per_graph);
+        hipGraph_t graph = nullptr;
+        hipError_t ec = hipStreamEndCapture(cap, &graph);
+        (void)hipStreamDestroy(cap);
+        if (rc != DH_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (ec != hipSuccess) return fail(DH_EHIP, std::string("capture: ") + hipGetErrorString(ec));
+        hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess)
+            return fail(DH_EHIP, std::string("instantiate: ") + hipGetErrorString(ei));
+        std::lock_guard<std::mutex> lk(g_graph_mu);
+        if (g_graphs.size() > 64) {
+            for (auto &kv : g_graphs) (void)hipGraphExecDestroy(kv.second);
+            g_graphs.clear();
+        }
+        g_graphs[key] = exec;
+    }
+    hipLaunchKernelGGL(k_set_ll, dim3(1), dim3(1), 0, st, step_base_dev, 0LL);
+    const long long reps = (total_steps + nsteps_per_graph - 1) / nsteps_per_graph;
+    for (long long r = 0; r < reps; ++r) DH_HIP(hipGraphLaunch(exec, st));
+    return DH_OK;
+}
+
+int dh_sample_axis(const double *in_dev, long long P, int n, const double *xx_dev, int m,
+                   const double *db1_dev, const double *db2_dev, double *work_dev,
+                   double *out_dev, void *stream)
+{
+    if (int e = check_init()) return e;
+    if (!in_dev || !xx_dev || !db1_dev || !db2_dev || !work_dev || !out_dev)
+        return fail(DH_EINVAL, "null pointer");
+    if (P <= 0 || n < 2 || m < 0) return fail(DH_EINVAL, "bad sampling shape");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const long long tot = P * (n - 1);
+    double *sT = work_dev, *cT = work_dev + tot;
+    const int B = 256;
+    hipLaunchKernelGGL(k_sample_prep, dim3((unsigned)((tot + B - 1) / B)), dim3(B), 0, st, in_dev,
+                       P, n, xx_dev, m, sT, cT);
+    const long long outn = (long long)(m + 1) * P;
+    hipLaunchKernelGGL(k_sample_contract, dim3((unsigned)((outn + B - 1) / B)), dim3(B), 0, st,
+                       sT, cT, P, n, m, db1_dev, db2_dev, out_dev);
+    DH_HIP(hipGetLastError());
+    return DH_OK;
+}
+
+}  // extern "C"

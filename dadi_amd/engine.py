@@ -1,0 +1,265 @@
+"""
+Device side of the integration: packs schedules and instance descriptors into
+device buffers (torch tensors), picks a launch shape and calls the C ABI.
+
+Three execution modes, all computing bit-identical results:
+
+* ``run_single``  one integration per call (the "plain" path): a single
+  persistent workgroup when the swept axis has few lines, otherwise one
+  multi-workgroup launch per axis per step replayed from a hipGraph.
+* ``run_batch``   many independent integrations in one persistent launch per
+  (D, order) class, workgroups (D >= 2) or lanes (D == 1) pulling instances
+  from a device-side queue.
+* quanta (``max_steps``) let a caller advance a resident batch a few steps at
+  a time; the serving loop in :mod:`dadi_amd.serve` builds on that.
+"""
+import math
+import threading
+
+import numpy as np
+
+from . import _lib
+from ._lib import INST_DTYPE, SEG_DTYPE, DhInstance, DhLaunch, check
+
+_state = threading.local()
+
+
+def torch():
+    import torch as _t
+    return _t
+
+
+class Context:
+    """Per-device state: library handle, device, scratch buffers."""
+
+    def __init__(self, device=0):
+        t = torch()
+        self.lib = _lib.init(device)
+        self.device = t.device("cuda", device)
+        props = t.cuda.get_device_properties(self.device)
+        self.num_cu = props.multi_processor_count
+        self.max_lds = int(self.lib.dh_max_lds())
+        self._xx_cache = {}
+        self._scratch = None
+        self.tuning = None
+
+    # -- helpers ---------------------------------------------------------
+    def stream(self):
+        return torch().cuda.current_stream(self.device).cuda_stream
+
+    def to_dev(self, arr, dtype=None):
+        t = torch()
+        a = np.ascontiguousarray(arr)
+        return t.from_numpy(a).to(self.device, non_blocking=False)
+
+    def grid(self, xx):
+        key = (len(xx), xx.tobytes())
+        g = self._xx_cache.get(key)
+        if g is None:
+            g = self.to_dev(np.asarray(xx, dtype=np.float64))
+            if len(self._xx_cache) > 256:
+                self._xx_cache.clear()
+            self._xx_cache[key] = g
+        return g
+
+    def scratch(self, ndoubles):
+        t = torch()
+        if self._scratch is None or self._scratch.numel() < ndoubles:
+            self._scratch = t.empty(int(ndoubles), dtype=t.float64, device=self.device)
+        return self._scratch
+
+    def max_waves(self, D, n):
+        """Largest waves-per-workgroup whose LDS fits."""
+        per = 8 * n
+        free = self.max_lds // per - (4 + 3 * D)
+        return max(0, min(8, free // 64))
+
+
+def context(device=0):
+    ctx = getattr(_state, "ctx", None)
+    if ctx is None or ctx.device.index != device:
+        ctx = Context(device)
+        _state.ctx = ctx
+    return ctx
+
+
+def _lines(D, n):
+    return n ** (D - 1)
+
+
+def _inst_record(phi_ptr, xx_ptr, seg_ptr, nseg, D, n, order, frozen, inject):
+    rec = np.zeros(1, dtype=INST_DTYPE)
+    rec["phi"] = phi_ptr
+    rec["xx"] = xx_ptr
+    rec["seg"] = seg_ptr
+    rec["nseg"] = nseg
+    rec["D"] = D
+    rec["n"] = n
+    rec["order"] = order
+    rec["frozen"] = frozen
+    rec["inject"] = inject
+    return rec
+
+
+def run_single(phi, xx, seg, order, frozen, inject, ctx=None, mode="auto", waves=None,
+               steps_per_graph=32):
+    """Integrate one phi in place on the device.
+
+    phi: torch float64 tensor on the device (C-contiguous, shape (n,)*D), or a
+    numpy array (copied in and out; the numpy result is returned).
+    """
+    t = torch()
+    ctx = ctx or context()
+    host = isinstance(phi, np.ndarray)
+    phid = ctx.to_dev(np.ascontiguousarray(phi, dtype=np.float64)) if host else phi
+    if not (phid.is_cuda and phid.dtype == t.float64 and phid.is_contiguous()):
+        raise ValueError("phi must be a contiguous float64 device tensor")
+    D = phid.dim()
+    n = int(phid.shape[0])
+    if n < 3:
+        raise ValueError("grids need at least 3 points")
+    if len(seg) == 0 or frozen == (1 << D) - 1:
+        return phid.cpu().numpy() if host else phid
+    xxd = ctx.grid(np.asarray(xx, dtype=np.float64))
+    segd = ctx.to_dev(seg.view(np.uint8))
+    nsys = _lines(D, n)
+    wmax = ctx.max_waves(D, n)
+    if mode == "auto":
+        mode = "wg" if (D == 1 or nsys <= 64 * max(wmax, 1)) and wmax >= 1 else "graph"
+    stream = ctx.stream()
+    if mode == "wg":
+        if wmax < 1:
+            raise ValueError("n=%d too large for the workgroup kernel at D=%d" % (n, D))
+        w = waves or max(1, min(wmax, math.ceil(nsys / 64)))
+        rec = _inst_record(phid.data_ptr(), xxd.data_ptr(), segd.data_ptr(), len(seg), D, n,
+                           order, frozen, inject)
+        tab = ctx.to_dev(rec.view(np.uint8))
+        counter = t.zeros(1, dtype=t.int32, device=ctx.device)
+        cfg = DhLaunch(w, 1, 0)
+        check(ctx.lib.dh_integrate_nd(tab.data_ptr(), 1, D, order, n, counter.data_ptr(),
+                                      cfg, stream), "dh_integrate_nd")
+        keep = (tab, counter, segd)
+    elif mode == "graph":
+        nst = int(np.sum(seg["nsteps"]))
+        seg_of_step = np.repeat(np.arange(len(seg), dtype=np.int32), seg["nsteps"])
+        sosd = ctx.to_dev(seg_of_step)
+        base = t.zeros(1, dtype=t.int64, device=ctx.device)
+        inst = DhInstance(phid.data_ptr(), xxd.data_ptr(), segd.data_ptr(), len(seg), 0, 0,
+                          D, n, order, frozen, inject, 0)
+        w = waves or 2
+        check(ctx.lib.dh_integrate_graph(inst, base.data_ptr(), sosd.data_ptr(), nst,
+                                         steps_per_graph, w, stream), "dh_integrate_graph")
+        keep = (sosd, base, segd)
+    else:
+        raise ValueError(mode)
+    if host:
+        out = phid.cpu().numpy()
+        del keep
+        return out
+    t.cuda.current_stream(ctx.device).synchronize()
+    del keep
+    return phid
+
+
+class Job:
+    """One integration of a batch: device phi (updated in place) plus schedule."""
+    __slots__ = ("phi", "xx", "seg", "order", "frozen", "inject", "tag")
+
+    def __init__(self, phi, xx, seg, order, frozen, inject, tag=None):
+        self.phi = phi
+        self.xx = np.asarray(xx, dtype=np.float64)
+        self.seg = seg
+        self.order = order
+        self.frozen = frozen
+        self.inject = inject
+        self.tag = tag
+
+    @property
+    def D(self):
+        return self.phi.dim()
+
+    @property
+    def n(self):
+        return int(self.phi.shape[0])
+
+    def work(self):
+        act = sum(1 for k in range(self.D) if not (self.frozen >> k) & 1)
+        return int(np.sum(self.seg["nsteps"])) * (self.n ** self.D) * max(act, 1)
+
+
+class PackedBatch:
+    """Device-resident instance table of one (D, order) class."""
+
+    def __init__(self, ctx, jobs):
+        t = torch()
+        self.ctx = ctx
+        self.jobs = jobs
+        self.D = jobs[0].D
+        self.order = jobs[0].order
+        segs = [j.seg for j in jobs]
+        offs = np.cumsum([0] + [len(s) for s in segs])
+        allseg = np.concatenate(segs) if segs else np.zeros(0, dtype=SEG_DTYPE)
+        self.segd = ctx.to_dev(allseg.view(np.uint8))
+        base = self.segd.data_ptr()
+        tab = np.zeros(len(jobs), dtype=INST_DTYPE)
+        self.xxs = []
+        for i, j in enumerate(jobs):
+            xxd = ctx.grid(j.xx)
+            self.xxs.append(xxd)
+            tab[i] = _inst_record(j.phi.data_ptr(), xxd.data_ptr(),
+                                  base + int(offs[i]) * SEG_DTYPE.itemsize, len(j.seg), j.D, j.n,
+                                  j.order, j.frozen, j.inject)[0]
+        self.host_tab = tab
+        self.tab = ctx.to_dev(tab.view(np.uint8))
+        self.counter = t.zeros(1, dtype=t.int32, device=ctx.device)
+        self.nmax = max(j.n for j in jobs)
+
+    def launch(self, waves=None, groups=0, max_steps=0, stream=None):
+        ctx = self.ctx
+        stream = ctx.stream() if stream is None else stream
+        if self.D == 1:
+            w = waves or 1
+            thr = 64 * w
+            g = groups or max(1, math.ceil(len(self.jobs) / thr))
+            nl = g * thr
+            need = 6 * self.nmax * nl
+            scr = ctx.scratch(need)
+            cfg = DhLaunch(w, g, max_steps)
+            check(ctx.lib.dh_integrate_1d(self.tab.data_ptr(), len(self.jobs), self.order,
+                                          self.nmax, self.counter.data_ptr(), scr.data_ptr(),
+                                          scr.numel(), cfg, stream), "dh_integrate_1d")
+        else:
+            wmax = ctx.max_waves(self.D, self.nmax)
+            if wmax < 1:
+                raise ValueError("n=%d too large for the workgroup kernel at D=%d"
+                                 % (self.nmax, self.D))
+            nsys = _lines(self.D, self.nmax)
+            w = waves or max(1, min(wmax, math.ceil(nsys / 64)))
+            w = min(w, wmax)
+            cfg = DhLaunch(w, groups, max_steps)
+            check(ctx.lib.dh_integrate_nd(self.tab.data_ptr(), len(self.jobs), self.D,
+                                          self.order, self.nmax, self.counter.data_ptr(), cfg,
+                                          stream), "dh_integrate_nd")
+
+    def progress(self):
+        raw = self.tab.cpu().numpy().view(INST_DTYPE)
+        return raw
+
+
+def run_batch(jobs, ctx=None, waves=None, groups=0, sort=True):
+    """Run every job to completion; jobs are grouped by (D, order) and each
+    group runs as one persistent launch (longest job first)."""
+    ctx = ctx or context()
+    groups_by = {}
+    for j in jobs:
+        groups_by.setdefault((j.D, j.order), []).append(j)
+    packs = []
+    for key in sorted(groups_by):
+        js = groups_by[key]
+        if sort:
+            js = sorted(js, key=lambda j: -j.work())
+        pb = PackedBatch(ctx, js)
+        pb.launch(waves=waves, groups=groups)
+        packs.append(pb)
+    torch().cuda.current_stream(ctx.device).synchronize()
+    return packs

@@ -1,0 +1,168 @@
+/*
+ * dadi_hip.h -- C ABI of libdadi_hip.so, the MI355X (gfx950) implementation of
+ * dadi's ADI time stepping of the allele-frequency density phi in 1-5
+ * populations and of the phi -> frequency-spectrum sampling integral.
+ *
+ * Conventions
+ *   - Every pointer named *_dev is a device pointer; the caller owns all
+ *     memory (the Python layer allocates it as torch tensors).  The library
+ *     never allocates device memory.
+ *   - Every entry point returns 0 on success and a negative DH_E* code on
+ *     failure; dh_last_error() then holds a message.  Errors are per host
+ *     thread.
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream).
+ *     Calls only enqueue work; they do not synchronise.
+ *   - All arithmetic is IEEE FP64 with no contraction into FMA, in the
+ *     reference C's operation order, so results are bit-identical to
+ *     dadi/integration{1..5}D.c and dadi/tridiag.c on the same inputs.
+ */
+#ifndef DADI_HIP_H
+#define DADI_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DH_MAXD 5
+
+#define DH_OK 0
+#define DH_EINVAL -1   /* bad argument */
+#define DH_EHIP -2     /* HIP runtime error */
+#define DH_ERANGE -3   /* size beyond what the kernel supports */
+
+/* Coefficient/RHS order of the implicit system (see DH_ORDER_* users):
+ *  TV   = dadi/integration{1..5}D.c implicit_* : b = 1/dt + fluxes, r = phi/dt
+ *  PRE  = dadi/Integration.py:1037-1242 + implicit_precalc_{2,3}D* :
+ *         b = (fluxes + boundary) + 1/dt, r = (1/dt)*phi
+ *  PRE1 = dadi/Integration.py:986-1035 (1D constant parameters):
+ *         b = (fluxes + boundary) + 1/dt, r = phi/dt                      */
+#define DH_ORDER_TV 0
+#define DH_ORDER_PRE 1
+#define DH_ORDER_PRE1 2
+
+/* A run of identical time steps: every step of the segment injects
+ * `inject` at index 1 of each injecting axis (dadi/Integration.py:70-145),
+ * then sweeps every non-frozen axis in order with step `dt`.  Parameters are
+ * per population k; mig[k][j] is the migration rate into population k from
+ * the j-th other population counted in increasing order (axis 1 of a
+ * 3-population phi: m21, m23). beta_p/beta_q scale V in 1D only:
+ * V = x(1-x)/nu * beta_p / beta_q with beta_p = pow(beta+1,2), beta_q = 4*beta
+ * (dadi/integration_shared.c:8-10). */
+typedef struct dh_segment {
+    long long nsteps;
+    double dt;
+    double inject;
+    double beta_p, beta_q;
+    double nu[DH_MAXD];
+    double gamma[DH_MAXD];
+    double h[DH_MAXD];
+    double mig[DH_MAXD][DH_MAXD - 1];
+} dh_segment;
+
+/* One integration ("instance").  phi_dev holds n^D doubles in C order and is
+ * updated in place.  cur_seg/cur_step record progress: a launch resumes from
+ * them and writes them back, so a schedule can be run in several quanta. */
+typedef struct dh_instance {
+    double *phi_dev;
+    const double *xx_dev;
+    const dh_segment *seg_dev;
+    long long nseg;
+    long long cur_seg;
+    long long cur_step;
+    int D;
+    int n;
+    int order;
+    int frozen;  /* bit k set: population k frozen (its axis is not swept) */
+    int inject;  /* bit k set: population k receives new mutations */
+    int status;  /* 1 when the schedule has completed */
+} dh_instance;
+
+/* Launch shape of the workgroup-per-instance kernel. */
+typedef struct dh_launch {
+    int waves;             /* 64-lane waves per workgroup, 1..8 */
+    int groups;            /* persistent workgroups; 0 = one per instance, capped */
+    long long max_steps;   /* steps per instance in this call; <= 0 = to completion */
+} dh_launch;
+
+/* Library / device set-up; must be called once per process before use.
+ * Replaces the pycuda context creation of dadi/cuda/__init__.py:3-10. */
+int dh_init(int device);
+/* Copies the last error message of this thread into buf; returns its length. */
+int dh_last_error(char *buf, int len);
+/* ABI self-description, checked by the Python loader. */
+int dh_abi_version(void);
+size_t dh_sizeof_segment(void);
+size_t dh_sizeof_instance(void);
+/* LDS footprint per workgroup of the nD kernel for (D, n, waves), bytes. */
+size_t dh_lds_bytes(int D, int n, int waves);
+/* Largest LDS a workgroup may use on the initialised device (0 before dh_init). */
+size_t dh_max_lds(void);
+
+/*
+ * Runs ninst instances, one workgroup per instance at a time, pulled from a
+ * device-side queue.  All instances must share D and order.  counter_dev is
+ * one int of device memory used as the queue head (reset by this call).
+ * Each workgroup keeps the per-axis coefficient vectors in LDS and one Thomas
+ * scratch column per lane.  Replaces, per time step, the Python-driven loop of
+ * dadi/Integration.py:1108-1116 / 355-386 / 514-553 / 682-726 / 886-936 and
+ * the reference GPU path dadi/cuda/Integration.py:89-995.
+ */
+int dh_integrate_nd(dh_instance *table_dev, int ninst, int D, int order, int nmax,
+                    int *counter_dev, const dh_launch *cfg, void *stream);
+
+/*
+ * Runs ninst 1-population instances, one lane per instance, lanes pulling
+ * instances from a device-side queue independently (lane-level continuous
+ * batching).  scratch_dev must hold 6 * nmax * lanes doubles where
+ * lanes = 64 * waves * groups; nmax is the largest n in the table.
+ * Replaces dadi/Integration.py:986-1035 / 219-258 with
+ * dadi/tridiag.c:36-40 and dadi/integration1D.c:7-66.
+ */
+int dh_integrate_1d(dh_instance *table_dev, int ninst, int order, int nmax,
+                    int *counter_dev, double *scratch_dev, long long scratch_doubles,
+                    const dh_launch *cfg, void *stream);
+
+/*
+ * One implicit half-step along `axis` for a single instance, spread over as
+ * many workgroups as the axis has 64-line tiles.  The step is taken from the
+ * schedule: s = *step_base_dev + step_offset; seg_of_step_dev[s] names the
+ * segment; nothing happens when s >= total_steps.  first_axis = 1 makes the
+ * sweep add the step's injected mass first.  Used by the plain one-call path
+ * for large phi, captured per step into a hipGraph.  Replaces one
+ * implicit_<D>D<axis> call (dadi/integration{1..5}D.c) /
+ * implicit_precalc_{2,3}D<axis>.
+ */
+int dh_sweep_axis(const dh_instance *inst_host, int axis, int first_axis,
+                  const long long *step_base_dev, long long step_offset,
+                  const int *seg_of_step_dev, long long total_steps,
+                  int waves, void *stream);
+/* Adds `by` to *step_base_dev (one thread), closing a captured group of steps. */
+int dh_advance_steps(long long *step_base_dev, long long by, void *stream);
+/* Captures `nsteps_per_graph` steps of dh_sweep_axis launches (all active axes)
+ * plus dh_advance_steps into a hipGraph and replays it until total_steps are
+ * done.  The graph is cached per instance descriptor and reused. */
+int dh_integrate_graph(const dh_instance *inst_host, long long *step_base_dev,
+                       const int *seg_of_step_dev, long long total_steps,
+                       int nsteps_per_graph, int waves, void *stream);
+
+/*
+ * Sampling phi -> expected spectrum (dadi/Spectrum_mod.py:1307-1348,
+ * 1509-1548, 1774-1871): contracts the last axis of in_dev, shaped
+ * (P, n), against the binomial sampling weights of sample size m and writes
+ * out_dev shaped (m+1, P) (the new sample axis first), i.e.
+ *   out[k][p] = sum_i c1[p][i]*db1[k][i] + (k+1)/((m+1)(m+2)) * sum_i s[p][i]*db2[k][i]
+ * with s = dphi/dx and c1 = (phi - s*x)/(m+1) of the piecewise-linear phi.
+ * db1_dev/db2_dev are (m+1, n-1) tables of regularised incomplete-beta
+ * differences; work_dev needs 2*P*(n-1) doubles.  Calling it D times
+ * (last axis first) yields the (m0+1, ..., m_{D-1}+1) spectrum.
+ */
+int dh_sample_axis(const double *in_dev, long long P, int n, const double *xx_dev,
+                   int m, const double *db1_dev, const double *db2_dev,
+                   double *work_dev, double *out_dev, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,155 @@
+"""
+CPU checks that pin the oracle (oracle/dadi_oracle.{c,py}) before it is used
+to judge the GPU path:
+  * bit-for-bit against the reference's own C kernels, through the committed
+    golden vectors and, when oracle/_ref is built, live on random inputs;
+  * against mathematics the implementation cannot fake: dense linear solves,
+    mass conservation of a sweep, closed-form sampling integrals;
+  * against the reference's stored spectrum tests/test_2D_frozen1.fs.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from tests import golden_cases as G
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def test_tridiag_matches_dense_solve(oracle):
+    """Thomas solve against numpy's dense solve (pattern of
+    tests/test_Tridiagonal.py:1-36 in the reference)."""
+    rng = np.random.default_rng(5)
+    for n in (3, 10, 100, 600):
+        a = rng.uniform(-1, 0, n)
+        c = rng.uniform(-1, 0, n)
+        b = 2.5 + rng.uniform(0, 1, n)
+        a[0] = 0
+        c[-1] = 0
+        r = rng.normal(size=n)
+        M = np.diag(b) + np.diag(a[1:], -1) + np.diag(c[:-1], 1)
+        u = np.empty(n)
+        gam = np.empty(n)
+        P = oracle._ptr
+        oracle.lib().orc_tridiag(P(a), P(b), P(c), P(r), P(u), n, P(gam))
+        np.testing.assert_allclose(u, np.linalg.solve(M, r), rtol=1e-12, atol=1e-13)
+
+
+@pytest.mark.parametrize("case", range(len(G.meta())))
+def test_oracle_matches_reference_golden(oracle, case):
+    """The generic C restatement reproduces the reference kernels' stored
+    outputs bit for bit, for both coefficient orders."""
+    c, phi0, want = G.case(case)
+    xx = G.grid_of(c)
+    got_tv = G.run_oracle(oracle, c, phi0, xx, "tv")
+    assert np.array_equal(got_tv, want["tv"])
+    if "const" in want:
+        got_c = G.run_oracle(oracle, c, phi0, xx, "const")
+        assert np.array_equal(got_c, want["const"])
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(HERE), "oracle", "_ref",
+                                                    "libdadi_ref.so")),
+                    reason="oracle/_ref not built (reference sources absent)")
+def test_oracle_sweeps_match_reference_live(oracle):
+    rng = np.random.default_rng(11)
+    for D, n in [(1, 30), (2, 15), (3, 9), (4, 6), (5, 5)]:
+        for grid in ("default", "lin"):
+            xx = oracle.default_grid(n) if grid == "default" else np.linspace(0, 1, n)
+            for axis in range(D):
+                phi = rng.uniform(size=(n,) * D)
+                nu = rng.uniform(0.1, 3)
+                mig = rng.uniform(0, 3, D - 1)
+                g = rng.uniform(-50, 50)
+                h = rng.uniform(0, 1)
+                beta = rng.uniform(0.3, 3) if D == 1 else 1
+                dt = 10 ** rng.uniform(-5, -1)
+                p1, p2 = phi.copy(), phi.copy()
+                oracle.sweep(oracle.ORC_TV, p1, axis, xx, nu, mig, g, h, beta, dt)
+                oracle.ref_sweep_tv(p2, axis, xx, nu, mig, g, h, beta, dt)
+                assert np.array_equal(p1, p2)
+                if D <= 3:
+                    p1, p2 = phi.copy(), phi.copy()
+                    order = oracle.ORC_PRE1 if D == 1 else oracle.ORC_PRE
+                    oracle.sweep(order, p1, axis, xx, nu, mig, g, h, beta, dt)
+                    arr = oracle.precalc_arrays(D, xx, axis, nu, mig, g, h, beta)
+                    oracle.ref_sweep_pre(p2, axis, arr, dt)
+                    assert np.array_equal(p1, p2)
+
+
+def test_sweep_conserves_trapezoid_mass(oracle):
+    """With no migration, selection or boundary flux the implicit step is
+    conservative in the trapezoid measure dfactor is built for: the interior
+    mass changes only through the boundary rows."""
+    n = 50
+    xx = oracle.default_grid(n)
+    rng = np.random.default_rng(3)
+    phi = np.zeros(n)
+    phi[10:40] = rng.uniform(1, 2, 30)   # compactly supported away from the edges
+    w = np.zeros(n)
+    dx = np.diff(xx)
+    w[:-1] += dx / 2
+    w[1:] += dx / 2
+    before = np.dot(w, phi)
+    oracle.sweep(oracle.ORC_TV, phi, 0, xx, 1.0, [], 0.0, 0.5, 1, 1e-4)
+    after = np.dot(w, phi)
+    # only the tiny amount that reaches the absorbing-ish boundaries is lost
+    assert abs(after - before) / before < 1e-6
+
+
+def test_sampler_exact_for_linear_phi(oracle):
+    """For phi = a + b x the analytic sampler is exact:
+    E[k] = C(n,k) int x^k (1-x)^(n-k) (a + b x) dx = a/(n+1) + b (k+1)/((n+1)(n+2))."""
+    xx = oracle.default_grid(37)
+    a, b = 0.7, -0.3
+    for n in (5, 20, 40):
+        fs = oracle.from_phi(a + b * xx, [n], xx)
+        k = np.arange(n + 1)
+        want = a / (n + 1) + b * (k + 1) / ((n + 1) * (n + 2))
+        np.testing.assert_allclose(fs, want, rtol=1e-12, atol=1e-15)
+    # 2D: separable product phi = (a + b x)(c + d y)
+    c_, d_ = 1.1, 0.4
+    phi2 = (a + b * xx)[:, None] * (c_ + d_ * xx)[None, :]
+    fs2 = oracle.from_phi(phi2, [6, 9], xx)
+    e1 = a / 7 + b * (np.arange(7) + 1) / (7 * 8)
+    e2 = c_ / 10 + d_ * (np.arange(10) + 1) / (10 * 11)
+    np.testing.assert_allclose(fs2, np.outer(e1, e2), rtol=1e-12, atol=1e-15)
+
+
+def test_snm_spectrum_is_theta_over_i(oracle):
+    """Standard neutral model: the extrapolated spectrum is 1/i
+    (reference tests/test_Results.py:5-12, atol 1e-3 there)."""
+    n = 20
+    res = []
+    xs = []
+    for pts in (100, 120, 140):
+        xx = oracle.default_grid(pts)
+        res.append(oracle.from_phi(oracle.phi_1D(xx), [n], xx))
+        xs.append(xx[1])
+    ex = oracle.extrapolate(res, xs)
+    i = np.arange(1, n)
+    np.testing.assert_allclose(ex[1:-1], 1.0 / i, atol=1e-3)
+
+
+def _read_fs(path):
+    lines = [l for l in open(path).read().split("\n") if l and not l.startswith("#")]
+    shape = tuple(int(v) for v in lines[0].split()[:-1])
+    data = np.array([float(v) for v in lines[1].split()]).reshape(shape)
+    mask = np.array([int(v) for v in lines[2].split()]).reshape(shape).astype(bool)
+    return data, mask
+
+
+def test_frozen1_reference_golden(oracle):
+    """tests/test_2D_frozen1.fs of the reference (tests/test_Freezing.py:142-153):
+    2D frozen population 1, pts 30, n (10,10).  The reference checks rtol 1e-2;
+    the oracle reproduces the stored digits to ~1e-7."""
+    data, mask = _read_fs(os.path.join(HERE, "golden", "test_2D_frozen1.fs"))
+    xx = oracle.default_grid(30)
+    phi = oracle.phi_1D_to_2D(xx, oracle.phi_1D(xx))
+    phi = oracle.integrate_const(phi, xx, 0.1, [1, 0.3], [[0], [0]], [0, 0.5], [0.5, 0.5],
+                                 frozen=[True, False])
+    fs = oracle.from_phi(phi, [10, 10], xx)
+    rel = np.abs(fs - data)[~mask] / np.abs(data[~mask])
+    assert rel.max() < 1e-6
