@@ -1,0 +1,215 @@
+"""
+Benchmark: 2-population split_mig integrations per second at pts=100, FP64,
+batched on one MI355X, beside the plain one-integration-per-call GPU path and
+the reference C path on the host.
+
+Workload (seeded, numpy.random.default_rng): nu1, nu2 ~ logU[0.1, 10],
+T ~ U[0.01, 2], m ~ U[0, 5]; phi0 = phi_1D_to_2D(phi_1D) on default_grid(100);
+one unit = one two_pops(phi0, xx, T, nu1, nu2, m12=m, m21=m) integration (the
+constant-parameter path, bit-identical to dadi's CPU result).
+
+A step = one persistent launch integrating --batch instances (resident in
+HBM, reset from phi0 by a device copy before each launch).  value =
+batch * steps / wall time of the timed steps.
+
+Prints one JSON line.  --gpus > 1 is refused: this build uses one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM = 8.0e12
+
+
+def draws(rng, k):
+    nu1 = np.exp(rng.uniform(np.log(0.1), np.log(10), k))
+    nu2 = np.exp(rng.uniform(np.log(0.1), np.log(10), k))
+    T = rng.uniform(0.01, 2, k)
+    m = rng.uniform(0, 5, k)
+    return list(zip(nu1, nu2, T, m))
+
+
+def build_jobs(dadi_amd, params, pts):
+    from dadi_amd import schedule as S, _lib
+    xx = dadi_amd.Numerics.default_grid(pts)
+    phi0 = dadi_amd.PhiManip.phi_1D_to_2D(xx, dadi_amd.PhiManip.phi_1D(xx))
+    segs = []
+    for nu1, nu2, T, m in params:
+        dt = min(S.compute_dt(nu1, [m], 0, 0.5), S.compute_dt(nu2, [m], 0, 0.5))
+        segs.append(S.const_segments(2, xx, 0.0, T, dt, [nu1, nu2], [[m], [m]], [0, 0],
+                                     [0.5, 0.5], 1.0))
+    return xx, phi0, segs
+
+
+def cpu_baseline(params, pts, budget_s):
+    """The reference's own C kernels (oracle/_ref, compiled from dadi's
+    sources) driven by a restatement of dadi's constant-parameter Python loop,
+    one core; falls back to the oracle's C restatement when _ref is absent."""
+    from oracle import dadi_oracle as O
+    kind = "reference" if O.ref_available() else "port"
+    backend = "ref" if kind == "reference" else "oracle"
+    xx = O.default_grid(pts)
+    phi0 = O.phi_1D_to_2D(xx, O.phi_1D(xx))
+    done = 0
+    steps = 0
+    t0 = time.perf_counter()
+    for nu1, nu2, T, m in params:
+        O.integrate_const(phi0, xx, T, [nu1, nu2], [[m], [m]], [0, 0], [0.5, 0.5],
+                          backend=backend)
+        dt = min(O.compute_dt(nu1, [m], 0, 0.5), O.compute_dt(nu2, [m], 0, 0.5))
+        steps += int(np.ceil(T / dt))
+        done += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    el = time.perf_counter() - t0
+    return {"value": done / el, "unit": "integrations/s", "cores": 1, "kind": kind,
+            "sample": "first %d draws of the same seeded split_mig distribution at pts=%d "
+                      "(%d time steps), full two_pops integrations, single thread, %.1f s"
+                      % (done, pts, steps, el)}
+
+
+def measured_traffic(pts, batch, seed):
+    """HBM bytes per launch of this exact workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (each counter scaled by what it
+    reads for a device copy of known size); None when no summary matches."""
+    path = os.path.join(ROOT, "profiles", "traffic_k_integrate_nd_2d.json")
+    try:
+        rec = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    if rec.get("pts") != pts or rec.get("batch") != batch or rec.get("seed") != seed:
+        return None
+    return (rec["fetch_kib"] * rec["fetch_scale"] + rec["write_kib"] * rec["write_scale"]) * 1024
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=8192)
+    ap.add_argument("--pts", type=int, default=100)
+    ap.add_argument("--plain", type=int, default=12)
+    ap.add_argument("--waves", type=int, default=0)
+    ap.add_argument("--chunk", type=int, default=-1)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=20250101)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: every integration is independent and this "
+              "build runs on a single MI355X; run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import dadi_amd
+    from dadi_amd import engine, _lib
+    if not dadi_amd.cuda_enabled(True):
+        raise SystemExit("HIP backend unavailable")
+    ctx = engine.context()
+    rng = np.random.default_rng(args.seed)
+    params = draws(rng, args.batch)
+    xx, phi0, segs = build_jobs(dadi_amd, params, args.pts)
+    n = args.pts
+    phi0d = ctx.to_dev(phi0)
+    arena = torch.empty((args.batch, n, n), dtype=torch.float64, device=ctx.device)
+    arena.copy_(phi0d.expand(args.batch, n, n))
+    jobs = [engine.Job(arena[i], xx, segs[i], _lib.ORDER_PRE, 0, 3) for i in range(args.batch)]
+    order = sorted(range(args.batch), key=lambda i: -jobs[i].work())
+    pb = engine.PackedBatch(ctx, [jobs[i] for i in order])
+    steps_total = sum(int(np.sum(s["nsteps"])) for s in segs)
+    alg_bytes = 16.0 * n * n * 2 * steps_total   # read+write phi per point per axis sweep
+
+    chunk = None if args.chunk < 0 else args.chunk
+    waves, chunk = ctx.shape(2, n, args.waves or None, chunk)
+
+    def one_step():
+        arena.copy_(phi0d.expand(args.batch, n, n))
+        pb.restart()
+        pb.launch(waves=waves, chunk=chunk)
+
+    for _ in range(args.warmup):
+        one_step()
+    torch.cuda.synchronize()
+    # correctness spot check of the batch against the plain path (bit-exact)
+    spot = order[0]
+    solo = engine.run_single(phi0.copy(), xx, segs[spot], _lib.ORDER_PRE, 0, 3)
+    one_step()
+    torch.cuda.synchronize()
+    assert np.array_equal(arena[spot].cpu().numpy(), solo), "batched != plain result"
+
+    kern_ms = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        arena.copy_(phi0d.expand(args.batch, n, n))
+        pb.restart()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        pb.launch(waves=waves, chunk=chunk)
+        e1.record()
+        kern_ms.append((e0, e1))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern = [a.elapsed_time(b) for a, b in kern_ms]
+    kavg = float(np.mean(kern)) / 1e3
+    value = args.batch * args.steps / wall
+
+    # plain path: one integration per call (same draws), host copies included
+    lat = []
+    for i in range(args.plain):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        engine.run_single(phi0.copy(), xx, segs[i], _lib.ORDER_PRE, 0, 3)
+        lat.append(time.perf_counter() - t1)
+    plain_rate = len(lat) / sum(lat)
+    plain_steps = sum(int(np.sum(segs[i]["nsteps"])) for i in range(args.plain))
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(params, n, args.cpu_seconds)
+
+    achieved = alg_bytes / kavg
+    traffic = measured_traffic(n, args.batch, args.seed)
+    out = {
+        "metric": "2D split_mig integrations/s at pts=100 FP64, batched vs plain; p99 req latency",
+        "value": value,
+        "unit": "integrations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded parameter draws; phi from phi_1D_to_2D(phi_1D))",
+        "config": {"workload": "2D split_mig two_pops integrations, pts=%d, batch of %d seeded "
+                               "draws per persistent launch" % (n, args.batch),
+                   "pts": n, "batch": args.batch, "time_steps_per_batch": steps_total,
+                   "parallelism": "single GPU, persistent workgroup per integration",
+                   "waves_per_workgroup": waves, "thomas_chunk": chunk},
+        "plain": {"value": plain_rate, "unit": "integrations/s", "calls": len(lat),
+                  "time_steps": plain_steps,
+                  "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
+                                 "p99": float(np.percentile(lat, 99) * 1e3)}},
+        "batched_over_plain": value / plain_rate,
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
+                     "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": traffic,
+                     "kernel": "k_integrate_nd<2,1>", "kernel_ms_avg": kavg * 1e3,
+                     "algorithmic_bytes_per_launch": alg_bytes},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

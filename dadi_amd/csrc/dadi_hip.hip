@@ -86,6 +86,150 @@ __device__ __forceinline__ double d_M(const double *c, const double *m, double x
     }
 }
 
+// Rows fetched ahead of use in the Thomas sweeps.  The store of row j and the
+// load of row j+PF go to different addresses, so issuing the loads a block
+// early hides their latency without changing any result.
+constexpr int PF = 4;
+// Columns per staging chunk of the contiguous-axis tiles (16 doubles = 128 B).
+constexpr int STAGE_C = 16;
+
+// Coefficients a_j, b_j and c_{j-1} of row j >= 1 (compute_abc_nobc,
+// dadi/integration_shared.c:72-93, with the boundary term of the last row);
+// pa/pc carry atemp/ctemp of interface j-1 and are advanced to interface j.
+template <int NO, int ORDER>
+__device__ __forceinline__ void row_coeffs(int j, int n, const double *__restrict__ xInt,
+                                           const double *__restrict__ df,
+                                           const double *__restrict__ W0,
+                                           const double *__restrict__ W1,
+                                           const double *__restrict__ S, long long ts,
+                                           const double *c, const double *m, double inv_dt,
+                                           bool bcl_on, double bcl, double &pa, double &pc,
+                                           double &a, double &b, double &cm)
+{
+    const double dfj = df[j * ts];
+    a = -dfj * pa;
+    cm = -df[(j - 1) * ts] * pc;
+    double na = 0., nc = 0.;
+    if (j < n - 1) {
+        const double Mj = d_M<NO>(c, m, xInt[j * ts], S[j * ts]);
+        na = Mj * 0.5 + W0[j * ts];
+        nc = -Mj * (1 - 0.5) + W1[j * ts];
+        if constexpr (ORDER == DH_ORDER_TV)
+            b = (inv_dt + dfj * pc) + dfj * na;
+        else
+            b = (dfj * na + dfj * pc) + inv_dt;
+    } else {
+        if constexpr (ORDER == DH_ORDER_TV) {
+            b = inv_dt + dfj * pc;
+            if (bcl_on) b += bcl;
+        } else {
+            b = dfj * pc;
+            if (bcl_on) b += bcl;
+            b = b + inv_dt;
+        }
+    }
+    pa = na;
+    pc = nc;
+}
+
+// Row 0: coefficients, boundary term and the first Thomas step.
+template <int NO, int ORDER>
+__device__ __forceinline__ void row0(double p0, const double *__restrict__ xInt,
+                                     const double *__restrict__ df,
+                                     const double *__restrict__ W0,
+                                     const double *__restrict__ W1,
+                                     const double *__restrict__ S, const double *c,
+                                     const double *m, double dt, double inv_dt, bool bcf_on,
+                                     double bcf, double &bet, double &u, double &pa, double &pc)
+{
+    const double M = d_M<NO>(c, m, xInt[0], S[0]);
+    pa = M * 0.5 + W0[0];
+    pc = -M * (1 - 0.5) + W1[0];
+    double b;
+    if constexpr (ORDER == DH_ORDER_TV) {
+        b = inv_dt + df[0] * pa;
+        if (bcf_on) b += bcf;
+    } else {
+        b = df[0] * pa;
+        if (bcf_on) b += bcf;
+        b = b + inv_dt;
+    }
+    const double r = (ORDER == DH_ORDER_PRE) ? inv_dt * p0 : p0 / dt;
+    bet = b;
+    u = r / bet;
+}
+
+// One forward-elimination row j >= 1 (dadi/tridiag.c:25-29); pj is phi at
+// row j.  Returns gamma_j; updates bet, u (= u'_j) and the coefficient carry.
+template <int NO, int ORDER>
+__device__ __forceinline__ double fwd_core(int j, int n, double pj,
+                                           const double *__restrict__ xInt,
+                                           const double *__restrict__ df,
+                                           const double *__restrict__ W0,
+                                           const double *__restrict__ W1,
+                                           const double *__restrict__ S, long long ts,
+                                           const double *c, const double *m, double dt,
+                                           double inv_dt, bool bcl_on, double bcl,
+                                           double &bet, double &u, double &pa, double &pc)
+{
+    double a, b, cm;
+    row_coeffs<NO, ORDER>(j, n, xInt, df, W0, W1, S, ts, c, m, inv_dt, bcl_on, bcl, pa, pc, a,
+                          b, cm);
+    const double g = cm / bet;
+    bet = b - a * g;
+    const double r = (ORDER == DH_ORDER_PRE) ? inv_dt * pj : pj / dt;
+    u = (r - a * u) / bet;
+    return g;
+}
+
+template <int NO, int ORDER>
+__device__ __forceinline__ void fwd_row(int j, int n, double pj,
+                                        const double *__restrict__ xInt,
+                                        const double *__restrict__ df,
+                                        const double *__restrict__ W0,
+                                        const double *__restrict__ W1,
+                                        const double *__restrict__ S, long long ts,
+                                        const double *c, const double *m, double dt,
+                                        double inv_dt, bool bcl_on, double bcl,
+                                        double &bet, double &u, double &pa, double &pc,
+                                        double *__restrict__ p, long long s,
+                                        double *__restrict__ gam, long long gs)
+{
+    const double g = fwd_core<NO, ORDER>(j, n, pj, xInt, df, W0, W1, S, ts, c, m, dt, inv_dt,
+                                         bcl_on, bcl, bet, u, pa, pc);
+    gam[j * gs] = g;
+    p[j * s] = u;
+}
+
+// Gammas of indices cb+1 .. cb+CK recomputed from the checkpoint bet_cb with
+// the forward sweep's own operations (bit-identical values).
+template <int NO, int ORDER, int CK>
+__device__ __forceinline__ void regamma(int cb, int n, double bt,
+                                        const double *__restrict__ xInt,
+                                        const double *__restrict__ df,
+                                        const double *__restrict__ W0,
+                                        const double *__restrict__ W1,
+                                        const double *__restrict__ S, long long ts,
+                                        const double *c, const double *m, double inv_dt,
+                                        bool bcl_on, double bcl, double (&g)[CK])
+{
+    const double Mc = d_M<NO>(c, m, xInt[cb * ts], S[cb * ts]);
+    double qa = Mc * 0.5 + W0[cb * ts];
+    double qc = -Mc * (1 - 0.5) + W1[cb * ts];
+#pragma unroll
+    for (int q = 0; q < CK; ++q) {
+        const int j = cb + 1 + q;
+        g[q] = 0.;
+        if (j <= n - 1) {
+            double a, b, cm;
+            row_coeffs<NO, ORDER>(j, n, xInt, df, W0, W1, S, ts, c, m, inv_dt, bcl_on, bcl, qa,
+                                  qc, a, b, cm);
+            g[q] = cm / bt;
+            bt = b - a * g[q];
+        }
+    }
+}
+
 // One line: inject (optional), build a/b/c on the fly, Thomas forward with u'
 // over phi in place and gamma_j in `gam`, then back substitution.
 //   ts: stride of the 1D tables; gs: stride of the gamma column.
@@ -103,64 +247,273 @@ __device__ __forceinline__ void solve_line(
 {
     if (inj_row >= 0) p[inj_row * s] = p[inj_row * s] + inj;
 
-    // row 0 (compute_abc_nobc, dadi/integration_shared.c:72-93)
-    double M = d_M<NO>(c, m, xInt[0], S[0]);
-    double at = M * 0.5 + W0[0];
-    double ct = -M * (1 - 0.5) + W1[0];
-    double b;
-    if constexpr (ORDER == DH_ORDER_TV) {
-        b = inv_dt + df[0] * at;
-        if (bcf_on) b += bcf;
-    } else {
-        b = df[0] * at;
-        if (bcf_on) b += bcf;
-        b = b + inv_dt;
+    // rows 1..PF in flight while row 0 is formed
+    double nxt[PF];
+#pragma unroll
+    for (int q = 0; q < PF; ++q) {
+        const int jj = 1 + q;
+        nxt[q] = p[(jj < n ? jj : n - 1) * s];
     }
-    double p0 = p[0];
-    double r = (ORDER == DH_ORDER_PRE) ? inv_dt * p0 : p0 / dt;
-    double bet = b;
-    double u = r / bet;
-    p[0] = u;
-    double pa = at, pc = ct;
 
-    // forward elimination (dadi/tridiag.c:25-29)
-    for (int j = 1; j < n; ++j) {
-        const double a = -df[j * ts] * pa;
-        const double cm = -df[(j - 1) * ts] * pc;
-        double na = 0., nc = 0.;
-        if (j < n - 1) {
-            const double Mj = d_M<NO>(c, m, xInt[j * ts], S[j * ts]);
-            na = Mj * 0.5 + W0[j * ts];
-            nc = -Mj * (1 - 0.5) + W1[j * ts];
-            if constexpr (ORDER == DH_ORDER_TV)
-                b = (inv_dt + df[j * ts] * pc) + df[j * ts] * na;
-            else
-                b = (df[j * ts] * na + df[j * ts] * pc) + inv_dt;
-        } else {
-            if constexpr (ORDER == DH_ORDER_TV) {
-                b = inv_dt + df[j * ts] * pc;
-                if (bcl_on) b += bcl;
-            } else {
-                b = df[j * ts] * pc;
-                if (bcl_on) b += bcl;
-                b = b + inv_dt;
+    // row 0 (compute_abc_nobc, dadi/integration_shared.c:72-93)
+    double bet, u, pa, pc;
+    row0<NO, ORDER>(p[0], xInt, df, W0, W1, S, c, m, dt, inv_dt, bcf_on, bcf, bet, u, pa, pc);
+    p[0] = u;
+
+    // forward elimination, PF rows per block, the next block's rows loading
+    for (int j0 = 1; j0 < n; j0 += PF) {
+        double cur[PF];
+#pragma unroll
+        for (int q = 0; q < PF; ++q) cur[q] = nxt[q];
+#pragma unroll
+        for (int q = 0; q < PF; ++q) {
+            const int jj = j0 + PF + q;
+            nxt[q] = p[(jj < n ? jj : n - 1) * s];
+        }
+#pragma unroll
+        for (int q = 0; q < PF; ++q) {
+            const int j = j0 + q;
+            if (j < n)
+                fwd_row<NO, ORDER>(j, n, cur[q], xInt, df, W0, W1, S, ts, c, m, dt, inv_dt,
+                                   bcl_on, bcl, bet, u, pa, pc, p, s, gam, gs);
+        }
+    }
+
+    // back substitution (dadi/tridiag.c:31-33), rows and gammas loaded a
+    // block ahead
+    double bp[PF], bg[PF];
+#pragma unroll
+    for (int q = 0; q < PF; ++q) {
+        const int jj = n - 2 - q;
+        const int jc = jj >= 0 ? jj : 0;
+        bp[q] = p[jc * s];
+        bg[q] = gam[(jc + 1) * gs];
+    }
+    for (int j0 = n - 2; j0 >= 0; j0 -= PF) {
+        double cp[PF], cg[PF];
+#pragma unroll
+        for (int q = 0; q < PF; ++q) {
+            cp[q] = bp[q];
+            cg[q] = bg[q];
+        }
+#pragma unroll
+        for (int q = 0; q < PF; ++q) {
+            const int jj = j0 - PF - q;
+            const int jc = jj >= 0 ? jj : 0;
+            bp[q] = p[jc * s];
+            bg[q] = gam[(jc + 1) * gs];
+        }
+#pragma unroll
+        for (int q = 0; q < PF; ++q) {
+            const int j = j0 - q;
+            if (j >= 0) {
+                const double v = cp[q] - cg[q] * u;
+                p[j * s] = v;
+                u = v;
             }
         }
-        const double g = cm / bet;
-        bet = b - a * g;
-        const double pj = p[j * s];
-        r = (ORDER == DH_ORDER_PRE) ? inv_dt * pj : pj / dt;
-        u = (r - a * u) / bet;
-        gam[j * gs] = g;
-        p[j * s] = u;
-        pa = na;
-        pc = nc;
     }
-    // back substitution (dadi/tridiag.c:31-33)
-    for (int j = n - 2; j >= 0; --j) {
-        const double v = p[j * s] - gam[(j + 1) * gs] * u;
-        p[j * s] = v;
-        u = v;
+}
+
+// Thomas with checkpoints: the forward sweep keeps only bet at rows
+// 0, CK, 2CK, ... (not the whole gamma column); the back substitution
+// recomputes each chunk's gammas from its checkpoint with the very same
+// operations, holding them in registers.  Same bits as solve_line, ~CK times
+// less LDS per lane, more VALU.
+template <int NO, int ORDER, int CK>
+__device__ __forceinline__ void solve_line_ck(
+    double *__restrict__ p, long long s, int n,
+    const double *__restrict__ xInt, const double *__restrict__ df,
+    const double *__restrict__ W0, const double *__restrict__ W1,
+    const double *__restrict__ S, long long ts,
+    const double *c, const double *m,
+    double dt, double inv_dt,
+    bool bcf_on, double bcf, bool bcl_on, double bcl,
+    int inj_row, double inj,
+    double *__restrict__ chk, long long gs)
+{
+    if (inj_row >= 0) p[inj_row * s] = p[inj_row * s] + inj;
+    double nxt[PF];
+#pragma unroll
+    for (int q = 0; q < PF; ++q) {
+        const int jj = 1 + q;
+        nxt[q] = p[(jj < n ? jj : n - 1) * s];
+    }
+    double bet, u, pa, pc;
+    row0<NO, ORDER>(p[0], xInt, df, W0, W1, S, c, m, dt, inv_dt, bcf_on, bcf, bet, u, pa, pc);
+    p[0] = u;
+    chk[0] = bet;
+    for (int j0 = 1; j0 < n; j0 += PF) {
+        double cur[PF];
+#pragma unroll
+        for (int q = 0; q < PF; ++q) cur[q] = nxt[q];
+#pragma unroll
+        for (int q = 0; q < PF; ++q) {
+            const int jj = j0 + PF + q;
+            nxt[q] = p[(jj < n ? jj : n - 1) * s];
+        }
+#pragma unroll
+        for (int q = 0; q < PF; ++q) {
+            const int j = j0 + q;
+            if (j < n) {
+                fwd_core<NO, ORDER>(j, n, cur[q], xInt, df, W0, W1, S, ts, c, m, dt, inv_dt,
+                                    bcl_on, bcl, bet, u, pa, pc);
+                p[j * s] = u;
+                if (j % CK == 0) chk[(j / CK) * gs] = bet;
+            }
+        }
+    }
+    // back substitution, one chunk of gamma indices [cb+1, cb+CK] at a time
+    for (int cb = ((n - 2) / CK) * CK; cb >= 0; cb -= CK) {
+        double row[CK];
+#pragma unroll
+        for (int q = 0; q < CK; ++q) {
+            const int jr = cb + q;
+            row[q] = p[(jr <= n - 2 ? jr : n - 2) * s];
+        }
+        double g[CK];
+        regamma<NO, ORDER, CK>(cb, n, chk[(cb / CK) * gs], xInt, df, W0, W1, S, ts, c, m,
+                               inv_dt, bcl_on, bcl, g);
+#pragma unroll
+        for (int q = CK - 1; q >= 0; --q) {
+            const int jr = cb + q;
+            if (jr <= n - 2) {
+                const double v = row[q] - g[q] * u;
+                p[jr * s] = v;
+                u = v;
+            }
+        }
+    }
+}
+
+// Orders loads/stores to LDS between lanes of one wave: a compiler barrier
+// (LDS operations of a wave complete in issue order).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// 64 lines of the contiguous axis solved by one wave: the lines are 64
+// consecutive rows of n doubles, moved between HBM and LDS in chunks of C
+// columns with every lane touching consecutive addresses (each row chunk is
+// one 128-byte run), while each lane runs its own Thomas row out of LDS.
+// Checkpointed gammas (CK), u' written back between the sweeps.
+template <int NO, int ORDER, int CK, int C>
+__device__ __forceinline__ void solve_tile_staged(
+    double *__restrict__ tile, int rows_active, int n,
+    const double *__restrict__ xInt, const double *__restrict__ df,
+    const double *__restrict__ W0, const double *__restrict__ W1,
+    const double *__restrict__ S,
+    const double *c, const double *m, double dt, double inv_dt,
+    bool bcf_on, double bcf, bool bcl_on, double bcl, int inj_row, double inj,
+    double *__restrict__ chk, double *__restrict__ buf, int lane)
+{
+    static_assert(C % CK == 0, "staging chunk must hold whole checkpoint chunks");
+    constexpr int LD = C + 1;
+    double *mine = buf + lane * LD;
+    const int nch = (n + C - 1) / C;
+    double bet = 0., u = 0., pa = 0., pc = 0.;
+    for (int k = 0; k < nch; ++k) {
+        const int j0 = k * C;
+        const int cw = n - j0 < C ? n - j0 : C;
+        {
+            double v[C];
+#pragma unroll
+            for (int q = 0; q < C; ++q) {
+                const int e = q * 64 + lane, row = e / C, col = e - row * C;
+                const bool ok = row < rows_active && col < cw;
+                v[q] = tile[ok ? (long long)row * n + j0 + col : 0];
+                v[q] = ok ? v[q] : 0.;
+            }
+#pragma unroll
+            for (int q = 0; q < C; ++q) {
+                const int e = q * 64 + lane, row = e / C, col = e - row * C;
+                buf[row * LD + col] = v[q];
+            }
+        }
+        wave_sync();
+        if (k == 0 && inj_row >= 0) mine[inj_row] = mine[inj_row] + inj;
+        for (int cc = 0; cc < cw; ++cc) {
+            const int j = j0 + cc;
+            const double pj = mine[cc];
+            if (j == 0) {
+                row0<NO, ORDER>(pj, xInt, df, W0, W1, S, c, m, dt, inv_dt, bcf_on, bcf, bet, u,
+                                pa, pc);
+                chk[0] = bet;
+            } else {
+                fwd_core<NO, ORDER>(j, n, pj, xInt, df, W0, W1, S, 1, c, m, dt, inv_dt, bcl_on,
+                                    bcl, bet, u, pa, pc);
+                if (j % CK == 0) chk[(j / CK) * 64] = bet;
+            }
+            mine[cc] = u;
+        }
+        wave_sync();
+        {
+            double v[C];
+#pragma unroll
+            for (int q = 0; q < C; ++q) {
+                const int e = q * 64 + lane, row = e / C, col = e - row * C;
+                v[q] = buf[row * LD + col];
+            }
+#pragma unroll
+            for (int q = 0; q < C; ++q) {
+                const int e = q * 64 + lane, row = e / C, col = e - row * C;
+                if (row < rows_active && col < cw) tile[(long long)row * n + j0 + col] = v[q];
+            }
+        }
+        wave_sync();
+    }
+    for (int k = nch - 1; k >= 0; --k) {
+        const int j0 = k * C;
+        const int cw = n - j0 < C ? n - j0 : C;
+        {
+            double v[C];
+#pragma unroll
+            for (int q = 0; q < C; ++q) {
+                const int e = q * 64 + lane, row = e / C, col = e - row * C;
+                const bool ok = row < rows_active && col < cw;
+                v[q] = tile[ok ? (long long)row * n + j0 + col : 0];
+                v[q] = ok ? v[q] : 0.;
+            }
+#pragma unroll
+            for (int q = 0; q < C; ++q) {
+                const int e = q * 64 + lane, row = e / C, col = e - row * C;
+                buf[row * LD + col] = v[q];
+            }
+        }
+        wave_sync();
+        const int top = (n - 2 < j0 + cw - 1) ? n - 2 : j0 + cw - 1;
+        for (int cb = top >= j0 ? (top / CK) * CK : j0 - CK; cb >= j0; cb -= CK) {
+            double g[CK];
+            regamma<NO, ORDER, CK>(cb, n, chk[(cb / CK) * 64], xInt, df, W0, W1, S, 1, c, m,
+                                   inv_dt, bcl_on, bcl, g);
+#pragma unroll
+            for (int q = CK - 1; q >= 0; --q) {
+                const int jr = cb + q;
+                if (jr <= n - 2) {
+                    const double v = mine[jr - j0] - g[q] * u;
+                    mine[jr - j0] = v;
+                    u = v;
+                }
+            }
+        }
+        wave_sync();
+        {
+            double v[C];
+#pragma unroll
+            for (int q = 0; q < C; ++q) {
+                const int e = q * 64 + lane, row = e / C, col = e - row * C;
+                v[q] = buf[row * LD + col];
+            }
+#pragma unroll
+            for (int q = 0; q < C; ++q) {
+                const int e = q * 64 + lane, row = e / C, col = e - row * C;
+                if (row < rows_active && col < cw) tile[(long long)row * n + j0 + col] = v[q];
+            }
+        }
+        wave_sync();
     }
 }
 
@@ -287,14 +640,15 @@ __device__ __forceinline__ void build_grid_tables(const double *__restrict__ xx,
 }
 
 // All lines of axis `a`, tiles of 64 lines per wave.
-template <int D, int ORDER>
+template <int D, int ORDER, int CK>
 __device__ __forceinline__ void sweep_tiles(double *__restrict__ phi, int n, int a, bool first_axis,
                                             int injm, const dh_segment *sp,
                                             const double *sx, const double *sxi,
                                             const double *sdf, const double *sdx,
                                             const double *W0, const double *W1, const double *S,
                                             double dt, double inv_dt, double *gamcol,
-                                            long long tile0, long long tstep, int lane)
+                                            double *stage, long long tile0, long long tstep,
+                                            int lane)
 {
     constexpr int NO = D - 1;
     long long s = 1, nsys = 1;
@@ -305,6 +659,24 @@ __device__ __forceinline__ void sweep_tiles(double *__restrict__ phi, int n, int
     for (int q = 0; q < NO; ++q) mig[q] = sp->mig[a][q];
     const double nu = sp->nu[a], ga = sp->gamma[a], ha = sp->h[a];
     const double inj = sp->inject;
+    if constexpr (CK != 0 && D >= 2) {
+        if (a == D - 1) {
+            // contiguous axis: whole-wave tiles staged through LDS
+            for (long long t = tile0; t * 64 < nsys; t += tstep) {
+                const long long L0 = t * 64;
+                const int rows = nsys - L0 < 64 ? (int)(nsys - L0) : 64;
+                const long long line = L0 + lane;
+                LineSetup<D, ORDER> L;
+                line_setup<D, ORDER>(L, line < nsys ? line : nsys - 1, n, a, sx, mig, nu, ga, ha,
+                                     sdx[0], sdx[n - 2], first_axis, injm);
+                if (line >= nsys) L.inj_row = -1;
+                solve_tile_staged<NO, ORDER, CK, STAGE_C>(
+                    phi + L0 * n, rows, n, sxi, sdf, W0, W1, S, L.c, mig, dt, inv_dt, L.bcf_on,
+                    L.bcf, L.bcl_on, L.bcl, L.inj_row, inj, gamcol, stage, lane);
+            }
+            return;
+        }
+    }
     for (long long t = tile0; t * 64 < nsys; t += tstep) {
         const long long line = t * 64 + lane;
         if (line < nsys) {
@@ -313,8 +685,14 @@ __device__ __forceinline__ void sweep_tiles(double *__restrict__ phi, int n, int
                                  first_axis, injm);
             const long long hi = line / s, lo = line - hi * s;
             double *p = phi + hi * (long long)n * s + lo;
-            solve_line<NO, ORDER>(p, s, n, sxi, sdf, W0, W1, S, 1, L.c, mig, dt, inv_dt,
-                                  L.bcf_on, L.bcf, L.bcl_on, L.bcl, L.inj_row, inj, gamcol, 64);
+            if constexpr (CK == 0)
+                solve_line<NO, ORDER>(p, s, n, sxi, sdf, W0, W1, S, 1, L.c, mig, dt, inv_dt,
+                                      L.bcf_on, L.bcf, L.bcl_on, L.bcl, L.inj_row, inj, gamcol,
+                                      64);
+            else
+                solve_line_ck<NO, ORDER, CK>(p, s, n, sxi, sdf, W0, W1, S, 1, L.c, mig, dt,
+                                             inv_dt, L.bcf_on, L.bcf, L.bcl_on, L.bcl, L.inj_row,
+                                             inj, gamcol, 64);
         }
     }
 }
@@ -323,7 +701,7 @@ __device__ __forceinline__ void sweep_tiles(double *__restrict__ phi, int n, int
 // Kernel 1: persistent, one workgroup per instance (any D), whole schedule.
 // LDS: x, xInt, dfactor, dx (4n) | per axis W0,W1,S (3Dn) | gamma (waves*64*n)
 // ---------------------------------------------------------------------------
-template <int D, int ORDER>
+template <int D, int ORDER, int CK>
 __global__ void __launch_bounds__(512) k_integrate_nd(dh_instance *__restrict__ tab, int ninst,
                                                       int *__restrict__ counter,
                                                       long long max_steps)
@@ -342,7 +720,10 @@ __global__ void __launch_bounds__(512) k_integrate_nd(dh_instance *__restrict__ 
         const int n = ip->n;
         double *sx = lds, *sxi = sx + n, *sdf = sxi + n, *sdx = sdf + n;
         double *tabs = sdx + n;
-        double *gamcol = tabs + LDS_TABLES_PER_AXIS * D * n + (long long)wave * 64 * n + lane;
+        const int col = CK ? (n - 1) / (CK ? CK : 1) + 1 : n;   // doubles per lane
+        double *gamcol = tabs + LDS_TABLES_PER_AXIS * D * n + (long long)wave * 64 * col + lane;
+        double *stage = tabs + LDS_TABLES_PER_AXIS * D * n + (long long)nw * 64 * col +
+                        (long long)wave * 64 * (STAGE_C + 1);
         build_grid_tables(ip->xx_dev, n, sx, sxi, sdf, sdx, tid, nthr);
         long long seg = ip->cur_seg, step = ip->cur_step;
         const long long nseg = ip->nseg;
@@ -366,8 +747,9 @@ __global__ void __launch_bounds__(512) k_integrate_nd(dh_instance *__restrict__ 
                 for (int a = 0; a < D; ++a) {
                     if ((frozen >> a) & 1) continue;
                     const double *W0 = tabs + LDS_TABLES_PER_AXIS * a * n;
-                    sweep_tiles<D, ORDER>(phi, n, a, first, injm, sp, sx, sxi, sdf, sdx, W0,
-                                          W0 + n, W0 + 2 * n, dt, inv_dt, gamcol, wave, nw, lane);
+                    sweep_tiles<D, ORDER, CK>(phi, n, a, first, injm, sp, sx, sxi, sdf, sdx, W0,
+                                          W0 + n, W0 + 2 * n, dt, inv_dt, gamcol, stage, wave, nw,
+                                          lane);
                     __syncthreads();
                     first = false;
                 }
@@ -492,8 +874,9 @@ __global__ void __launch_bounds__(256) k_sweep_axis(dh_instance inst, int a, int
     build_axis_tables<D>(sp, a, n, sx, sxi, sdx, W0, W1, S, tid, nthr);
     __syncthreads();
     const double dt = sp->dt, inv_dt = 1. / dt;
-    sweep_tiles<D, ORDER>(inst.phi_dev, n, a, first_axis != 0, inst.inject, sp, sx, sxi, sdf,
-                          sdx, W0, W1, S, dt, inv_dt, gamcol, (long long)blockIdx.x * nw + wave,
+    sweep_tiles<D, ORDER, 0>(inst.phi_dev, n, a, first_axis != 0, inst.inject, sp, sx, sxi, sdf,
+                          sdx, W0, W1, S, dt, inv_dt, gamcol, nullptr,
+                          (long long)blockIdx.x * nw + wave,
                           (long long)gridDim.x * nw, lane);
 }
 
@@ -555,12 +938,16 @@ __global__ void k_sample_contract(const double *__restrict__ sT, const double *_
 namespace {
 
 template <int D, int ORDER>
-const void *nd_kernel() { return reinterpret_cast<const void *>(&k_integrate_nd<D, ORDER>); }
+const void *nd_kernel(int chunk)
+{
+    if (chunk == 8) return reinterpret_cast<const void *>(&k_integrate_nd<D, ORDER, 8>);
+    return reinterpret_cast<const void *>(&k_integrate_nd<D, ORDER, 0>);
+}
 
-const void *pick_nd(int D, int order)
+const void *pick_nd(int D, int order, int chunk)
 {
     switch (D * 4 + order) {
-#define DH_CASE(d, o) case d * 4 + o: return nd_kernel<d, o>();
+#define DH_CASE(d, o) case d * 4 + o: return nd_kernel<d, o>(chunk);
         DH_CASE(1, 0) DH_CASE(1, 1) DH_CASE(1, 2)
         DH_CASE(2, 0) DH_CASE(2, 1) DH_CASE(2, 2)
         DH_CASE(3, 0) DH_CASE(3, 1) DH_CASE(3, 2)
@@ -598,10 +985,12 @@ const void *pick_1d(int order)
     return nullptr;
 }
 
-size_t nd_lds(int D, int n, int waves)
+size_t nd_lds(int D, int n, int waves, int chunk)
 {
-    return sizeof(double) * (size_t)n *
-           (LDS_GRID_TABLES + LDS_TABLES_PER_AXIS * (size_t)D + 64 * (size_t)waves);
+    const size_t col = chunk ? (size_t)((n - 1) / chunk + 1) : (size_t)n;
+    const size_t stage = chunk ? 64 * (size_t)(STAGE_C + 1) : 0;
+    return sizeof(double) * ((size_t)n * (LDS_GRID_TABLES + LDS_TABLES_PER_AXIS * (size_t)D) +
+                             (size_t)waves * (64 * col + stage));
 }
 
 size_t sweep_lds(int n, int waves)
@@ -634,7 +1023,7 @@ extern "C" {
 int dh_abi_version(void) { return 1; }
 size_t dh_sizeof_segment(void) { return sizeof(dh_segment); }
 size_t dh_sizeof_instance(void) { return sizeof(dh_instance); }
-size_t dh_lds_bytes(int D, int n, int waves) { return nd_lds(D, n, waves); }
+size_t dh_lds_bytes(int D, int n, int waves, int chunk) { return nd_lds(D, n, waves, chunk); }
 size_t dh_max_lds(void) { return g_max_lds; }
 
 int dh_last_error(char *buf, int len)
@@ -673,8 +1062,10 @@ int dh_integrate_nd(dh_instance *table_dev, int ninst, int D, int order, int nma
     if (nmax < 3) return fail(DH_EINVAL, "n must be >= 3");
     const int waves = cfg->waves;
     if (waves < 1 || waves > 8) return fail(DH_EINVAL, "waves must be 1..8");
-    const void *fn = pick_nd(D, order);
-    const size_t lds = nd_lds(D, nmax, waves);
+    const int chunk = cfg->chunk;
+    if (chunk != 0 && chunk != 8) return fail(DH_EINVAL, "chunk must be 0 or 8");
+    const void *fn = pick_nd(D, order, chunk);
+    const size_t lds = nd_lds(D, nmax, waves, chunk);
     if (int e = set_lds(fn, lds)) return e;
     int groups = cfg->groups;
     if (groups <= 0) {

@@ -68,11 +68,34 @@ class Context:
             self._scratch = t.empty(int(ndoubles), dtype=t.float64, device=self.device)
         return self._scratch
 
-    def max_waves(self, D, n):
+    def max_waves(self, D, n, chunk=0):
         """Largest waves-per-workgroup whose LDS fits."""
-        per = 8 * n
-        free = self.max_lds // per - (4 + 3 * D)
-        return max(0, min(8, free // 64))
+        col = (n - 1) // chunk + 1 if chunk else n
+        fixed = 8 * n * (4 + 3 * D)
+        per_wave = 8 * 64 * (col + (17 if chunk else 0))
+        return max(0, min(8, (self.max_lds - fixed) // per_wave))
+
+    def shape(self, D, n, waves=None, chunk=None):
+        """Launch shape (waves per workgroup, chunk) of the nD kernel for
+        instances of D populations on n points: the autotuned choice when one
+        is recorded, else a rule of thumb."""
+        if (waves is None or chunk is None) and self.tuning is not None:
+            hit = self.tuning.lookup(D, n)
+            if hit is not None:
+                waves = hit[0] if waves is None else waves
+                chunk = hit[1] if chunk is None else chunk
+        if chunk is None:
+            chunk = 8 if (D >= 2 and n >= 40) else 0
+        nsys = n ** (D - 1)
+        wmax = self.max_waves(D, n, chunk)
+        if wmax < 1 and chunk == 0:
+            chunk = 8
+            wmax = self.max_waves(D, n, chunk)
+        if wmax < 1:
+            raise ValueError("n=%d too large for the workgroup kernel at D=%d" % (n, D))
+        if waves is None:
+            waves = max(1, min(wmax, 4, math.ceil(nsys / 64)))
+        return min(waves, wmax), chunk
 
 
 def context(device=0):
@@ -123,19 +146,19 @@ def run_single(phi, xx, seg, order, frozen, inject, ctx=None, mode="auto", waves
     xxd = ctx.grid(np.asarray(xx, dtype=np.float64))
     segd = ctx.to_dev(seg.view(np.uint8))
     nsys = _lines(D, n)
-    wmax = ctx.max_waves(D, n)
     if mode == "auto":
+        wmax = max(ctx.max_waves(D, n, 0), ctx.max_waves(D, n, 8))
         mode = "wg" if (D == 1 or nsys <= 64 * max(wmax, 1)) and wmax >= 1 else "graph"
     stream = ctx.stream()
     if mode == "wg":
-        if wmax < 1:
-            raise ValueError("n=%d too large for the workgroup kernel at D=%d" % (n, D))
-        w = waves or max(1, min(wmax, math.ceil(nsys / 64)))
+        # one instance: the full gamma column (less VALU) when it fits
+        ck = 0 if ctx.max_waves(D, n, 0) >= min(8, math.ceil(nsys / 64)) else 8
+        w = waves or max(1, min(ctx.max_waves(D, n, ck), math.ceil(nsys / 64)))
         rec = _inst_record(phid.data_ptr(), xxd.data_ptr(), segd.data_ptr(), len(seg), D, n,
                            order, frozen, inject)
         tab = ctx.to_dev(rec.view(np.uint8))
         counter = t.zeros(1, dtype=t.int32, device=ctx.device)
-        cfg = DhLaunch(w, 1, 0)
+        cfg = DhLaunch(w, 1, 0, ck, 0)
         check(ctx.lib.dh_integrate_nd(tab.data_ptr(), 1, D, order, n, counter.data_ptr(),
                                       cfg, stream), "dh_integrate_nd")
         keep = (tab, counter, segd)
@@ -210,11 +233,17 @@ class PackedBatch:
                                   base + int(offs[i]) * SEG_DTYPE.itemsize, len(j.seg), j.D, j.n,
                                   j.order, j.frozen, j.inject)[0]
         self.host_tab = tab
-        self.tab = ctx.to_dev(tab.view(np.uint8))
+        self.tab0 = ctx.to_dev(tab.view(np.uint8))
+        self.tab = self.tab0.clone()
         self.counter = t.zeros(1, dtype=t.int32, device=ctx.device)
         self.nmax = max(j.n for j in jobs)
 
-    def launch(self, waves=None, groups=0, max_steps=0, stream=None):
+    def restart(self):
+        """Reset every instance's progress to the start of its schedule
+        (device copy; the caller resets phi)."""
+        self.tab.copy_(self.tab0)
+
+    def launch(self, waves=None, groups=0, max_steps=0, stream=None, chunk=None):
         ctx = self.ctx
         stream = ctx.stream() if stream is None else stream
         if self.D == 1:
@@ -224,19 +253,13 @@ class PackedBatch:
             nl = g * thr
             need = 6 * self.nmax * nl
             scr = ctx.scratch(need)
-            cfg = DhLaunch(w, g, max_steps)
+            cfg = DhLaunch(w, g, max_steps, 0, 0)
             check(ctx.lib.dh_integrate_1d(self.tab.data_ptr(), len(self.jobs), self.order,
                                           self.nmax, self.counter.data_ptr(), scr.data_ptr(),
                                           scr.numel(), cfg, stream), "dh_integrate_1d")
         else:
-            wmax = ctx.max_waves(self.D, self.nmax)
-            if wmax < 1:
-                raise ValueError("n=%d too large for the workgroup kernel at D=%d"
-                                 % (self.nmax, self.D))
-            nsys = _lines(self.D, self.nmax)
-            w = waves or max(1, min(wmax, math.ceil(nsys / 64)))
-            w = min(w, wmax)
-            cfg = DhLaunch(w, groups, max_steps)
+            w, ck = ctx.shape(self.D, self.nmax, waves, chunk)
+            cfg = DhLaunch(w, groups, max_steps, ck, 0)
             check(ctx.lib.dh_integrate_nd(self.tab.data_ptr(), len(self.jobs), self.D,
                                           self.order, self.nmax, self.counter.data_ptr(), cfg,
                                           stream), "dh_integrate_nd")
@@ -246,7 +269,7 @@ class PackedBatch:
         return raw
 
 
-def run_batch(jobs, ctx=None, waves=None, groups=0, sort=True):
+def run_batch(jobs, ctx=None, waves=None, groups=0, sort=True, chunk=None):
     """Run every job to completion; jobs are grouped by (D, order) and each
     group runs as one persistent launch (longest job first)."""
     ctx = ctx or context()
@@ -259,7 +282,7 @@ def run_batch(jobs, ctx=None, waves=None, groups=0, sort=True):
         if sort:
             js = sorted(js, key=lambda j: -j.work())
         pb = PackedBatch(ctx, js)
-        pb.launch(waves=waves, groups=groups)
+        pb.launch(waves=waves, groups=groups, chunk=chunk)
         packs.append(pb)
     torch().cuda.current_stream(ctx.device).synchronize()
     return packs

@@ -82,8 +82,12 @@ typedef struct dh_instance {
 /* Launch shape of the workgroup-per-instance kernel. */
 typedef struct dh_launch {
     int waves;             /* 64-lane waves per workgroup, 1..8 */
-    int groups;            /* persistent workgroups; 0 = one per instance, capped */
+    int groups;            /* persistent workgroups; 0 = as many as fit, capped by ninst */
     long long max_steps;   /* steps per instance in this call; <= 0 = to completion */
+    int chunk;             /* Thomas scratch: 0 = full gamma column per lane in LDS;
+                              8 = bet checkpoint every 8 rows, gammas recomputed in
+                              registers (less LDS, more VALU, same bits) */
+    int reserved;
 } dh_launch;
 
 /* Library / device set-up; must be called once per process before use.
@@ -95,8 +99,8 @@ int dh_last_error(char *buf, int len);
 int dh_abi_version(void);
 size_t dh_sizeof_segment(void);
 size_t dh_sizeof_instance(void);
-/* LDS footprint per workgroup of the nD kernel for (D, n, waves), bytes. */
-size_t dh_lds_bytes(int D, int n, int waves);
+/* LDS footprint per workgroup of the nD kernel for (D, n, waves, chunk), bytes. */
+size_t dh_lds_bytes(int D, int n, int waves, int chunk);
 /* Largest LDS a workgroup may use on the initialised device (0 before dh_init). */
 size_t dh_max_lds(void);
 

@@ -104,3 +104,25 @@ def api_kwargs(c, force_tv):
     if D == 2:
         kw["nomut1"], kw["nomut2"] = c["nomut"]
     return kw
+
+
+def segments(c, xx, which):
+    """Schedule of a golden case built by the package's host code."""
+    from dadi_amd import schedule as S, _lib
+    from dadi_amd.Misc import ensure_1arg_func as f
+    nus, migs, gammas, hs = params(c)
+    D = c["D"]
+    fz, inj = S.masks(D, c["frozen"], c["nomut"])
+    if which == "const":
+        dt = min(S.compute_dt(nus[k], migs[k] if D > 1 else [0], gammas[k], hs[k])
+                 for k in range(D))
+        seg = S.const_segments(D, xx, c["initial_t"], c["T"], dt, nus, migs, gammas, hs,
+                               c["theta0"], c["beta"])
+        order = _lib.ORDER_PRE1 if D == 1 else _lib.ORDER_PRE
+    else:
+        seg = S.tv_segments(D, xx, c["initial_t"], c["T"], [f(v) for v in nus],
+                            [[f(m) for m in ms] for ms in migs], [f(v) for v in gammas],
+                            [f(v) for v in hs], f(c["theta0"]),
+                            f(c["beta"]) if D == 1 else None)
+        order = _lib.ORDER_TV
+    return seg, order, fz, inj

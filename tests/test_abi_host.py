@@ -40,11 +40,12 @@ def test_struct_layout_and_errors_without_gpu():
     assert L.dh_sizeof_instance() == _lib.INST_DTYPE.itemsize
     assert L.dh_abi_version() == 1
     # calls before dh_init fail cleanly with a message (no GPU touched)
-    cfg = _lib.DhLaunch(1, 1, 0)
+    cfg = _lib.DhLaunch(1, 1, 0, 0, 0)
     rc = L.dh_integrate_nd(None, 1, 2, 0, 10, None, cfg, None)
     assert rc == -1
     assert "dh_init" in _lib.last_error()
-    assert L.dh_lds_bytes(2, 100, 1) == 8 * 100 * (4 + 6 + 64)
+    assert L.dh_lds_bytes(2, 100, 1, 0) == 8 * 100 * (4 + 6 + 64)
+    assert L.dh_lds_bytes(2, 100, 2, 8) == 8 * (100 * 10 + 2 * 64 * (13 + 17))
 
 
 def test_const_steps_match_python_loop():

@@ -170,3 +170,26 @@ def test_split_mig_extrapolated_spectrum_and_ll(dadi_amd, oracle):
     ll_gpu = dadi_amd.Inference.ll(1000 * model, data_fs)
     ll_ref = oracle.ll(1000 * want, data)
     assert abs(ll_gpu - ll_ref) <= 1e-9
+
+
+@pytest.mark.parametrize("chunk", [0, 8])
+@pytest.mark.parametrize("waves", [1, 2])
+def test_batched_variants_bit_exact_on_golden(dadi_amd, chunk, waves):
+    """Every golden case in one batch per (D, order), with the full gamma
+    column (chunk 0) and with checkpointed gammas (chunk 8), at 1 and 2 waves
+    per workgroup: identical bits to the reference kernels."""
+    from dadi_amd import engine
+    ctx = engine.context()
+    jobs, want = [], []
+    for i, c in enumerate(G.meta()):
+        if c["D"] < 2:
+            continue
+        _, phi0, w = G.case(i)
+        xx = G.grid_of(c)
+        for which in (["tv", "const"] if "const" in w else ["tv"]):
+            seg, order, fz, inj = G.segments(c, xx, which)
+            jobs.append(engine.Job(ctx.to_dev(phi0), xx, seg, order, fz, inj))
+            want.append(w[which])
+    engine.run_batch(jobs, waves=waves, chunk=chunk)
+    for j, w in zip(jobs, want):
+        assert np.array_equal(j.phi.cpu().numpy(), w)
