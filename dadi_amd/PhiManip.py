@@ -202,3 +202,74 @@ def filter_pops(phi, xx, tokeep):
     for pop_ii in sorted(toremove)[::-1]:
         phi = remove_pop(phi, xx, pop_ii)
     return phi
+
+
+# ---------------------------------------------------------------------------
+# Device-resident splits: the same operations on a float64 torch tensor that
+# stays on the GPU between epochs (used by the serving loop and by model
+# code that passes device tensors).  Each elementwise step is the IEEE
+# operation the numpy version performs, so results are bit-identical.
+# ---------------------------------------------------------------------------
+def _dev_grid(xx, like):
+    import torch
+    return torch.as_tensor(numpy.asarray(xx, dtype=numpy.float64), device=like.device)
+
+
+def phi_1D_to_2D_device(xx, phi_1D):
+    """phi_1D_to_2D for a device tensor (dadi/PhiManip.py:214-234)."""
+    import torch
+    x = _dev_grid(xx, phi_1D)
+    pts = len(xx)
+    out = torch.zeros((pts, pts), dtype=torch.float64, device=phi_1D.device)
+    ii = torch.arange(1, pts - 1, device=phi_1D.device)
+    out[ii, ii] = phi_1D[1:-1] * 2 / (x[2:] - x[:-2])
+    return out
+
+
+def _new_axis_device(phi, ad, zz):
+    """Device twin of _new_axis / _admixture_intermediates."""
+    import torch
+    z = _dev_grid(zz, phi)
+    nz = len(zz)
+    upper = torch.searchsorted(z, ad.contiguous())
+    upper = torch.clamp(upper, max=nz - 1)
+    upper = torch.clamp(upper, min=1)
+    lower = upper - 1
+    upper_z = z[upper]
+    lower_z = z[lower]
+    zero = torch.zeros((), dtype=torch.float64, device=phi.device)
+    delz0 = torch.where(lower == 0, zero, z[lower] - z[(lower - 1) % nz])
+    delz1 = torch.where(upper == 0, zero, z[upper] - z[lower])
+    delz2 = torch.where(upper == nz - 1, zero, z[(upper + 1) % nz] - z[upper])
+    frac_lower = (upper_z - ad) / (upper_z - lower_z)
+    frac_upper = (ad - lower_z) / (upper_z - lower_z)
+    norm = 2 * phi / (frac_lower * delz0 + delz1 + frac_upper * delz2)
+    out = torch.zeros(tuple(phi.shape) + (nz,), dtype=torch.float64, device=phi.device)
+    flat = out.view(-1, nz)
+    rows = torch.arange(flat.shape[0], device=phi.device)
+    flat[rows, lower.reshape(-1)] = (frac_lower * norm).reshape(-1)
+    flat[rows, upper.reshape(-1)] += (frac_upper * norm).reshape(-1)
+    return out
+
+
+def phi_2D_to_3D_admix_device(phi, f1, xx, yy, zz):
+    """phi_2D_to_3D_admix for a device tensor."""
+    x = _dev_grid(xx, phi)
+    y = _dev_grid(yy, phi)
+    ad_z = f1 * x[:, None] + (1 - f1) * y[None, :]
+    return _new_axis_device(phi, ad_z, zz)
+
+
+def phi_2D_to_3D_split_2_device(xx, phi_2D):
+    return phi_2D_to_3D_admix_device(phi_2D, 0, xx, xx, xx)
+
+
+def phi_3D_to_4D_device(phi, f1, f2, xx, yy, zz, aa):
+    """phi_3D_to_4D for a device tensor."""
+    if f1 + f2 > 1:
+        raise ValueError('Admixture proportions (f1=%f, f2 = %f) are non-sensible.' % (f1, f2))
+    x = _dev_grid(xx, phi)
+    y = _dev_grid(yy, phi)
+    z = _dev_grid(zz, phi)
+    ad_w = f1 * x[:, None, None] + f2 * y[None, :, None] + (1 - f1 - f2) * z[None, None, :]
+    return _new_axis_device(phi, ad_w, aa)

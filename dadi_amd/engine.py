@@ -94,7 +94,10 @@ class Context:
         if wmax < 1:
             raise ValueError("n=%d too large for the workgroup kernel at D=%d" % (n, D))
         if waves is None:
-            waves = max(1, min(wmax, 4, math.ceil(nsys / 64)))
+            # large instances get up to 8 waves (lower latency per step);
+            # small ones few, so more instances share a compute unit
+            cap = 8 if nsys >= 1024 else 4
+            waves = max(1, min(wmax, cap, math.ceil(nsys / 64)))
         return min(waves, wmax), chunk
 
 

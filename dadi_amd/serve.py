@@ -1,0 +1,391 @@
+"""
+Continuous-batching request server for model evaluations on one GPU.
+
+A request is one model evaluation at one grid size: a short program of
+operations on a phi that lives on the device from start to finish -- its
+initial density, integration epochs (1-5 populations), population splits
+between epochs, and the final sampling into a spectrum.  The server admits
+requests as they arrive, as long as the HBM footprint of everything in flight
+stays within a budget, and advances all in-flight integration epochs
+together: every quantum is one persistent launch per (populations, order)
+class, each on its own stream so the classes overlap on the device, and each
+instance advances up to ``quantum`` time steps (its progress is kept in the
+instance table between launches).  A request whose epoch finishes runs its
+next operations (device-side splits, sampling) and either starts its next
+epoch in the following quantum or completes.  New arrivals join at the next
+quantum -- they do not wait for a batch to drain.
+
+Every request's arithmetic is independent of what it shares a launch with,
+so a result is bit-identical alone and in any mix.
+"""
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib, engine, Numerics, PhiManip, Integration
+from . import schedule as S
+from ._lib import INST_DTYPE
+
+
+# ---------------------------------------------------------------------------
+# request programs
+# ---------------------------------------------------------------------------
+@dataclass
+class Epoch:
+    D: int
+    seg: np.ndarray
+    order: int
+    frozen: int
+    inject: int
+
+
+@dataclass
+class Program:
+    """init -> [epoch | split]* -> sample; ops are ('epoch', Epoch),
+    ('split', callable(phi_dev) -> phi_dev)."""
+    pts: int
+    xx: np.ndarray
+    phi0: np.ndarray
+    ops: list
+    ns: tuple
+    kind: str = ""
+
+    def max_points(self):
+        D = 1
+        best = self.pts
+        for op, arg in self.ops:
+            if op == "epoch":
+                D = max(D, arg.D)
+        return self.pts ** D
+
+
+def _const_epoch(D, xx, T, nus, migs, gammas=None, hs=None, theta0=1.0, frozen=None):
+    gammas = gammas or [0.0] * D
+    hs = hs or [0.5] * D
+    frozen = frozen or [False] * D
+    dt = min(S.compute_dt(nus[k], migs[k] if D > 1 else [0], gammas[k], hs[k],
+                          Integration.timescale_factor) for k in range(D))
+    seg = S.const_segments(D, xx, 0.0, T, dt, nus, migs, gammas, hs, theta0)
+    fz, inj = S.masks(D, frozen, [False] * D)
+    order = _lib.ORDER_PRE1 if D == 1 else (_lib.ORDER_PRE if D <= 3 else _lib.ORDER_TV)
+    return Epoch(D, seg, order, fz, inj)
+
+
+def prog_split_mig(params, ns, pts):
+    """Demographics2D.split_mig (dadi/Demographics2D.py:133-167)."""
+    nu1, nu2, T, m = params
+    xx = Numerics.default_grid(pts)
+    phi0 = PhiManip.phi_1D(xx)
+    ops = [("split", lambda phi: PhiManip.phi_1D_to_2D_device(xx, phi)),
+           ("epoch", _const_epoch(2, xx, T, [nu1, nu2], [[m], [m]]))]
+    return Program(pts, xx, phi0, ops, tuple(ns), "split_mig")
+
+
+def prog_three_epoch(params, ns, pts):
+    """Demographics1D.three_epoch (dadi/Demographics1D.py:128-161)."""
+    nuB, nuF, TB, TF = params
+    xx = Numerics.default_grid(pts)
+    phi0 = PhiManip.phi_1D(xx)
+    ops = [("epoch", _const_epoch(1, xx, TB, [nuB], [[]])),
+           ("epoch", _const_epoch(1, xx, TF, [nuF], [[]]))]
+    return Program(pts, xx, phi0, ops, tuple(ns), "three_epoch")
+
+
+def prog_three_pop_chain(params, ns, pts):
+    """Demographics3D.three_pop_chain: 1D epoch -> split -> 2D epoch -> split ->
+    3D epoch with symmetric migration."""
+    nuA, TA, nu1, nu2, T2, m2, nu1b, nu2b, nu3, T3, m3 = params
+    xx = Numerics.default_grid(pts)
+    phi0 = PhiManip.phi_1D(xx)
+    ops = [("epoch", _const_epoch(1, xx, TA, [nuA], [[]])),
+           ("split", lambda phi: PhiManip.phi_1D_to_2D_device(xx, phi)),
+           ("epoch", _const_epoch(2, xx, T2, [nu1, nu2], [[m2], [m2]])),
+           ("split", lambda phi: PhiManip.phi_2D_to_3D_split_2_device(xx, phi)),
+           ("epoch", _const_epoch(3, xx, T3, [nu1b, nu2b, nu3], [[m3, m3]] * 3))]
+    return Program(pts, xx, phi0, ops, tuple(ns), "three_pop_chain")
+
+
+def prog_four_pop_chain(params, ns, pts):
+    """Demographics3D.four_pop_chain: three_pop_chain, a split of population 3
+    into 3 and 4, and a 4-population epoch."""
+    prog = prog_three_pop_chain(params[:11], ns, pts)
+    nu1c, nu2c, nu3c, nu4, T4, m4 = params[11:]
+    xx = prog.xx
+    prog.ops = prog.ops + [
+        ("split", lambda phi: PhiManip.phi_3D_to_4D_device(phi, 0, 0, xx, xx, xx, xx)),
+        ("epoch", _const_epoch(4, xx, T4, [nu1c, nu2c, nu3c, nu4], [[m4] * 3] * 4))]
+    prog.kind = "four_pop_chain"
+    return prog
+
+
+# ---------------------------------------------------------------------------
+# server
+# ---------------------------------------------------------------------------
+@dataclass
+class Request:
+    rid: int
+    prog: Program
+    arrival: float = 0.0
+    state: int = 0
+    phi: object = None
+    rec: np.ndarray = None
+    segd: object = None
+    result: np.ndarray = None
+    error: str = None
+    t_admit: float = None
+    t_done: float = None
+    footprint: int = 0
+
+
+class Server:
+    """Continuous batching of model-evaluation requests.
+
+    budget_bytes: HBM the in-flight requests may occupy (phi, sampling work
+    and schedules); requests that do not fit wait in arrival order.
+    quantum: time steps each in-flight integration advances per launch.
+    """
+
+    def __init__(self, budget_bytes=8 << 30, quantum=200, ctx=None):
+        import torch
+        self.torch = torch
+        self.ctx = ctx or engine.context()
+        self.budget = int(budget_bytes)
+        self.quantum = int(quantum)
+        self.streams = {}
+        self.scratch = {}
+        self.used = 0
+        self.launches = 0
+
+    @staticmethod
+    def footprint(prog):
+        """Bytes a request holds while in flight: phi at its largest
+        dimensionality plus the split/sampling temporaries (about two more
+        copies) and its schedules."""
+        seg_bytes = sum(arg.seg.nbytes for op, arg in prog.ops if op == "epoch")
+        return 3 * 8 * prog.max_points() + seg_bytes + 4096
+
+    def _stream(self, key):
+        st = self.streams.get(key)
+        if st is None:
+            st = self.torch.cuda.Stream(device=self.ctx.device)
+            self.streams[key] = st
+        return st
+
+    # -- per-request state machine ------------------------------------------
+    def _advance(self, r):
+        """Run host/device operations until the request reaches an epoch or
+        completes."""
+        t = self.torch
+        while r.state < len(r.prog.ops):
+            op, arg = r.prog.ops[r.state]
+            if op == "split":
+                r.phi = arg(r.phi).contiguous()
+                r.state += 1
+                continue
+            ep = arg
+            if r.rec is None:
+                if len(ep.seg) == 0 or ep.frozen == (1 << ep.D) - 1:
+                    r.state += 1
+                    continue
+                if r.phi.dim() != ep.D:
+                    raise ValueError("epoch dimensionality does not match phi")
+                r.segd = self.ctx.to_dev(ep.seg.view(np.uint8))
+                xxd = self.ctx.grid(r.prog.xx)
+                r.rec = engine._inst_record(r.phi.data_ptr(), xxd.data_ptr(), r.segd.data_ptr(),
+                                            len(ep.seg), ep.D, r.prog.pts, ep.order, ep.frozen,
+                                            ep.inject)
+                return "epoch"
+            if r.rec["status"][0] == 1:
+                r.rec = None
+                r.segd = None
+                r.state += 1
+                continue
+            return "epoch"
+        from .Spectrum_mod import sample_device
+        r.result = sample_device(r.phi, r.prog.ns, r.prog.xx, self.ctx).cpu().numpy()
+        return "done"
+
+    def _launch(self, group):
+        """One quantum for every in-flight epoch of one (D, order) class."""
+        t = self.torch
+        ctx = self.ctx
+        D, order = group[0].rec["D"][0], group[0].rec["order"][0]
+        tab = np.concatenate([r.rec for r in group])
+        key = (int(D), int(order))
+        st = self._stream(key)
+        # splits and uploads of this quantum were issued on the current stream
+        st.wait_stream(t.cuda.current_stream(ctx.device))
+        with t.cuda.stream(st):
+            tabd = t.from_numpy(tab.view(np.uint8).copy()).to(ctx.device, non_blocking=False)
+            counter = t.zeros(1, dtype=t.int32, device=ctx.device)
+            nmax = int(tab["n"].max())
+            h = st.cuda_stream
+            if D == 1:
+                thr = 64
+                g = max(1, math.ceil(len(group) / thr))
+                need = 6 * nmax * g * thr
+                scr = self.scratch.get(key)
+                if scr is None or scr.numel() < need:
+                    scr = t.empty(need, dtype=t.float64, device=ctx.device)
+                    self.scratch[key] = scr
+                cfg = _lib.DhLaunch(1, g, self.quantum, 0, 0)
+                _lib.check(ctx.lib.dh_integrate_1d(tabd.data_ptr(), len(group), int(order), nmax,
+                                                   counter.data_ptr(), scr.data_ptr(), scr.numel(),
+                                                   cfg, h), "dh_integrate_1d")
+            else:
+                w, ck = ctx.shape(int(D), nmax)
+                cfg = _lib.DhLaunch(w, 0, self.quantum, ck, 0)
+                _lib.check(ctx.lib.dh_integrate_nd(tabd.data_ptr(), len(group), int(D), int(order),
+                                                   nmax, counter.data_ptr(), cfg, h),
+                           "dh_integrate_nd")
+        self.launches += 1
+        return st, tabd, counter
+
+    def run(self, requests, realtime=True, clock=time.perf_counter):
+        """Serve ``requests`` (arrival times in seconds from the start).
+        realtime=False admits every request at time 0.  Returns a stats dict;
+        results are left on the requests.
+
+        Each (populations, order) class runs its own sequence of quanta on its
+        own stream: as soon as a class's launch finishes, its requests advance
+        and the class is relaunched with whatever is in flight then, including
+        requests admitted meanwhile -- a slow class never holds back a fast
+        one."""
+        t = self.torch
+        pending = sorted(requests, key=lambda r: r.arrival)
+        for r in pending:
+            r.footprint = self.footprint(r.prog)
+        waiting = {}      # class -> requests ready for their next quantum
+        inflight = {}     # class -> (requests, stream, table, counter, event)
+        done = []
+        t0 = clock()
+
+        def enqueue(r):
+            try:
+                status = self._advance(r)
+            except Exception as e:  # a bad request fails alone
+                r.error = repr(e)
+                status = "done"
+            if status == "done":
+                self._retire(r, done, t0, clock)
+            else:
+                key = (int(r.rec["D"][0]), int(r.rec["order"][0]))
+                waiting.setdefault(key, []).append(r)
+
+        while pending or waiting or inflight:
+            now = clock() - t0
+            while pending and (not realtime or pending[0].arrival <= now):
+                r = pending[0]
+                if r.footprint > self.budget:
+                    pending.pop(0)
+                    r.error = "request needs %d bytes, over the %d-byte budget" % (
+                        r.footprint, self.budget)
+                    r.t_done = clock() - t0
+                    done.append(r)
+                    continue
+                if self.used + r.footprint > self.budget:
+                    break
+                pending.pop(0)
+                self.used += r.footprint
+                r.t_admit = now
+                try:
+                    r.phi = self.ctx.to_dev(np.ascontiguousarray(r.prog.phi0, dtype=np.float64))
+                except Exception as e:
+                    r.error = repr(e)
+                    self._retire(r, done, t0, clock)
+                    continue
+                enqueue(r)
+            for key in sorted(waiting):
+                if key in inflight or not waiting[key]:
+                    continue
+                grp = waiting.pop(key)
+                st, tabd, counter = self._launch(grp)
+                ev = t.cuda.Event()
+                ev.record(st)
+                inflight[key] = (grp, st, tabd, counter, ev)
+            progressed = False
+            for key in list(inflight):
+                grp, st, tabd, counter, ev = inflight[key]
+                if not ev.query():
+                    continue
+                progressed = True
+                del inflight[key]
+                back = tabd.cpu().numpy().view(INST_DTYPE)
+                for r, rec in zip(grp, back):
+                    r.rec["cur_seg"] = rec["cur_seg"]
+                    r.rec["cur_step"] = rec["cur_step"]
+                    r.rec["status"] = rec["status"]
+                for r in grp:
+                    enqueue(r)
+            waiting = {k: v for k, v in waiting.items() if v}
+            if not progressed:
+                if inflight:
+                    time.sleep(0.0002)
+                elif pending and realtime and not waiting:
+                    time.sleep(max(0.0, min(pending[0].arrival - (clock() - t0), 0.002)))
+        wall = clock() - t0
+        lat = np.array([r.t_done - r.arrival for r in done if r.error is None])
+        return {
+            "requests": len(done),
+            "failed": sum(1 for r in done if r.error is not None),
+            "wall_s": wall,
+            "throughput_req_s": len(lat) / wall if wall > 0 else 0.0,
+            "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3) if len(lat) else None,
+                           "p99": float(np.percentile(lat, 99) * 1e3) if len(lat) else None},
+            "launches": self.launches,
+        }
+
+    def _retire(self, r, done, t0, clock):
+        r.t_done = clock() - t0
+        r.phi = None
+        r.rec = None
+        r.segd = None
+        self.used -= r.footprint
+        done.append(r)
+
+
+def poisson_arrivals(rng, n, rate):
+    """n arrival times of a Poisson process of the given rate (per second)."""
+    return np.cumsum(rng.exponential(1.0 / rate, n))
+
+
+def mixed_requests(rng, n, rate, scale=1.0):
+    """The serving mix: 55% 2D split_mig (pts U{40..120}), 30% 1D three_epoch
+    (pts U{100..600}), 12% 3D chain (pts U{30..60}), 3% 4D chain
+    (pts U{16..24}); sample sizes U{10..40} per population; sizes
+    logU[0.1,10], migration U[0,5], times U[0.01,2] (2D) or U[0.01,1] (chains).
+    ``scale`` multiplies every epoch time (a shorter mix for tests)."""
+    arr = poisson_arrivals(rng, n, rate)
+    out = []
+    lu = lambda: float(np.exp(rng.uniform(np.log(0.1), np.log(10))))
+    for i in range(n):
+        u = rng.uniform()
+        if u < 0.55:
+            pts = int(rng.integers(40, 121))
+            ns = (int(rng.integers(10, 41)), int(rng.integers(10, 41)))
+            prog = prog_split_mig((lu(), lu(), scale * rng.uniform(0.01, 2), rng.uniform(0, 5)),
+                                  ns, pts)
+        elif u < 0.85:
+            pts = int(rng.integers(100, 601))
+            ns = (int(rng.integers(10, 41)),)
+            prog = prog_three_epoch((lu(), lu(), scale * rng.uniform(0.01, 2),
+                                     scale * rng.uniform(0.01, 2)), ns, pts)
+        elif u < 0.97:
+            pts = int(rng.integers(30, 61))
+            ns = tuple(int(rng.integers(10, 41)) for _ in range(3))
+            p = (lu(), scale * rng.uniform(0.01, 1), lu(), lu(), scale * rng.uniform(0.01, 1),
+                 rng.uniform(0, 5), lu(), lu(), lu(), scale * rng.uniform(0.01, 1),
+                 rng.uniform(0, 5))
+            prog = prog_three_pop_chain(p, ns, pts)
+        else:
+            pts = int(rng.integers(16, 25))
+            ns = tuple(int(rng.integers(10, 41)) for _ in range(4))
+            p = (lu(), scale * rng.uniform(0.01, 1), lu(), lu(), scale * rng.uniform(0.01, 1),
+                 rng.uniform(0, 5), lu(), lu(), lu(), scale * rng.uniform(0.01, 1),
+                 rng.uniform(0, 5), lu(), lu(), lu(), lu(), scale * rng.uniform(0.01, 1),
+                 rng.uniform(0, 5))
+            prog = prog_four_pop_chain(p, ns, pts)
+        out.append(Request(i, prog, float(arr[i])))
+    return out

@@ -1,0 +1,100 @@
+"""
+Serving loop: continuous batching of mixed-dimensionality model evaluations.
+GPU tests check that every result is bit-identical alone and in the mix (and
+equal to the sequential dadi-style model functions), that admission by HBM
+footprint queues instead of failing, and that a poisoned request fails alone.
+The CPU test checks the host-side request construction.
+"""
+import numpy as np
+import pytest
+
+
+def test_request_mix_host_side():
+    from dadi_amd import serve
+    rng = np.random.default_rng(3)
+    reqs = serve.mixed_requests(rng, 400, rate=100.0, scale=0.01)
+    kinds = [r.prog.kind for r in reqs]
+    frac = {k: kinds.count(k) / len(kinds) for k in set(kinds)}
+    assert abs(frac["split_mig"] - 0.55) < 0.08
+    assert abs(frac["three_epoch"] - 0.30) < 0.08
+    assert frac.get("four_pop_chain", 0) < 0.08
+    arr = np.array([r.arrival for r in reqs])
+    assert np.all(np.diff(arr) >= 0)
+    assert abs(len(arr) / arr[-1] - 100.0) < 20
+    for r in reqs:
+        fp = serve.Server.footprint(r.prog)
+        assert fp >= 3 * 8 * r.prog.pts ** {"split_mig": 2, "three_epoch": 1,
+                                            "three_pop_chain": 3,
+                                            "four_pop_chain": 4}[r.prog.kind]
+
+
+@pytest.fixture(scope="module")
+def dadi_amd():
+    import dadi_amd as d
+    assert d.cuda_enabled(True)
+    return d
+
+
+def _small_mix(seed, n):
+    from dadi_amd import serve
+    rng = np.random.default_rng(seed)
+    reqs = serve.mixed_requests(rng, n, rate=1e6, scale=0.01)
+    return reqs
+
+
+@pytest.mark.gpu
+def test_results_identical_alone_and_in_mix(dadi_amd):
+    from dadi_amd import serve
+    mix = _small_mix(11, 24)
+    srv = serve.Server(quantum=7)
+    stats = srv.run(mix, realtime=False)
+    assert stats["failed"] == 0 and stats["requests"] == 24
+    for r in mix[:8]:
+        alone = serve.Request(0, r.prog, 0.0)
+        serve.Server(quantum=1000).run([alone], realtime=False)
+        assert np.array_equal(alone.result, r.result), r.prog.kind
+
+
+@pytest.mark.gpu
+def test_served_split_mig_equals_model_function(dadi_amd):
+    """A served split_mig request equals Demographics2D.split_mig called
+    directly (same integration bits, same device sampling)."""
+    from dadi_amd import serve
+    params, ns, pts = (2.0, 0.4, 0.03, 1.5), (12, 15), 37
+    req = serve.Request(0, serve.prog_split_mig(params, ns, pts), 0.0)
+    serve.Server().run([req], realtime=False)
+    fs = dadi_amd.Demographics2D.split_mig(params, ns, pts)
+    assert np.array_equal(req.result, fs.data)
+
+
+@pytest.mark.gpu
+def test_budget_queues_and_oversize_fails_alone(dadi_amd):
+    from dadi_amd import serve
+    mix = _small_mix(5, 10)
+    biggest = max(serve.Server.footprint(r.prog) for r in mix)
+    srv = serve.Server(budget_bytes=biggest, quantum=50)   # one at a time at worst
+    stats = srv.run(mix, realtime=False)
+    assert stats["failed"] == 0 and stats["requests"] == 10
+    big = serve.Request(99, serve.prog_split_mig((1, 1, 0.01, 1), (10, 10), 120), 0.0)
+    small = serve.Request(100, serve.prog_split_mig((1, 1, 0.01, 1), (10, 10), 40), 0.0)
+    srv2 = serve.Server(budget_bytes=serve.Server.footprint(small.prog), quantum=50)
+    st2 = srv2.run([big, small], realtime=False)
+    assert big.error is not None and small.error is None
+    assert st2["failed"] == 1
+
+
+@pytest.mark.gpu
+def test_poisoned_request_fails_alone(dadi_amd):
+    from dadi_amd import serve
+    mix = _small_mix(8, 12)
+    bad = serve.Request(50, serve.prog_split_mig((float("nan"), 1.0, 0.02, 1.0), (10, 10), 50),
+                        0.0)
+    solo = []
+    for r in mix[:4]:
+        a = serve.Request(0, r.prog, 0.0)
+        serve.Server().run([a], realtime=False)
+        solo.append(a.result)
+    serve.Server(quantum=9).run(mix + [bad], realtime=False)
+    assert not np.all(np.isfinite(bad.result))
+    for r, s in zip(mix[:4], solo):
+        assert np.array_equal(r.result, s)
