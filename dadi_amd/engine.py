@@ -21,7 +21,9 @@ import numpy as np
 from . import _lib
 from ._lib import INST_DTYPE, SEG_DTYPE, DhInstance, DhLaunch, check
 
-_state = threading.local()
+_ctx_lock = threading.Lock()
+_ctx = {}
+_local = threading.local()
 
 
 def torch():
@@ -41,7 +43,8 @@ class Context:
         self.max_lds = int(self.lib.dh_max_lds())
         self._xx_cache = {}
         self._scratch = None
-        self.tuning = None
+        from .autotune import Tuning
+        self.tuning = Tuning.load()
 
     # -- helpers ---------------------------------------------------------
     def stream(self):
@@ -102,11 +105,13 @@ class Context:
 
 
 def context(device=0):
-    ctx = getattr(_state, "ctx", None)
-    if ctx is None or ctx.device.index != device:
-        ctx = Context(device)
-        _state.ctx = ctx
-    return ctx
+    """The process-wide Context of ``device``."""
+    with _ctx_lock:
+        ctx = _ctx.get(device)
+        if ctx is None:
+            ctx = Context(device)
+            _ctx[device] = ctx
+        return ctx
 
 
 def _lines(D, n):
@@ -138,6 +143,11 @@ def run_single(phi, xx, seg, order, frozen, inject, ctx=None, mode="auto", waves
     ctx = ctx or context()
     host = isinstance(phi, np.ndarray)
     phid = ctx.to_dev(np.ascontiguousarray(phi, dtype=np.float64)) if host else phi
+    co = getattr(_local, "coalescer", None)
+    if co is not None and len(seg) and frozen != (1 << phid.dim()) - 1:
+        job = Job(phid, xx, seg, order, frozen, inject)
+        co.submit(job)
+        return phid.cpu().numpy() if host else phid
     if not (phid.is_cuda and phid.dtype == t.float64 and phid.is_contiguous()):
         raise ValueError("phi must be a contiguous float64 device tensor")
     D = phid.dim()
@@ -189,9 +199,10 @@ def run_single(phi, xx, seg, order, frozen, inject, ctx=None, mode="auto", waves
 
 class Job:
     """One integration of a batch: device phi (updated in place) plus schedule."""
-    __slots__ = ("phi", "xx", "seg", "order", "frozen", "inject", "tag")
+    __slots__ = ("phi", "xx", "seg", "order", "frozen", "inject", "tag", "done")
 
     def __init__(self, phi, xx, seg, order, frozen, inject, tag=None):
+        self.done = False
         self.phi = phi
         self.xx = np.asarray(xx, dtype=np.float64)
         self.seg = seg
@@ -289,3 +300,81 @@ def run_batch(jobs, ctx=None, waves=None, groups=0, sort=True, chunk=None):
         packs.append(pb)
     torch().cuda.current_stream(ctx.device).synchronize()
     return packs
+
+
+class Coalescer:
+    """Gathers the integrations that worker threads issue concurrently and
+    runs them as one batched persistent launch whenever every live worker is
+    waiting for one.  Model functions written against the ordinary
+    Integration API thereby batch across evaluations without changes."""
+
+    def __init__(self, ctx=None):
+        self.ctx = ctx or context()
+        self.cv = threading.Condition()
+        self.waiting = []
+        self.live = 0
+        self.launches = 0
+
+    def register(self):
+        with self.cv:
+            self.live += 1
+        _local.coalescer = self
+
+    def unregister(self):
+        _local.coalescer = None
+        with self.cv:
+            self.live -= 1
+            self._maybe_flush()
+
+    def submit(self, job):
+        with self.cv:
+            self.waiting.append(job)
+            self._maybe_flush()
+            while not job.done:
+                self.cv.wait()
+
+    def _maybe_flush(self):
+        if self.waiting and len(self.waiting) >= self.live:
+            jobs, self.waiting = self.waiting, []
+            try:
+                run_batch(jobs, self.ctx)
+            finally:
+                for j in jobs:
+                    j.done = True
+                self.launches += 1
+                self.cv.notify_all()
+
+
+def map_batched(func, args_list, workers=256, ctx=None):
+    """[func(*a) for a in args_list], evaluated by ``workers`` threads whose
+    integrations are coalesced into batched launches.  Exceptions are
+    returned in place of results (one failing evaluation does not stop the
+    others)."""
+    co = Coalescer(ctx)
+    results = [None] * len(args_list)
+    nxt = [0]
+    lock = threading.Lock()
+
+    def worker():
+        co.register()
+        try:
+            while True:
+                with lock:
+                    i = nxt[0]
+                    nxt[0] += 1
+                if i >= len(args_list):
+                    return
+                try:
+                    results[i] = func(*args_list[i])
+                except Exception as e:  # noqa: BLE001  (reported to the caller)
+                    results[i] = e
+        finally:
+            co.unregister()
+
+    nthreads = max(1, min(workers, len(args_list)))
+    threads = [threading.Thread(target=worker, daemon=True) for _ in range(nthreads)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    return results
