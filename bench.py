@@ -99,10 +99,27 @@ def main():
     ap.add_argument("--plain", type=int, default=12)
     ap.add_argument("--waves", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=-1)
+    ap.add_argument("--recompute", type=int, default=-1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--seed", type=int, default=20250101)
+    ap.add_argument("--calibrate-copy", type=float, default=0.0,
+                    help="only run 3 device copies of this many GiB (rocprofv3 counter "
+                         "calibration of FETCH_SIZE/WRITE_SIZE) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy > 0:
+        import torch
+        n = int(args.calibrate_copy * (1 << 30)) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes_each": n * 8, "copies": 3}))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: every integration is independent and this "
               "build runs on a single MI355X; run with --gpus 1", file=sys.stderr)
@@ -128,22 +145,24 @@ def main():
     alg_bytes = 16.0 * n * n * 2 * steps_total   # read+write phi per point per axis sweep
 
     chunk = None if args.chunk < 0 else args.chunk
-    waves, chunk = ctx.shape(2, n, args.waves or None, chunk)
+    recompute = None if args.recompute < 0 else args.recompute
+    waves, chunk, recompute = ctx.shape(2, n, args.waves or None, chunk, recompute)
 
     def one_step():
         arena.copy_(phi0d.expand(args.batch, n, n))
         pb.restart()
-        pb.launch(waves=waves, chunk=chunk)
+        pb.launch(waves=waves, chunk=chunk, recompute=recompute)
 
     for _ in range(args.warmup):
         one_step()
     torch.cuda.synchronize()
-    # correctness spot check of the batch against the plain path (bit-exact)
-    spot = order[0]
-    solo = engine.run_single(phi0.copy(), xx, segs[spot], _lib.ORDER_PRE, 0, 3)
-    one_step()
-    torch.cuda.synchronize()
-    assert np.array_equal(arena[spot].cpu().numpy(), solo), "batched != plain result"
+    if not args.no_check:
+        # correctness spot check of the batch against the plain path (bit-exact)
+        spot = order[0]
+        solo = engine.run_single(phi0.copy(), xx, segs[spot], _lib.ORDER_PRE, 0, 3)
+        one_step()
+        torch.cuda.synchronize()
+        assert np.array_equal(arena[spot].cpu().numpy(), solo), "batched != plain result"
 
     kern_ms = []
     torch.cuda.synchronize()
@@ -154,7 +173,7 @@ def main():
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record()
-        pb.launch(waves=waves, chunk=chunk)
+        pb.launch(waves=waves, chunk=chunk, recompute=recompute)
         e1.record()
         kern_ms.append((e0, e1))
     torch.cuda.synchronize()
@@ -170,8 +189,18 @@ def main():
         t1 = time.perf_counter()
         engine.run_single(phi0.copy(), xx, segs[i], _lib.ORDER_PRE, 0, 3)
         lat.append(time.perf_counter() - t1)
-    plain_rate = len(lat) / sum(lat)
+    plain_rate = len(lat) / sum(lat) if lat else None
     plain_steps = sum(int(np.sum(segs[i]["nsteps"])) for i in range(args.plain))
+
+    serving = None
+    if args.serve_requests > 0:
+        from dadi_amd import serve
+        reqs = serve.mixed_requests(np.random.default_rng(args.seed + 1), args.serve_requests,
+                                    args.serve_rate)
+        serving = serve.Server().run(reqs)
+        serving.update({"rate_req_s": args.serve_rate, "mix": "55% 2D split_mig pts 40-120, "
+                        "30% 1D three_epoch pts 100-600, 12% 3D chain pts 30-60, "
+                        "3% 4D chain pts 16-24; Poisson arrivals, seeded"})
 
     cpu = None
     if not args.no_cpu:
@@ -196,12 +225,14 @@ def main():
                                "draws per persistent launch" % (n, args.batch),
                    "pts": n, "batch": args.batch, "time_steps_per_batch": steps_total,
                    "parallelism": "single GPU, persistent workgroup per integration",
-                   "waves_per_workgroup": waves, "thomas_chunk": chunk},
+                   "waves_per_workgroup": waves, "thomas_chunk": chunk,
+                   "thomas_recompute": recompute},
         "plain": {"value": plain_rate, "unit": "integrations/s", "calls": len(lat),
                   "time_steps": plain_steps,
-                  "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
-                                 "p99": float(np.percentile(lat, 99) * 1e3)}},
-        "batched_over_plain": value / plain_rate,
+                  "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3) if lat else None,
+                                 "p99": float(np.percentile(lat, 99) * 1e3) if lat else None}},
+        "batched_over_plain": value / plain_rate if plain_rate else None,
+        "serving": serving,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": traffic,
                      "kernel": "k_integrate_nd<2,1>", "kernel_ms_avg": kavg * 1e3,

@@ -53,7 +53,7 @@ class DhInstance(ctypes.Structure):
 class DhLaunch(ctypes.Structure):
     _fields_ = [("waves", ctypes.c_int), ("groups", ctypes.c_int),
                 ("max_steps", ctypes.c_longlong), ("chunk", ctypes.c_int),
-                ("reserved", ctypes.c_int)]
+                ("recompute", ctypes.c_int)]
 
 
 _vp = ctypes.c_void_p
@@ -63,7 +63,7 @@ _SIGS = {
     "dh_abi_version": ([], ctypes.c_int),
     "dh_sizeof_segment": ([], ctypes.c_size_t),
     "dh_sizeof_instance": ([], ctypes.c_size_t),
-    "dh_lds_bytes": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_size_t),
+    "dh_lds_bytes": ([ctypes.c_int] * 5, ctypes.c_size_t),
     "dh_max_lds": ([], ctypes.c_size_t),
     "dh_integrate_nd": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp,
                          ctypes.POINTER(DhLaunch), _vp], ctypes.c_int),

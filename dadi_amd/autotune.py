@@ -9,7 +9,10 @@ supports on a synthetic batch that fills the GPU and records the fastest:
 * the Thomas scratch layout: the whole gamma column of each lane in LDS
   (chunk 0: least VALU, most LDS) or a bet checkpoint every 8 rows with the
   gammas recomputed in registers (chunk 8: ~8x less LDS, more VALU, and the
-  contiguous axis staged through LDS tiles).
+  contiguous axis staged through LDS tiles);
+* with chunk 8, whether u' is written over phi between the two Thomas passes
+  or recomputed from checkpoints (recompute 1: a quarter less phi traffic,
+  half the writes, more VALU).
 
 The axis order of the sweeps is not a tunable: it is the operator splitting
 of the reference (x, then y, ...) and changing it would change results.
@@ -56,10 +59,12 @@ class Tuning:
                 if d == D and abs(m - n) <= 0.25 * n and (best is None or abs(m - n) < best[0]):
                     best = (abs(m - n), v)
             hit = best[1] if best else None
-        return None if hit is None else (int(hit["waves"]), int(hit["chunk"]))
+        return None if hit is None else (int(hit["waves"]), int(hit["chunk"]),
+                                         int(hit.get("recompute", 0)))
 
-    def record(self, D, n, waves, chunk, rate, candidates):
+    def record(self, D, n, waves, chunk, recompute, rate, candidates):
         self.table["%d:%d" % (D, n)] = {"waves": int(waves), "chunk": int(chunk),
+                                        "recompute": int(recompute),
                                         "points_per_s": float(rate),
                                         "candidates": candidates}
 
@@ -90,11 +95,11 @@ def _synthetic_jobs(ctx, D, n, count, steps, rng):
 def candidates(ctx, D, n):
     nsys = n ** (D - 1)
     out = []
-    for chunk in (0, 8):
-        wmax = ctx.max_waves(D, n, chunk)
+    for chunk, ru in ((0, 0), (8, 0), (8, 1)):
+        wmax = ctx.max_waves(D, n, chunk, ru)
         for w in (1, 2, 4, 8):
             if w <= wmax and (w == 1 or 64 * (w // 2) < nsys):
-                out.append((w, chunk))
+                out.append((w, chunk, ru))
     return out
 
 
@@ -109,20 +114,21 @@ def tune(D, n, ctx=None, steps=30, count=None, reps=2, tuning=None, save=False):
     pb = engine.PackedBatch(ctx, jobs)
     work = sum(int(np.sum(j.seg["nsteps"])) for j in jobs) * (n ** D) * D
     results = []
-    for w, ck in candidates(ctx, D, n):
+    for w, ck, ru in candidates(ctx, D, n):
         best = None
         for _ in range(reps + 1):
             pb.restart()
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            pb.launch(waves=w, chunk=ck)
+            pb.launch(waves=w, chunk=ck, recompute=ru)
             torch.cuda.synchronize()
             el = time.perf_counter() - t0
             best = el if best is None else min(best, el)   # first run warms up
-        results.append({"waves": w, "chunk": ck, "points_per_s": work / best})
+        results.append({"waves": w, "chunk": ck, "recompute": ru, "points_per_s": work / best})
     top = max(results, key=lambda r: r["points_per_s"])
-    tuning.record(D, n, top["waves"], top["chunk"], top["points_per_s"], results)
+    tuning.record(D, n, top["waves"], top["chunk"], top["recompute"], top["points_per_s"],
+                  results)
     if save:
         tuning.save()
     ctx.tuning = tuning
-    return top["waves"], top["chunk"]
+    return top["waves"], top["chunk"], top["recompute"]

@@ -90,8 +90,9 @@ __device__ __forceinline__ double d_M(const double *c, const double *m, double x
 // load of row j+PF go to different addresses, so issuing the loads a block
 // early hides their latency without changing any result.
 constexpr int PF = 4;
-// Columns per staging chunk of the contiguous-axis tiles (16 doubles = 128 B).
-constexpr int STAGE_C = 16;
+// Columns per staging chunk of the contiguous-axis tiles (8 doubles = 64 B:
+// fewer registers in flight than a full 128-B line, for 3 waves per SIMD).
+constexpr int STAGE_C = 8;
 
 // Coefficients a_j, b_j and c_{j-1} of row j >= 1 (compute_abc_nobc,
 // dadi/integration_shared.c:72-93, with the boundary term of the last row);
@@ -517,6 +518,233 @@ __device__ __forceinline__ void solve_tile_staged(
     }
 }
 
+// Recompute variant: the forward sweep keeps (bet, u') only at rows
+// 0, CK, 2CK, ... and writes nothing to phi but its last row; the back
+// substitution reloads each chunk's original phi rows and recomputes the
+// chunk's gammas and u' from the checkpoint with the forward sweep's own
+// operations (same bits), so phi makes one read-read-write trip per sweep
+// instead of read-write-read-write.
+template <int NO, int ORDER, int CK>
+__device__ __forceinline__ void regamma_u(int cb, int n, double bt, double u0,
+                                          const double (&ph)[CK],
+                                          const double *__restrict__ xInt,
+                                          const double *__restrict__ df,
+                                          const double *__restrict__ W0,
+                                          const double *__restrict__ W1,
+                                          const double *__restrict__ S, long long ts,
+                                          const double *c, const double *m, double dt,
+                                          double inv_dt, bool bcl_on, double bcl,
+                                          double (&g)[CK], double (&up)[CK])
+{
+    const double Mc = d_M<NO>(c, m, xInt[cb * ts], S[cb * ts]);
+    double qa = Mc * 0.5 + W0[cb * ts];
+    double qc = -Mc * (1 - 0.5) + W1[cb * ts];
+    double uu = u0;
+    up[0] = u0;
+#pragma unroll
+    for (int q = 0; q < CK; ++q) {
+        const int j = cb + 1 + q;
+        g[q] = 0.;
+        if (q + 1 < CK) up[q + 1] = 0.;
+        if (j <= n - 1) {
+            double a, b, cm;
+            row_coeffs<NO, ORDER>(j, n, xInt, df, W0, W1, S, ts, c, m, inv_dt, bcl_on, bcl, qa,
+                                  qc, a, b, cm);
+            g[q] = cm / bt;
+            bt = b - a * g[q];
+            if (q + 1 < CK && j <= n - 2) {
+                const double r = (ORDER == DH_ORDER_PRE) ? inv_dt * ph[q + 1] : ph[q + 1] / dt;
+                uu = (r - a * uu) / bt;
+                up[q + 1] = uu;
+            }
+        }
+    }
+}
+
+template <int NO, int ORDER, int CK>
+__device__ __forceinline__ void solve_line_ru(
+    double *__restrict__ p, long long s, int n,
+    const double *__restrict__ xInt, const double *__restrict__ df,
+    const double *__restrict__ W0, const double *__restrict__ W1,
+    const double *__restrict__ S, long long ts,
+    const double *c, const double *m,
+    double dt, double inv_dt,
+    bool bcf_on, double bcf, bool bcl_on, double bcl,
+    int inj_row, double inj,
+    double *__restrict__ chk, long long gs, int col)
+{
+    double *chku = chk + (long long)col * gs;
+    if (inj_row >= 0) p[inj_row * s] = p[inj_row * s] + inj;
+    double nxt[PF];
+#pragma unroll
+    for (int q = 0; q < PF; ++q) {
+        const int jj = 1 + q;
+        nxt[q] = p[(jj < n ? jj : n - 1) * s];
+    }
+    double bet, u, pa, pc;
+    row0<NO, ORDER>(p[0], xInt, df, W0, W1, S, c, m, dt, inv_dt, bcf_on, bcf, bet, u, pa, pc);
+    chk[0] = bet;
+    chku[0] = u;
+    for (int j0 = 1; j0 < n; j0 += PF) {
+        double cur[PF];
+#pragma unroll
+        for (int q = 0; q < PF; ++q) cur[q] = nxt[q];
+#pragma unroll
+        for (int q = 0; q < PF; ++q) {
+            const int jj = j0 + PF + q;
+            nxt[q] = p[(jj < n ? jj : n - 1) * s];
+        }
+#pragma unroll
+        for (int q = 0; q < PF; ++q) {
+            const int j = j0 + q;
+            if (j < n) {
+                fwd_core<NO, ORDER>(j, n, cur[q], xInt, df, W0, W1, S, ts, c, m, dt, inv_dt,
+                                    bcl_on, bcl, bet, u, pa, pc);
+                if (j % CK == 0) {
+                    chk[(j / CK) * gs] = bet;
+                    chku[(j / CK) * gs] = u;
+                }
+            }
+        }
+    }
+    p[(n - 1) * s] = u;   // u'_{n-1} is already the solution
+    for (int cb = ((n - 2) / CK) * CK; cb >= 0; cb -= CK) {
+        double ph[CK];
+#pragma unroll
+        for (int q = 0; q < CK; ++q) {
+            const int jr = cb + q;
+            ph[q] = p[(jr <= n - 2 ? jr : n - 2) * s];
+        }
+        double g[CK], up[CK];
+        regamma_u<NO, ORDER, CK>(cb, n, chk[(cb / CK) * gs], chku[(cb / CK) * gs], ph, xInt, df,
+                                 W0, W1, S, ts, c, m, dt, inv_dt, bcl_on, bcl, g, up);
+#pragma unroll
+        for (int q = CK - 1; q >= 0; --q) {
+            const int jr = cb + q;
+            if (jr <= n - 2) {
+                const double v = up[q] - g[q] * u;
+                p[jr * s] = v;
+                u = v;
+            }
+        }
+    }
+}
+
+// Contiguous-axis tile of the recompute variant: staging chunks of C = CK
+// columns; forward loads only, backward loads, recomputes and stores.
+template <int NO, int ORDER, int CK, int C>
+__device__ __forceinline__ void solve_tile_staged_ru(
+    double *__restrict__ tile, int rows_active, int n,
+    const double *__restrict__ xInt, const double *__restrict__ df,
+    const double *__restrict__ W0, const double *__restrict__ W1,
+    const double *__restrict__ S,
+    const double *c, const double *m, double dt, double inv_dt,
+    bool bcf_on, double bcf, bool bcl_on, double bcl, int inj_row, double inj,
+    double *__restrict__ chk, int col, double *__restrict__ buf, int lane)
+{
+    static_assert(C == CK, "one checkpoint chunk per staging chunk");
+    constexpr int LD = C + 1;
+    double *mine = buf + lane * LD;
+    double *chku = chk + (long long)col * 64;
+    const int nch = (n + C - 1) / C;
+    double bet = 0., u = 0., pa = 0., pc = 0.;
+    for (int k = 0; k < nch; ++k) {
+        const int j0 = k * C;
+        const int cw = n - j0 < C ? n - j0 : C;
+        {
+            double v[C];
+#pragma unroll
+            for (int q = 0; q < C; ++q) {
+                const int e = q * 64 + lane, row = e / C, cl = e - row * C;
+                const bool ok = row < rows_active && cl < cw;
+                v[q] = tile[ok ? (long long)row * n + j0 + cl : 0];
+                v[q] = ok ? v[q] : 0.;
+            }
+#pragma unroll
+            for (int q = 0; q < C; ++q) {
+                const int e = q * 64 + lane, row = e / C, cl = e - row * C;
+                buf[row * LD + cl] = v[q];
+            }
+        }
+        wave_sync();
+        if (k == 0 && inj_row >= 0) mine[inj_row] = mine[inj_row] + inj;
+        for (int cc = 0; cc < cw; ++cc) {
+            const int j = j0 + cc;
+            const double pj = mine[cc];
+            if (j == 0) {
+                row0<NO, ORDER>(pj, xInt, df, W0, W1, S, c, m, dt, inv_dt, bcf_on, bcf, bet, u,
+                                pa, pc);
+                chk[0] = bet;
+                chku[0] = u;
+            } else {
+                fwd_core<NO, ORDER>(j, n, pj, xInt, df, W0, W1, S, 1, c, m, dt, inv_dt, bcl_on,
+                                    bcl, bet, u, pa, pc);
+                if (j % CK == 0) {
+                    chk[(j / CK) * 64] = bet;
+                    chku[(j / CK) * 64] = u;
+                }
+            }
+        }
+        wave_sync();
+    }
+    const double ulast = u;
+    for (int k = nch - 1; k >= 0; --k) {
+        const int j0 = k * C;
+        const int cw = n - j0 < C ? n - j0 : C;
+        {
+            double v[C];
+#pragma unroll
+            for (int q = 0; q < C; ++q) {
+                const int e = q * 64 + lane, row = e / C, cl = e - row * C;
+                const bool ok = row < rows_active && cl < cw;
+                v[q] = tile[ok ? (long long)row * n + j0 + cl : 0];
+                v[q] = ok ? v[q] : 0.;
+            }
+#pragma unroll
+            for (int q = 0; q < C; ++q) {
+                const int e = q * 64 + lane, row = e / C, cl = e - row * C;
+                buf[row * LD + cl] = v[q];
+            }
+        }
+        wave_sync();
+        if (k == 0 && inj_row >= 0) mine[inj_row] = mine[inj_row] + inj;
+        if (j0 + cw - 1 == n - 1) mine[n - 1 - j0] = ulast;
+        const int cb = j0;
+        if (cb <= n - 2) {
+            double ph[CK];
+#pragma unroll
+            for (int q = 0; q < CK; ++q) ph[q] = mine[q < cw ? q : 0];
+            double g[CK], up[CK];
+            regamma_u<NO, ORDER, CK>(cb, n, chk[(cb / CK) * 64], chku[(cb / CK) * 64], ph, xInt,
+                                     df, W0, W1, S, 1, c, m, dt, inv_dt, bcl_on, bcl, g, up);
+#pragma unroll
+            for (int q = CK - 1; q >= 0; --q) {
+                const int jr = cb + q;
+                if (jr <= n - 2) {
+                    const double v = up[q] - g[q] * u;
+                    mine[q] = v;
+                    u = v;
+                }
+            }
+        }
+        wave_sync();
+        {
+            double v[C];
+#pragma unroll
+            for (int q = 0; q < C; ++q) {
+                const int e = q * 64 + lane, row = e / C, cl = e - row * C;
+                v[q] = buf[row * LD + cl];
+            }
+#pragma unroll
+            for (int q = 0; q < C; ++q) {
+                const int e = q * 64 + lane, row = e / C, cl = e - row * C;
+                if (row < rows_active && cl < cw) tile[(long long)row * n + j0 + cl] = v[q];
+            }
+        }
+        wave_sync();
+    }
+}
+
 // Boundary terms and injection row of one line of axis `a`.
 template <int D, int ORDER>
 struct LineSetup {
@@ -640,7 +868,7 @@ __device__ __forceinline__ void build_grid_tables(const double *__restrict__ xx,
 }
 
 // All lines of axis `a`, tiles of 64 lines per wave.
-template <int D, int ORDER, int CK>
+template <int D, int ORDER, int CK, int RU>
 __device__ __forceinline__ void sweep_tiles(double *__restrict__ phi, int n, int a, bool first_axis,
                                             int injm, const dh_segment *sp,
                                             const double *sx, const double *sxi,
@@ -670,9 +898,15 @@ __device__ __forceinline__ void sweep_tiles(double *__restrict__ phi, int n, int
                 line_setup<D, ORDER>(L, line < nsys ? line : nsys - 1, n, a, sx, mig, nu, ga, ha,
                                      sdx[0], sdx[n - 2], first_axis, injm);
                 if (line >= nsys) L.inj_row = -1;
-                solve_tile_staged<NO, ORDER, CK, STAGE_C>(
-                    phi + L0 * n, rows, n, sxi, sdf, W0, W1, S, L.c, mig, dt, inv_dt, L.bcf_on,
-                    L.bcf, L.bcl_on, L.bcl, L.inj_row, inj, gamcol, stage, lane);
+                if constexpr (RU)
+                    solve_tile_staged_ru<NO, ORDER, CK, CK>(
+                        phi + L0 * n, rows, n, sxi, sdf, W0, W1, S, L.c, mig, dt, inv_dt,
+                        L.bcf_on, L.bcf, L.bcl_on, L.bcl, L.inj_row, inj, gamcol,
+                        (n - 1) / CK + 1, stage, lane);
+                else
+                    solve_tile_staged<NO, ORDER, CK, STAGE_C>(
+                        phi + L0 * n, rows, n, sxi, sdf, W0, W1, S, L.c, mig, dt, inv_dt,
+                        L.bcf_on, L.bcf, L.bcl_on, L.bcl, L.inj_row, inj, gamcol, stage, lane);
             }
             return;
         }
@@ -689,6 +923,10 @@ __device__ __forceinline__ void sweep_tiles(double *__restrict__ phi, int n, int
                 solve_line<NO, ORDER>(p, s, n, sxi, sdf, W0, W1, S, 1, L.c, mig, dt, inv_dt,
                                       L.bcf_on, L.bcf, L.bcl_on, L.bcl, L.inj_row, inj, gamcol,
                                       64);
+            else if constexpr (RU)
+                solve_line_ru<NO, ORDER, CK>(p, s, n, sxi, sdf, W0, W1, S, 1, L.c, mig, dt,
+                                             inv_dt, L.bcf_on, L.bcf, L.bcl_on, L.bcl, L.inj_row,
+                                             inj, gamcol, 64, (n - 1) / CK + 1);
             else
                 solve_line_ck<NO, ORDER, CK>(p, s, n, sxi, sdf, W0, W1, S, 1, L.c, mig, dt,
                                              inv_dt, L.bcf_on, L.bcf, L.bcl_on, L.bcl, L.inj_row,
@@ -701,8 +939,8 @@ __device__ __forceinline__ void sweep_tiles(double *__restrict__ phi, int n, int
 // Kernel 1: persistent, one workgroup per instance (any D), whole schedule.
 // LDS: x, xInt, dfactor, dx (4n) | per axis W0,W1,S (3Dn) | gamma (waves*64*n)
 // ---------------------------------------------------------------------------
-template <int D, int ORDER, int CK>
-__global__ void __launch_bounds__(512) k_integrate_nd(dh_instance *__restrict__ tab, int ninst,
+template <int D, int ORDER, int CK, int RU>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3, 8))) k_integrate_nd(dh_instance *__restrict__ tab, int ninst,
                                                       int *__restrict__ counter,
                                                       long long max_steps)
 {
@@ -720,10 +958,12 @@ __global__ void __launch_bounds__(512) k_integrate_nd(dh_instance *__restrict__ 
         const int n = ip->n;
         double *sx = lds, *sxi = sx + n, *sdf = sxi + n, *sdx = sdf + n;
         double *tabs = sdx + n;
-        const int col = CK ? (n - 1) / (CK ? CK : 1) + 1 : n;   // doubles per lane
+        // per-lane scratch doubles: the gamma column, or (bet[, u']) checkpoints
+        const int col = CK ? ((n - 1) / (CK ? CK : 1) + 1) * (RU ? 2 : 1) : n;
+        const int sc = RU ? CK : STAGE_C;
         double *gamcol = tabs + LDS_TABLES_PER_AXIS * D * n + (long long)wave * 64 * col + lane;
         double *stage = tabs + LDS_TABLES_PER_AXIS * D * n + (long long)nw * 64 * col +
-                        (long long)wave * 64 * (STAGE_C + 1);
+                        (long long)wave * 64 * (sc + 1);
         build_grid_tables(ip->xx_dev, n, sx, sxi, sdf, sdx, tid, nthr);
         long long seg = ip->cur_seg, step = ip->cur_step;
         const long long nseg = ip->nseg;
@@ -747,7 +987,7 @@ __global__ void __launch_bounds__(512) k_integrate_nd(dh_instance *__restrict__ 
                 for (int a = 0; a < D; ++a) {
                     if ((frozen >> a) & 1) continue;
                     const double *W0 = tabs + LDS_TABLES_PER_AXIS * a * n;
-                    sweep_tiles<D, ORDER, CK>(phi, n, a, first, injm, sp, sx, sxi, sdf, sdx, W0,
+                    sweep_tiles<D, ORDER, CK, RU>(phi, n, a, first, injm, sp, sx, sxi, sdf, sdx, W0,
                                           W0 + n, W0 + 2 * n, dt, inv_dt, gamcol, stage, wave, nw,
                                           lane);
                     __syncthreads();
@@ -874,7 +1114,7 @@ __global__ void __launch_bounds__(256) k_sweep_axis(dh_instance inst, int a, int
     build_axis_tables<D>(sp, a, n, sx, sxi, sdx, W0, W1, S, tid, nthr);
     __syncthreads();
     const double dt = sp->dt, inv_dt = 1. / dt;
-    sweep_tiles<D, ORDER, 0>(inst.phi_dev, n, a, first_axis != 0, inst.inject, sp, sx, sxi, sdf,
+    sweep_tiles<D, ORDER, 0, 0>(inst.phi_dev, n, a, first_axis != 0, inst.inject, sp, sx, sxi, sdf,
                           sdx, W0, W1, S, dt, inv_dt, gamcol, nullptr,
                           (long long)blockIdx.x * nw + wave,
                           (long long)gridDim.x * nw, lane);
@@ -938,16 +1178,18 @@ __global__ void k_sample_contract(const double *__restrict__ sT, const double *_
 namespace {
 
 template <int D, int ORDER>
-const void *nd_kernel(int chunk)
+const void *nd_kernel(int chunk, int recompute)
 {
-    if (chunk == 8) return reinterpret_cast<const void *>(&k_integrate_nd<D, ORDER, 8>);
-    return reinterpret_cast<const void *>(&k_integrate_nd<D, ORDER, 0>);
+    if (chunk == 8 && recompute)
+        return reinterpret_cast<const void *>(&k_integrate_nd<D, ORDER, 8, 1>);
+    if (chunk == 8) return reinterpret_cast<const void *>(&k_integrate_nd<D, ORDER, 8, 0>);
+    return reinterpret_cast<const void *>(&k_integrate_nd<D, ORDER, 0, 0>);
 }
 
-const void *pick_nd(int D, int order, int chunk)
+const void *pick_nd(int D, int order, int chunk, int recompute)
 {
     switch (D * 4 + order) {
-#define DH_CASE(d, o) case d * 4 + o: return nd_kernel<d, o>(chunk);
+#define DH_CASE(d, o) case d * 4 + o: return nd_kernel<d, o>(chunk, recompute);
         DH_CASE(1, 0) DH_CASE(1, 1) DH_CASE(1, 2)
         DH_CASE(2, 0) DH_CASE(2, 1) DH_CASE(2, 2)
         DH_CASE(3, 0) DH_CASE(3, 1) DH_CASE(3, 2)
@@ -985,10 +1227,10 @@ const void *pick_1d(int order)
     return nullptr;
 }
 
-size_t nd_lds(int D, int n, int waves, int chunk)
+size_t nd_lds(int D, int n, int waves, int chunk, int recompute)
 {
-    const size_t col = chunk ? (size_t)((n - 1) / chunk + 1) : (size_t)n;
-    const size_t stage = chunk ? 64 * (size_t)(STAGE_C + 1) : 0;
+    const size_t col = chunk ? (size_t)((n - 1) / chunk + 1) * (recompute ? 2 : 1) : (size_t)n;
+    const size_t stage = chunk ? 64 * (size_t)((recompute ? chunk : STAGE_C) + 1) : 0;
     return sizeof(double) * ((size_t)n * (LDS_GRID_TABLES + LDS_TABLES_PER_AXIS * (size_t)D) +
                              (size_t)waves * (64 * col + stage));
 }
@@ -1023,7 +1265,10 @@ extern "C" {
 int dh_abi_version(void) { return 1; }
 size_t dh_sizeof_segment(void) { return sizeof(dh_segment); }
 size_t dh_sizeof_instance(void) { return sizeof(dh_instance); }
-size_t dh_lds_bytes(int D, int n, int waves, int chunk) { return nd_lds(D, n, waves, chunk); }
+size_t dh_lds_bytes(int D, int n, int waves, int chunk, int recompute)
+{
+    return nd_lds(D, n, waves, chunk, recompute);
+}
 size_t dh_max_lds(void) { return g_max_lds; }
 
 int dh_last_error(char *buf, int len)
@@ -1062,10 +1307,11 @@ int dh_integrate_nd(dh_instance *table_dev, int ninst, int D, int order, int nma
     if (nmax < 3) return fail(DH_EINVAL, "n must be >= 3");
     const int waves = cfg->waves;
     if (waves < 1 || waves > 8) return fail(DH_EINVAL, "waves must be 1..8");
-    const int chunk = cfg->chunk;
+    const int chunk = cfg->chunk, recompute = cfg->recompute;
     if (chunk != 0 && chunk != 8) return fail(DH_EINVAL, "chunk must be 0 or 8");
-    const void *fn = pick_nd(D, order, chunk);
-    const size_t lds = nd_lds(D, nmax, waves, chunk);
+    if (recompute && chunk != 8) return fail(DH_EINVAL, "recompute needs chunk 8");
+    const void *fn = pick_nd(D, order, chunk, recompute);
+    const size_t lds = nd_lds(D, nmax, waves, chunk, recompute);
     if (int e = set_lds(fn, lds)) return e;
     int groups = cfg->groups;
     if (groups <= 0) {

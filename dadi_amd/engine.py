@@ -71,29 +71,34 @@ class Context:
             self._scratch = t.empty(int(ndoubles), dtype=t.float64, device=self.device)
         return self._scratch
 
-    def max_waves(self, D, n, chunk=0):
-        """Largest waves-per-workgroup whose LDS fits."""
-        col = (n - 1) // chunk + 1 if chunk else n
-        fixed = 8 * n * (4 + 3 * D)
-        per_wave = 8 * 64 * (col + (17 if chunk else 0))
+    def max_waves(self, D, n, chunk=0, recompute=0):
+        """Largest waves-per-workgroup whose LDS fits (mirrors dh_lds_bytes)."""
+        per_wave = self.lib.dh_lds_bytes(D, n, 1, chunk, recompute) - \
+            self.lib.dh_lds_bytes(D, n, 0, chunk, recompute)
+        fixed = self.lib.dh_lds_bytes(D, n, 0, chunk, recompute)
         return max(0, min(8, (self.max_lds - fixed) // per_wave))
 
-    def shape(self, D, n, waves=None, chunk=None):
-        """Launch shape (waves per workgroup, chunk) of the nD kernel for
-        instances of D populations on n points: the autotuned choice when one
-        is recorded, else a rule of thumb."""
-        if (waves is None or chunk is None) and self.tuning is not None:
+    def shape(self, D, n, waves=None, chunk=None, recompute=None):
+        """Launch shape (waves per workgroup, chunk, recompute) of the nD
+        kernel for instances of D populations on n points: the autotuned
+        choice when one is recorded, else a rule of thumb."""
+        if (waves is None or chunk is None or recompute is None) and self.tuning is not None:
             hit = self.tuning.lookup(D, n)
             if hit is not None:
                 waves = hit[0] if waves is None else waves
                 chunk = hit[1] if chunk is None else chunk
+                recompute = hit[2] if recompute is None else recompute
         if chunk is None:
             chunk = 8 if (D >= 2 and n >= 40) else 0
+        if recompute is None:
+            recompute = 1 if chunk == 8 else 0
+        if chunk == 0:
+            recompute = 0
         nsys = n ** (D - 1)
-        wmax = self.max_waves(D, n, chunk)
+        wmax = self.max_waves(D, n, chunk, recompute)
         if wmax < 1 and chunk == 0:
-            chunk = 8
-            wmax = self.max_waves(D, n, chunk)
+            chunk, recompute = 8, 1
+            wmax = self.max_waves(D, n, chunk, recompute)
         if wmax < 1:
             raise ValueError("n=%d too large for the workgroup kernel at D=%d" % (n, D))
         if waves is None:
@@ -101,7 +106,7 @@ class Context:
             # small ones few, so more instances share a compute unit
             cap = 8 if nsys >= 1024 else 4
             waves = max(1, min(wmax, cap, math.ceil(nsys / 64)))
-        return min(waves, wmax), chunk
+        return min(waves, wmax), chunk, recompute
 
 
 def context(device=0):
@@ -171,7 +176,7 @@ def run_single(phi, xx, seg, order, frozen, inject, ctx=None, mode="auto", waves
                            order, frozen, inject)
         tab = ctx.to_dev(rec.view(np.uint8))
         counter = t.zeros(1, dtype=t.int32, device=ctx.device)
-        cfg = DhLaunch(w, 1, 0, ck, 0)
+        cfg = DhLaunch(w, 1, 0, ck, 0)   # one instance: full gamma column when it fits
         check(ctx.lib.dh_integrate_nd(tab.data_ptr(), 1, D, order, n, counter.data_ptr(),
                                       cfg, stream), "dh_integrate_nd")
         keep = (tab, counter, segd)
@@ -257,7 +262,8 @@ class PackedBatch:
         (device copy; the caller resets phi)."""
         self.tab.copy_(self.tab0)
 
-    def launch(self, waves=None, groups=0, max_steps=0, stream=None, chunk=None):
+    def launch(self, waves=None, groups=0, max_steps=0, stream=None, chunk=None,
+               recompute=None):
         ctx = self.ctx
         stream = ctx.stream() if stream is None else stream
         if self.D == 1:
@@ -272,8 +278,8 @@ class PackedBatch:
                                           self.nmax, self.counter.data_ptr(), scr.data_ptr(),
                                           scr.numel(), cfg, stream), "dh_integrate_1d")
         else:
-            w, ck = ctx.shape(self.D, self.nmax, waves, chunk)
-            cfg = DhLaunch(w, groups, max_steps, ck, 0)
+            w, ck, ru = ctx.shape(self.D, self.nmax, waves, chunk, recompute)
+            cfg = DhLaunch(w, groups, max_steps, ck, ru)
             check(ctx.lib.dh_integrate_nd(self.tab.data_ptr(), len(self.jobs), self.D,
                                           self.order, self.nmax, self.counter.data_ptr(), cfg,
                                           stream), "dh_integrate_nd")
@@ -283,7 +289,7 @@ class PackedBatch:
         return raw
 
 
-def run_batch(jobs, ctx=None, waves=None, groups=0, sort=True, chunk=None):
+def run_batch(jobs, ctx=None, waves=None, groups=0, sort=True, chunk=None, recompute=None):
     """Run every job to completion; jobs are grouped by (D, order) and each
     group runs as one persistent launch (longest job first)."""
     ctx = ctx or context()
@@ -296,7 +302,7 @@ def run_batch(jobs, ctx=None, waves=None, groups=0, sort=True, chunk=None):
         if sort:
             js = sorted(js, key=lambda j: -j.work())
         pb = PackedBatch(ctx, js)
-        pb.launch(waves=waves, groups=groups, chunk=chunk)
+        pb.launch(waves=waves, groups=groups, chunk=chunk, recompute=recompute)
         packs.append(pb)
     torch().cuda.current_stream(ctx.device).synchronize()
     return packs

@@ -235,8 +235,8 @@ class Server:
                                                    counter.data_ptr(), scr.data_ptr(), scr.numel(),
                                                    cfg, h), "dh_integrate_1d")
             else:
-                w, ck = ctx.shape(int(D), nmax)
-                cfg = _lib.DhLaunch(w, 0, self.quantum, ck, 0)
+                w, ck, ru = ctx.shape(int(D), nmax)
+                cfg = _lib.DhLaunch(w, 0, self.quantum, ck, ru)
                 _lib.check(ctx.lib.dh_integrate_nd(tabd.data_ptr(), len(group), int(D), int(order),
                                                    nmax, counter.data_ptr(), cfg, h),
                            "dh_integrate_nd")

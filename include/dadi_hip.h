@@ -87,7 +87,10 @@ typedef struct dh_launch {
     int chunk;             /* Thomas scratch: 0 = full gamma column per lane in LDS;
                               8 = bet checkpoint every 8 rows, gammas recomputed in
                               registers (less LDS, more VALU, same bits) */
-    int reserved;
+    int recompute;         /* with chunk 8: 1 = also checkpoint u' and recompute it in the
+                              back substitution from the untouched phi rows instead of
+                              writing u' over phi (24 instead of 32 bytes of phi traffic
+                              per point, same bits) */
 } dh_launch;
 
 /* Library / device set-up; must be called once per process before use.
@@ -99,8 +102,9 @@ int dh_last_error(char *buf, int len);
 int dh_abi_version(void);
 size_t dh_sizeof_segment(void);
 size_t dh_sizeof_instance(void);
-/* LDS footprint per workgroup of the nD kernel for (D, n, waves, chunk), bytes. */
-size_t dh_lds_bytes(int D, int n, int waves, int chunk);
+/* LDS footprint per workgroup of the nD kernel for (D, n, waves, chunk,
+ * recompute), bytes. */
+size_t dh_lds_bytes(int D, int n, int waves, int chunk, int recompute);
 /* Largest LDS a workgroup may use on the initialised device (0 before dh_init). */
 size_t dh_max_lds(void);
 

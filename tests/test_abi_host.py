@@ -44,8 +44,9 @@ def test_struct_layout_and_errors_without_gpu():
     rc = L.dh_integrate_nd(None, 1, 2, 0, 10, None, cfg, None)
     assert rc == -1
     assert "dh_init" in _lib.last_error()
-    assert L.dh_lds_bytes(2, 100, 1, 0) == 8 * 100 * (4 + 6 + 64)
-    assert L.dh_lds_bytes(2, 100, 2, 8) == 8 * (100 * 10 + 2 * 64 * (13 + 17))
+    assert L.dh_lds_bytes(2, 100, 1, 0, 0) == 8 * 100 * (4 + 6 + 64)
+    assert L.dh_lds_bytes(2, 100, 2, 8, 0) == 8 * (100 * 10 + 2 * 64 * (13 + 9))
+    assert L.dh_lds_bytes(2, 100, 2, 8, 1) == 8 * (100 * 10 + 2 * 64 * (26 + 9))
 
 
 def test_const_steps_match_python_loop():

@@ -172,9 +172,9 @@ def test_split_mig_extrapolated_spectrum_and_ll(dadi_amd, oracle):
     assert abs(ll_gpu - ll_ref) <= 1e-9
 
 
-@pytest.mark.parametrize("chunk", [0, 8])
+@pytest.mark.parametrize("chunk,recompute", [(0, 0), (8, 0), (8, 1)])
 @pytest.mark.parametrize("waves", [1, 2])
-def test_batched_variants_bit_exact_on_golden(dadi_amd, chunk, waves):
+def test_batched_variants_bit_exact_on_golden(dadi_amd, chunk, recompute, waves):
     """Every golden case in one batch per (D, order), with the full gamma
     column (chunk 0) and with checkpointed gammas (chunk 8), at 1 and 2 waves
     per workgroup: identical bits to the reference kernels."""
@@ -190,6 +190,6 @@ def test_batched_variants_bit_exact_on_golden(dadi_amd, chunk, waves):
             seg, order, fz, inj = G.segments(c, xx, which)
             jobs.append(engine.Job(ctx.to_dev(phi0), xx, seg, order, fz, inj))
             want.append(w[which])
-    engine.run_batch(jobs, waves=waves, chunk=chunk)
+    engine.run_batch(jobs, waves=waves, chunk=chunk, recompute=recompute)
     for j, w in zip(jobs, want):
         assert np.array_equal(j.phi.cpu().numpy(), w)
