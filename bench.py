@@ -100,6 +100,9 @@ def main():
     ap.add_argument("--waves", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=-1)
     ap.add_argument("--recompute", type=int, default=-1)
+    ap.add_argument("--gamma-global", type=int, default=-1)
+    ap.add_argument("--groups", type=int, default=0,
+                    help="persistent workgroups (0 = as many as fit)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
@@ -144,14 +147,20 @@ def main():
     steps_total = sum(int(np.sum(s["nsteps"])) for s in segs)
     alg_bytes = 16.0 * n * n * 2 * steps_total   # read+write phi per point per axis sweep
 
-    chunk = None if args.chunk < 0 else args.chunk
-    recompute = None if args.recompute < 0 else args.recompute
-    waves, chunk, recompute = ctx.shape(2, n, args.waves or None, chunk, recompute)
+    def opt(v):
+        return None if v < 0 else v
+
+    shape = ctx.shape(2, n, args.waves or None, opt(args.chunk), opt(args.recompute),
+                      opt(args.gamma_global))
+
+    def launch():
+        pb.launch(waves=shape.waves, groups=args.groups, chunk=shape.chunk,
+                  recompute=shape.recompute, gamma_global=shape.gamma_global)
 
     def one_step():
         arena.copy_(phi0d.expand(args.batch, n, n))
         pb.restart()
-        pb.launch(waves=waves, chunk=chunk, recompute=recompute)
+        launch()
 
     for _ in range(args.warmup):
         one_step()
@@ -173,7 +182,7 @@ def main():
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record()
-        pb.launch(waves=waves, chunk=chunk, recompute=recompute)
+        launch()
         e1.record()
         kern_ms.append((e0, e1))
     torch.cuda.synchronize()
@@ -225,8 +234,9 @@ def main():
                                "draws per persistent launch" % (n, args.batch),
                    "pts": n, "batch": args.batch, "time_steps_per_batch": steps_total,
                    "parallelism": "single GPU, persistent workgroup per integration",
-                   "waves_per_workgroup": waves, "thomas_chunk": chunk,
-                   "thomas_recompute": recompute},
+                   "waves_per_workgroup": shape.waves, "thomas_chunk": shape.chunk,
+                   "thomas_recompute": shape.recompute,
+                   "thomas_gamma_global": shape.gamma_global},
         "plain": {"value": plain_rate, "unit": "integrations/s", "calls": len(lat),
                   "time_steps": plain_steps,
                   "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3) if lat else None,
@@ -235,7 +245,8 @@ def main():
         "serving": serving,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": traffic,
-                     "kernel": "k_integrate_nd<2,1>", "kernel_ms_avg": kavg * 1e3,
+                     "kernel": "k_integrate_nd<2, 1, %d, %d, %d>" % (
+                         shape.chunk, shape.recompute, shape.gamma_global), "kernel_ms_avg": kavg * 1e3,
                      "algorithmic_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
     }

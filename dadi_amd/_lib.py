@@ -12,9 +12,12 @@ import threading
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libdadi_hip.so")
+# DADI_AMD_LIB names another build of the same library (e.g. one compiled
+# with different DH_* tunables) to load instead of the in-tree one.
+LIB_PATH = os.environ.get("DADI_AMD_LIB") or os.path.join(HERE, "libdadi_hip.so")
 
 MAXD = 5
+ABI_VERSION = 2
 ORDER_TV, ORDER_PRE, ORDER_PRE1 = 0, 1, 2
 
 #: numpy mirror of ``dh_segment``
@@ -53,7 +56,8 @@ class DhInstance(ctypes.Structure):
 class DhLaunch(ctypes.Structure):
     _fields_ = [("waves", ctypes.c_int), ("groups", ctypes.c_int),
                 ("max_steps", ctypes.c_longlong), ("chunk", ctypes.c_int),
-                ("recompute", ctypes.c_int)]
+                ("recompute", ctypes.c_int), ("gamma_dev", ctypes.c_void_p),
+                ("gamma_doubles", ctypes.c_longlong)]
 
 
 _vp = ctypes.c_void_p
@@ -63,7 +67,8 @@ _SIGS = {
     "dh_abi_version": ([], ctypes.c_int),
     "dh_sizeof_segment": ([], ctypes.c_size_t),
     "dh_sizeof_instance": ([], ctypes.c_size_t),
-    "dh_lds_bytes": ([ctypes.c_int] * 5, ctypes.c_size_t),
+    "dh_lds_bytes": ([ctypes.c_int] * 6, ctypes.c_size_t),
+    "dh_nd_groups": ([ctypes.c_int] * 8, ctypes.c_int),
     "dh_max_lds": ([], ctypes.c_size_t),
     "dh_integrate_nd": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp,
                          ctypes.POINTER(DhLaunch), _vp], ctypes.c_int),
@@ -108,7 +113,7 @@ def load(path=LIB_PATH):
             f = getattr(L, name)
             f.argtypes = args
             f.restype = res
-        if L.dh_abi_version() != 1:
+        if L.dh_abi_version() != ABI_VERSION:
             raise BackendUnavailable("libdadi_hip.so ABI version mismatch")
         if L.dh_sizeof_segment() != SEG_DTYPE.itemsize or \
                 L.dh_sizeof_instance() != INST_DTYPE.itemsize:

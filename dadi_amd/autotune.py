@@ -12,7 +12,10 @@ supports on a synthetic batch that fills the GPU and records the fastest:
   contiguous axis staged through LDS tiles);
 * with chunk 8, whether u' is written over phi between the two Thomas passes
   or recomputed from checkpoints (recompute 1: a quarter less phi traffic,
-  half the writes, more VALU).
+  half the writes, more VALU);
+* the gamma columns in a global workspace instead (gamma_global 1: no
+  recomputation and no gamma LDS, 16 more bytes of memory traffic per point
+  and sweep, most of it absorbed by the last-level cache).
 
 The axis order of the sweeps is not a tunable: it is the operator splitting
 of the reference (x, then y, ...) and changing it would change results.
@@ -49,7 +52,7 @@ class Tuning:
         return cls(table, path)
 
     def lookup(self, D, n):
-        """(waves, chunk) tuned for (D, n), or for the nearest tuned n of the
+        """engine.Shape tuned for (D, n), or for the nearest tuned n of the
         same D within 25%; None if nothing applies."""
         hit = self.table.get("%d:%d" % (D, n))
         if hit is None:
@@ -59,12 +62,15 @@ class Tuning:
                 if d == D and abs(m - n) <= 0.25 * n and (best is None or abs(m - n) < best[0]):
                     best = (abs(m - n), v)
             hit = best[1] if best else None
-        return None if hit is None else (int(hit["waves"]), int(hit["chunk"]),
-                                         int(hit.get("recompute", 0)))
+        if hit is None:
+            return None
+        return engine.Shape(hit["waves"], hit["chunk"], hit.get("recompute", 0),
+                            hit.get("gamma_global", 0))
 
-    def record(self, D, n, waves, chunk, recompute, rate, candidates):
-        self.table["%d:%d" % (D, n)] = {"waves": int(waves), "chunk": int(chunk),
-                                        "recompute": int(recompute),
+    def record(self, D, n, shape, rate, candidates):
+        self.table["%d:%d" % (D, n)] = {"waves": shape.waves, "chunk": shape.chunk,
+                                        "recompute": shape.recompute,
+                                        "gamma_global": shape.gamma_global,
                                         "points_per_s": float(rate),
                                         "candidates": candidates}
 
@@ -95,11 +101,11 @@ def _synthetic_jobs(ctx, D, n, count, steps, rng):
 def candidates(ctx, D, n):
     nsys = n ** (D - 1)
     out = []
-    for chunk, ru in ((0, 0), (8, 0), (8, 1)):
-        wmax = ctx.max_waves(D, n, chunk, ru)
+    for chunk, ru, gg in ((0, 0, 0), (8, 0, 0), (8, 1, 0), (0, 0, 1)):
+        wmax = ctx.max_waves(D, n, chunk, ru, gg)
         for w in (1, 2, 4, 8):
             if w <= wmax and (w == 1 or 64 * (w // 2) < nsys):
-                out.append((w, chunk, ru))
+                out.append(engine.Shape(w, chunk, ru, gg))
     return out
 
 
@@ -114,21 +120,23 @@ def tune(D, n, ctx=None, steps=30, count=None, reps=2, tuning=None, save=False):
     pb = engine.PackedBatch(ctx, jobs)
     work = sum(int(np.sum(j.seg["nsteps"])) for j in jobs) * (n ** D) * D
     results = []
-    for w, ck, ru in candidates(ctx, D, n):
+    for sh in candidates(ctx, D, n):
         best = None
         for _ in range(reps + 1):
             pb.restart()
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            pb.launch(waves=w, chunk=ck, recompute=ru)
+            pb.launch(waves=sh.waves, chunk=sh.chunk, recompute=sh.recompute,
+                      gamma_global=sh.gamma_global)
             torch.cuda.synchronize()
             el = time.perf_counter() - t0
             best = el if best is None else min(best, el)   # first run warms up
-        results.append({"waves": w, "chunk": ck, "recompute": ru, "points_per_s": work / best})
+        results.append({"waves": sh.waves, "chunk": sh.chunk, "recompute": sh.recompute,
+                        "gamma_global": sh.gamma_global, "points_per_s": work / best})
     top = max(results, key=lambda r: r["points_per_s"])
-    tuning.record(D, n, top["waves"], top["chunk"], top["recompute"], top["points_per_s"],
-                  results)
+    shape = engine.Shape(top["waves"], top["chunk"], top["recompute"], top["gamma_global"])
+    tuning.record(D, n, shape, top["points_per_s"], results)
     if save:
         tuning.save()
     ctx.tuning = tuning
-    return top["waves"], top["chunk"], top["recompute"]
+    return shape

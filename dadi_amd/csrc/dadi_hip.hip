@@ -86,13 +86,37 @@ __device__ __forceinline__ double d_M(const double *c, const double *m, double x
     }
 }
 
-// Rows fetched ahead of use in the Thomas sweeps.  The store of row j and the
-// load of row j+PF go to different addresses, so issuing the loads a block
-// early hides their latency without changing any result.
-constexpr int PF = 4;
-// Columns per staging chunk of the contiguous-axis tiles (8 doubles = 64 B:
-// fewer registers in flight than a full 128-B line, for 3 waves per SIMD).
-constexpr int STAGE_C = 8;
+// Device pointers stored in instance records are read as global-address-space
+// pointers, so the compiler emits global (not flat) memory instructions for
+// phi, the grid and the schedule: their completion is counted apart from LDS
+// traffic, and LDS waits no longer wait for phi rows still in flight.
+template <class T>
+__device__ __forceinline__ T *gbl(T *const &field)
+{
+    using GP = __attribute__((address_space(1))) T *;
+    return (T *)*(const GP *)&field;
+}
+
+// Compile-time tunables (defaults measured best on MI355X):
+//  DH_PF       rows fetched ahead of use in the Thomas sweeps.  The store of
+//              row j and the load of row j+PF go to different addresses, so
+//              issuing the loads a block early hides their latency without
+//              changing any result.
+//  DH_STAGE_C  columns per staging chunk of the contiguous-axis tiles (8
+//              doubles = 64 B: fewer registers in flight than a 128-B line).
+//  DH_WPE      minimum waves per SIMD the nD kernel is compiled for (bounds
+//              its registers: 3 -> at most 168 VGPRs).
+#ifndef DH_PF
+#define DH_PF 4
+#endif
+#ifndef DH_STAGE_C
+#define DH_STAGE_C 8
+#endif
+#ifndef DH_WPE
+#define DH_WPE 3
+#endif
+constexpr int PF = DH_PF;
+constexpr int STAGE_C = DH_STAGE_C;
 
 // Coefficients a_j, b_j and c_{j-1} of row j >= 1 (compute_abc_nobc,
 // dadi/integration_shared.c:72-93, with the boundary term of the last row);
@@ -400,7 +424,8 @@ __device__ __forceinline__ void wave_sync()
 // consecutive rows of n doubles, moved between HBM and LDS in chunks of C
 // columns with every lane touching consecutive addresses (each row chunk is
 // one 128-byte run), while each lane runs its own Thomas row out of LDS.
-// Checkpointed gammas (CK), u' written back between the sweeps.
+// Checkpointed gammas (CK); u' written back between the two passes except for
+// the last staging chunk, which stays in LDS.
 template <int NO, int ORDER, int CK, int C>
 __device__ __forceinline__ void solve_tile_staged(
     double *__restrict__ tile, int rows_active, int n,
@@ -451,6 +476,7 @@ __device__ __forceinline__ void solve_tile_staged(
             mine[cc] = u;
         }
         wave_sync();
+        if (k == nch - 1) break;   // the last chunk is consumed from LDS
         {
             double v[C];
 #pragma unroll
@@ -469,6 +495,76 @@ __device__ __forceinline__ void solve_tile_staged(
     for (int k = nch - 1; k >= 0; --k) {
         const int j0 = k * C;
         const int cw = n - j0 < C ? n - j0 : C;
+        if (k < nch - 1) {
+            double v[C];
+#pragma unroll
+            for (int q = 0; q < C; ++q) {
+                const int e = q * 64 + lane, row = e / C, col = e - row * C;
+                const bool ok = row < rows_active && col < cw;
+                v[q] = tile[ok ? (long long)row * n + j0 + col : 0];
+                v[q] = ok ? v[q] : 0.;
+            }
+#pragma unroll
+            for (int q = 0; q < C; ++q) {
+                const int e = q * 64 + lane, row = e / C, col = e - row * C;
+                buf[row * LD + col] = v[q];
+            }
+            wave_sync();
+        }
+        const int top = (n - 2 < j0 + cw - 1) ? n - 2 : j0 + cw - 1;
+        for (int cb = top >= j0 ? (top / CK) * CK : j0 - CK; cb >= j0; cb -= CK) {
+            double g[CK];
+            regamma<NO, ORDER, CK>(cb, n, chk[(cb / CK) * 64], xInt, df, W0, W1, S, 1, c, m,
+                                   inv_dt, bcl_on, bcl, g);
+#pragma unroll
+            for (int q = CK - 1; q >= 0; --q) {
+                const int jr = cb + q;
+                if (jr <= n - 2) {
+                    const double v = mine[jr - j0] - g[q] * u;
+                    mine[jr - j0] = v;
+                    u = v;
+                }
+            }
+        }
+        wave_sync();
+        {
+            double v[C];
+#pragma unroll
+            for (int q = 0; q < C; ++q) {
+                const int e = q * 64 + lane, row = e / C, col = e - row * C;
+                v[q] = buf[row * LD + col];
+            }
+#pragma unroll
+            for (int q = 0; q < C; ++q) {
+                const int e = q * 64 + lane, row = e / C, col = e - row * C;
+                if (row < rows_active && col < cw) tile[(long long)row * n + j0 + col] = v[q];
+            }
+        }
+        wave_sync();
+    }
+}
+
+// Contiguous-axis tile with each lane's gamma column in a global workspace
+// (gam: this lane's column, stride 64 doubles between rows): the back
+// substitution reads the forward sweep's gammas instead of recomputing them.
+// The last staging chunk stays in LDS between the two passes.
+template <int NO, int ORDER, int C>
+__device__ __forceinline__ void solve_tile_staged_gg(
+    double *__restrict__ tile, int rows_active, int n,
+    const double *__restrict__ xInt, const double *__restrict__ df,
+    const double *__restrict__ W0, const double *__restrict__ W1,
+    const double *__restrict__ S,
+    const double *c, const double *m, double dt, double inv_dt,
+    bool bcf_on, double bcf, bool bcl_on, double bcl, int inj_row, double inj,
+    double *__restrict__ gam, double *__restrict__ buf, int lane)
+{
+    constexpr int LD = C + 1;
+    double *mine = buf + lane * LD;
+    const int nch = (n + C - 1) / C;
+    double bet = 0., u = 0., pa = 0., pc = 0.;
+    for (int k = 0; k < nch; ++k) {
+        const int j0 = k * C;
+        const int cw = n - j0 < C ? n - j0 : C;
         {
             double v[C];
 #pragma unroll
@@ -485,19 +581,68 @@ __device__ __forceinline__ void solve_tile_staged(
             }
         }
         wave_sync();
-        const int top = (n - 2 < j0 + cw - 1) ? n - 2 : j0 + cw - 1;
-        for (int cb = top >= j0 ? (top / CK) * CK : j0 - CK; cb >= j0; cb -= CK) {
-            double g[CK];
-            regamma<NO, ORDER, CK>(cb, n, chk[(cb / CK) * 64], xInt, df, W0, W1, S, 1, c, m,
-                                   inv_dt, bcl_on, bcl, g);
+        if (k == 0 && inj_row >= 0) mine[inj_row] = mine[inj_row] + inj;
+        for (int cc = 0; cc < cw; ++cc) {
+            const int j = j0 + cc;
+            const double pj = mine[cc];
+            if (j == 0) {
+                row0<NO, ORDER>(pj, xInt, df, W0, W1, S, c, m, dt, inv_dt, bcf_on, bcf, bet, u,
+                                pa, pc);
+            } else {
+                gam[j * 64] = fwd_core<NO, ORDER>(j, n, pj, xInt, df, W0, W1, S, 1, c, m, dt,
+                                                  inv_dt, bcl_on, bcl, bet, u, pa, pc);
+            }
+            mine[cc] = u;
+        }
+        wave_sync();
+        if (k == nch - 1) break;   // the last chunk is consumed from LDS
+        {
+            double v[C];
 #pragma unroll
-            for (int q = CK - 1; q >= 0; --q) {
-                const int jr = cb + q;
-                if (jr <= n - 2) {
-                    const double v = mine[jr - j0] - g[q] * u;
-                    mine[jr - j0] = v;
-                    u = v;
-                }
+            for (int q = 0; q < C; ++q) {
+                const int e = q * 64 + lane, row = e / C, col = e - row * C;
+                v[q] = buf[row * LD + col];
+            }
+#pragma unroll
+            for (int q = 0; q < C; ++q) {
+                const int e = q * 64 + lane, row = e / C, col = e - row * C;
+                if (row < rows_active && col < cw) tile[(long long)row * n + j0 + col] = v[q];
+            }
+        }
+        wave_sync();
+    }
+    for (int k = nch - 1; k >= 0; --k) {
+        const int j0 = k * C;
+        const int cw = n - j0 < C ? n - j0 : C;
+        double g[C];
+#pragma unroll
+        for (int q = 0; q < C; ++q) {
+            const int jj = j0 + q + 1;
+            g[q] = gam[(jj <= n - 1 ? jj : n - 1) * 64];
+        }
+        if (k < nch - 1) {
+            double v[C];
+#pragma unroll
+            for (int q = 0; q < C; ++q) {
+                const int e = q * 64 + lane, row = e / C, col = e - row * C;
+                const bool ok = row < rows_active && col < cw;
+                v[q] = tile[ok ? (long long)row * n + j0 + col : 0];
+                v[q] = ok ? v[q] : 0.;
+            }
+#pragma unroll
+            for (int q = 0; q < C; ++q) {
+                const int e = q * 64 + lane, row = e / C, col = e - row * C;
+                buf[row * LD + col] = v[q];
+            }
+            wave_sync();
+        }
+#pragma unroll
+        for (int q = C - 1; q >= 0; --q) {
+            const int jr = j0 + q;
+            if (q < cw && jr <= n - 2) {
+                const double v = mine[q] - g[q] * u;
+                mine[q] = v;
+                u = v;
             }
         }
         wave_sync();
@@ -867,8 +1012,9 @@ __device__ __forceinline__ void build_grid_tables(const double *__restrict__ xx,
     }
 }
 
-// All lines of axis `a`, tiles of 64 lines per wave.
-template <int D, int ORDER, int CK, int RU>
+// All lines of axis `a`, tiles of 64 lines per wave.  GG: gamcol is this
+// lane's gamma column in the global workspace (rows 64 doubles apart).
+template <int D, int ORDER, int CK, int RU, int GG>
 __device__ __forceinline__ void sweep_tiles(double *__restrict__ phi, int n, int a, bool first_axis,
                                             int injm, const dh_segment *sp,
                                             const double *sx, const double *sxi,
@@ -887,7 +1033,7 @@ __device__ __forceinline__ void sweep_tiles(double *__restrict__ phi, int n, int
     for (int q = 0; q < NO; ++q) mig[q] = sp->mig[a][q];
     const double nu = sp->nu[a], ga = sp->gamma[a], ha = sp->h[a];
     const double inj = sp->inject;
-    if constexpr (CK != 0 && D >= 2) {
+    if constexpr ((CK != 0 || GG) && D >= 2) {
         if (a == D - 1) {
             // contiguous axis: whole-wave tiles staged through LDS
             for (long long t = tile0; t * 64 < nsys; t += tstep) {
@@ -898,7 +1044,11 @@ __device__ __forceinline__ void sweep_tiles(double *__restrict__ phi, int n, int
                 line_setup<D, ORDER>(L, line < nsys ? line : nsys - 1, n, a, sx, mig, nu, ga, ha,
                                      sdx[0], sdx[n - 2], first_axis, injm);
                 if (line >= nsys) L.inj_row = -1;
-                if constexpr (RU)
+                if constexpr (GG)
+                    solve_tile_staged_gg<NO, ORDER, STAGE_C>(
+                        phi + L0 * n, rows, n, sxi, sdf, W0, W1, S, L.c, mig, dt, inv_dt,
+                        L.bcf_on, L.bcf, L.bcl_on, L.bcl, L.inj_row, inj, gamcol, stage, lane);
+                else if constexpr (RU)
                     solve_tile_staged_ru<NO, ORDER, CK, CK>(
                         phi + L0 * n, rows, n, sxi, sdf, W0, W1, S, L.c, mig, dt, inv_dt,
                         L.bcf_on, L.bcf, L.bcl_on, L.bcl, L.inj_row, inj, gamcol,
@@ -939,10 +1089,10 @@ __device__ __forceinline__ void sweep_tiles(double *__restrict__ phi, int n, int
 // Kernel 1: persistent, one workgroup per instance (any D), whole schedule.
 // LDS: x, xInt, dfactor, dx (4n) | per axis W0,W1,S (3Dn) | gamma (waves*64*n)
 // ---------------------------------------------------------------------------
-template <int D, int ORDER, int CK, int RU>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3, 8))) k_integrate_nd(dh_instance *__restrict__ tab, int ninst,
-                                                      int *__restrict__ counter,
-                                                      long long max_steps)
+template <int D, int ORDER, int CK, int RU, int GG>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(DH_WPE, 8)))
+k_integrate_nd(dh_instance *__restrict__ tab, int ninst, int *__restrict__ counter,
+               long long max_steps, double *__restrict__ gws, int nmax)
 {
     extern __shared__ double lds[];
     __shared__ int s_next;
@@ -961,17 +1111,23 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3, 8))
         // per-lane scratch doubles: the gamma column, or (bet[, u']) checkpoints
         const int col = CK ? ((n - 1) / (CK ? CK : 1) + 1) * (RU ? 2 : 1) : n;
         const int sc = RU ? CK : STAGE_C;
-        double *gamcol = tabs + LDS_TABLES_PER_AXIS * D * n + (long long)wave * 64 * col + lane;
-        double *stage = tabs + LDS_TABLES_PER_AXIS * D * n + (long long)nw * 64 * col +
-                        (long long)wave * 64 * (sc + 1);
-        build_grid_tables(ip->xx_dev, n, sx, sxi, sdf, sdx, tid, nthr);
+        double *gamcol, *stage;
+        if constexpr (GG) {
+            gamcol = gws + ((long long)blockIdx.x * nw + wave) * 64 * nmax + lane;
+            stage = tabs + LDS_TABLES_PER_AXIS * D * n + (long long)wave * 64 * (STAGE_C + 1);
+        } else {
+            gamcol = tabs + LDS_TABLES_PER_AXIS * D * n + (long long)wave * 64 * col + lane;
+            stage = tabs + LDS_TABLES_PER_AXIS * D * n + (long long)nw * 64 * col +
+                    (long long)wave * 64 * (sc + 1);
+        }
+        build_grid_tables(gbl(ip->xx_dev), n, sx, sxi, sdf, sdx, tid, nthr);
         long long seg = ip->cur_seg, step = ip->cur_step;
         const long long nseg = ip->nseg;
         long long budget = max_steps > 0 ? max_steps : LLONG_MAX;
         const int frozen = ip->frozen, injm = ip->inject;
-        double *phi = ip->phi_dev;
+        double *phi = gbl(ip->phi_dev);
         while (seg < nseg && budget > 0) {
-            const dh_segment *sp = ip->seg_dev + seg;
+            const dh_segment *sp = gbl(ip->seg_dev) + seg;
             __syncthreads();
             for (int a = 0; a < D; ++a) {
                 if ((frozen >> a) & 1) continue;
@@ -987,7 +1143,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3, 8))
                 for (int a = 0; a < D; ++a) {
                     if ((frozen >> a) & 1) continue;
                     const double *W0 = tabs + LDS_TABLES_PER_AXIS * a * n;
-                    sweep_tiles<D, ORDER, CK, RU>(phi, n, a, first, injm, sp, sx, sxi, sdf, sdx, W0,
+                    sweep_tiles<D, ORDER, CK, RU, GG>(phi, n, a, first, injm, sp, sx, sxi, sdf, sdx, W0,
                                           W0 + n, W0 + 2 * n, dt, inv_dt, gamcol, stage, wave, nw,
                                           lane);
                     __syncthreads();
@@ -1033,7 +1189,7 @@ __global__ void __launch_bounds__(256) k_integrate_1d(dh_instance *__restrict__ 
         if (id >= ninst) break;
         dh_instance *ip = tab + id;
         const int n = ip->n;
-        const double *xx = ip->xx_dev;
+        const double *xx = gbl(ip->xx_dev);
         for (int i = 0; i < n - 1; ++i) txi[i * ts] = 0.5 * (xx[i + 1] + xx[i]);
         for (int i = 0; i < n; ++i) {
             double v;
@@ -1046,11 +1202,11 @@ __global__ void __launch_bounds__(256) k_integrate_1d(dh_instance *__restrict__ 
         long long seg = ip->cur_seg, step = ip->cur_step;
         const long long nseg = ip->nseg;
         long long budget = max_steps > 0 ? max_steps : LLONG_MAX;
-        double *phi = ip->phi_dev;
+        double *phi = gbl(ip->phi_dev);
         const bool injon = ip->inject & 1;
         const bool frz = ip->frozen & 1;
         while (seg < nseg && budget > 0) {
-            const dh_segment *sp = ip->seg_dev + seg;
+            const dh_segment *sp = gbl(ip->seg_dev) + seg;
             const double nu = sp->nu[0], g = sp->gamma[0], h = sp->h[0];
             const double bp = sp->beta_p, bq = sp->beta_q;
             for (int i = 0; i < n - 1; ++i) {
@@ -1105,16 +1261,16 @@ __global__ void __launch_bounds__(256) k_sweep_axis(dh_instance inst, int a, int
     const int n = inst.n;
     const int nthr = blockDim.x, tid = threadIdx.x;
     const int nw = nthr >> 6, wave = tid >> 6, lane = tid & 63;
-    const dh_segment *sp = inst.seg_dev + seg_of_step[st];
+    const dh_segment *sp = gbl(inst.seg_dev) + seg_of_step[st];
     double *sx = lds, *sxi = sx + n, *sdf = sxi + n, *sdx = sdf + n;
     double *W0 = sdx + n, *W1 = W0 + n, *S = W1 + n;
     double *gamcol = S + n + (long long)wave * 64 * n + lane;
-    build_grid_tables(inst.xx_dev, n, sx, sxi, sdf, sdx, tid, nthr);
+    build_grid_tables(gbl(inst.xx_dev), n, sx, sxi, sdf, sdx, tid, nthr);
     __syncthreads();
     build_axis_tables<D>(sp, a, n, sx, sxi, sdx, W0, W1, S, tid, nthr);
     __syncthreads();
     const double dt = sp->dt, inv_dt = 1. / dt;
-    sweep_tiles<D, ORDER, 0, 0>(inst.phi_dev, n, a, first_axis != 0, inst.inject, sp, sx, sxi, sdf,
+    sweep_tiles<D, ORDER, 0, 0, 0>(gbl(inst.phi_dev), n, a, first_axis != 0, inst.inject, sp, sx, sxi, sdf,
                           sdx, W0, W1, S, dt, inv_dt, gamcol, nullptr,
                           (long long)blockIdx.x * nw + wave,
                           (long long)gridDim.x * nw, lane);
@@ -1178,18 +1334,19 @@ __global__ void k_sample_contract(const double *__restrict__ sT, const double *_
 namespace {
 
 template <int D, int ORDER>
-const void *nd_kernel(int chunk, int recompute)
+const void *nd_kernel(int chunk, int recompute, int gg)
 {
+    if (gg) return reinterpret_cast<const void *>(&k_integrate_nd<D, ORDER, 0, 0, 1>);
     if (chunk == 8 && recompute)
-        return reinterpret_cast<const void *>(&k_integrate_nd<D, ORDER, 8, 1>);
-    if (chunk == 8) return reinterpret_cast<const void *>(&k_integrate_nd<D, ORDER, 8, 0>);
-    return reinterpret_cast<const void *>(&k_integrate_nd<D, ORDER, 0, 0>);
+        return reinterpret_cast<const void *>(&k_integrate_nd<D, ORDER, 8, 1, 0>);
+    if (chunk == 8) return reinterpret_cast<const void *>(&k_integrate_nd<D, ORDER, 8, 0, 0>);
+    return reinterpret_cast<const void *>(&k_integrate_nd<D, ORDER, 0, 0, 0>);
 }
 
-const void *pick_nd(int D, int order, int chunk, int recompute)
+const void *pick_nd(int D, int order, int chunk, int recompute, int gg)
 {
     switch (D * 4 + order) {
-#define DH_CASE(d, o) case d * 4 + o: return nd_kernel<d, o>(chunk, recompute);
+#define DH_CASE(d, o) case d * 4 + o: return nd_kernel<d, o>(chunk, recompute, gg);
         DH_CASE(1, 0) DH_CASE(1, 1) DH_CASE(1, 2)
         DH_CASE(2, 0) DH_CASE(2, 1) DH_CASE(2, 2)
         DH_CASE(3, 0) DH_CASE(3, 1) DH_CASE(3, 2)
@@ -1227,10 +1384,11 @@ const void *pick_1d(int order)
     return nullptr;
 }
 
-size_t nd_lds(int D, int n, int waves, int chunk, int recompute)
+size_t nd_lds(int D, int n, int waves, int chunk, int recompute, int gg)
 {
-    const size_t col = chunk ? (size_t)((n - 1) / chunk + 1) * (recompute ? 2 : 1) : (size_t)n;
-    const size_t stage = chunk ? 64 * (size_t)((recompute ? chunk : STAGE_C) + 1) : 0;
+    const size_t col = gg ? 0
+                          : chunk ? (size_t)((n - 1) / chunk + 1) * (recompute ? 2 : 1) : (size_t)n;
+    const size_t stage = (chunk || gg) ? 64 * (size_t)((recompute ? chunk : STAGE_C) + 1) : 0;
     return sizeof(double) * ((size_t)n * (LDS_GRID_TABLES + LDS_TABLES_PER_AXIS * (size_t)D) +
                              (size_t)waves * (64 * col + stage));
 }
@@ -1238,6 +1396,20 @@ size_t nd_lds(int D, int n, int waves, int chunk, int recompute)
 size_t sweep_lds(int n, int waves)
 {
     return sizeof(double) * (size_t)n * (LDS_GRID_TABLES + LDS_TABLES_PER_AXIS + 64 * (size_t)waves);
+}
+
+// Persistent workgroups of an nD launch: as many as fit on the device
+// (LDS-limited, at most ~16 waves per compute unit), never more than there
+// are instances.
+int nd_groups(size_t lds, int waves, int groups, int ninst)
+{
+    if (groups <= 0) {
+        int per_cu = lds ? (int)(g_max_lds / lds) : 1;
+        if (per_cu < 1) per_cu = 1;
+        if (per_cu > 16 / waves + 1) per_cu = 16 / waves + 1;
+        groups = g_num_cu * per_cu;
+    }
+    return groups > ninst ? ninst : groups;
 }
 
 int check_init()
@@ -1262,12 +1434,22 @@ std::map<std::string, hipGraphExec_t> g_graphs;
 
 extern "C" {
 
-int dh_abi_version(void) { return 1; }
+int dh_abi_version(void) { return 2; }
 size_t dh_sizeof_segment(void) { return sizeof(dh_segment); }
 size_t dh_sizeof_instance(void) { return sizeof(dh_instance); }
-size_t dh_lds_bytes(int D, int n, int waves, int chunk, int recompute)
+size_t dh_lds_bytes(int D, int n, int waves, int chunk, int recompute, int gamma_global)
 {
-    return nd_lds(D, n, waves, chunk, recompute);
+    return nd_lds(D, n, waves, chunk, recompute, gamma_global != 0);
+}
+
+int dh_nd_groups(int D, int nmax, int waves, int chunk, int recompute, int gamma_global,
+                 int groups, int ninst)
+{
+    if (int e = check_init()) return e;
+    if (D < 1 || D > DH_MAXD || nmax < 3 || waves < 1 || waves > 8)
+        return fail(DH_EINVAL, "bad launch description");
+    return nd_groups(nd_lds(D, nmax, waves, chunk, recompute, gamma_global != 0), waves, groups,
+                     ninst);
 }
 size_t dh_max_lds(void) { return g_max_lds; }
 
@@ -1308,23 +1490,24 @@ int dh_integrate_nd(dh_instance *table_dev, int ninst, int D, int order, int nma
     const int waves = cfg->waves;
     if (waves < 1 || waves > 8) return fail(DH_EINVAL, "waves must be 1..8");
     const int chunk = cfg->chunk, recompute = cfg->recompute;
+    const int gg = cfg->gamma_dev != nullptr;
     if (chunk != 0 && chunk != 8) return fail(DH_EINVAL, "chunk must be 0 or 8");
     if (recompute && chunk != 8) return fail(DH_EINVAL, "recompute needs chunk 8");
-    const void *fn = pick_nd(D, order, chunk, recompute);
-    const size_t lds = nd_lds(D, nmax, waves, chunk, recompute);
+    if (gg && (chunk || recompute))
+        return fail(DH_EINVAL, "a global gamma workspace needs chunk 0 and recompute 0");
+    const void *fn = pick_nd(D, order, chunk, recompute, gg);
+    const size_t lds = nd_lds(D, nmax, waves, chunk, recompute, gg);
     if (int e = set_lds(fn, lds)) return e;
-    int groups = cfg->groups;
-    if (groups <= 0) {
-        int per_cu = (int)(g_max_lds / lds);
-        if (per_cu < 1) per_cu = 1;
-        if (per_cu > 16 / waves + 1) per_cu = 16 / waves + 1;
-        groups = g_num_cu * per_cu;
-    }
-    if (groups > ninst) groups = ninst;
+    const int groups = nd_groups(lds, waves, cfg->groups, ninst);
+    if (gg && cfg->gamma_doubles < (long long)groups * waves * 64 * nmax)
+        return fail(DH_ERANGE, "gamma workspace too small: need " +
+                                   std::to_string((long long)groups * waves * 64 * nmax) +
+                                   " doubles");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, counter_dev, 0);
     long long ms = cfg->max_steps;
-    void *args[] = {&table_dev, &ninst, &counter_dev, &ms};
+    double *gws = cfg->gamma_dev;
+    void *args[] = {&table_dev, &ninst, &counter_dev, &ms, &gws, &nmax};
     DH_HIP(hipLaunchKernel(fn, dim3(groups), dim3(64 * waves), args, lds, st));
     return DH_OK;
 }

@@ -32,7 +32,7 @@ def torch():
 
 
 class Context:
-    """Per-device state: library handle, device, scratch buffers."""
+    """Per-device state: library handle, device, grid cache, tuning table."""
 
     def __init__(self, device=0):
         t = torch()
@@ -42,7 +42,6 @@ class Context:
         self.num_cu = props.multi_processor_count
         self.max_lds = int(self.lib.dh_max_lds())
         self._xx_cache = {}
-        self._scratch = None
         from .autotune import Tuning
         self.tuning = Tuning.load()
 
@@ -65,39 +64,37 @@ class Context:
             self._xx_cache[key] = g
         return g
 
-    def scratch(self, ndoubles):
-        t = torch()
-        if self._scratch is None or self._scratch.numel() < ndoubles:
-            self._scratch = t.empty(int(ndoubles), dtype=t.float64, device=self.device)
-        return self._scratch
-
-    def max_waves(self, D, n, chunk=0, recompute=0):
+    def max_waves(self, D, n, chunk=0, recompute=0, gamma_global=0):
         """Largest waves-per-workgroup whose LDS fits (mirrors dh_lds_bytes)."""
-        per_wave = self.lib.dh_lds_bytes(D, n, 1, chunk, recompute) - \
-            self.lib.dh_lds_bytes(D, n, 0, chunk, recompute)
-        fixed = self.lib.dh_lds_bytes(D, n, 0, chunk, recompute)
+        L = self.lib
+        fixed = L.dh_lds_bytes(D, n, 0, chunk, recompute, gamma_global)
+        per_wave = L.dh_lds_bytes(D, n, 1, chunk, recompute, gamma_global) - fixed
         return max(0, min(8, (self.max_lds - fixed) // per_wave))
 
-    def shape(self, D, n, waves=None, chunk=None, recompute=None):
-        """Launch shape (waves per workgroup, chunk, recompute) of the nD
-        kernel for instances of D populations on n points: the autotuned
-        choice when one is recorded, else a rule of thumb."""
-        if (waves is None or chunk is None or recompute is None) and self.tuning is not None:
+    def shape(self, D, n, waves=None, chunk=None, recompute=None, gamma_global=None):
+        """Launch shape of the nD kernel for instances of D populations on n
+        points: the autotuned choice when one is recorded, else a rule of
+        thumb.  Returns Shape(waves, chunk, recompute, gamma_global)."""
+        if None in (waves, chunk, recompute, gamma_global) and self.tuning is not None:
             hit = self.tuning.lookup(D, n)
             if hit is not None:
-                waves = hit[0] if waves is None else waves
-                chunk = hit[1] if chunk is None else chunk
-                recompute = hit[2] if recompute is None else recompute
+                waves = hit.waves if waves is None else waves
+                chunk = hit.chunk if chunk is None else chunk
+                recompute = hit.recompute if recompute is None else recompute
+                gamma_global = hit.gamma_global if gamma_global is None else gamma_global
+        gamma_global = int(bool(gamma_global))
+        if gamma_global:
+            chunk, recompute = 0, 0
         if chunk is None:
             chunk = 8 if (D >= 2 and n >= 40) else 0
         if recompute is None:
-            recompute = 1 if chunk == 8 else 0
+            recompute = 0
         if chunk == 0:
             recompute = 0
         nsys = n ** (D - 1)
-        wmax = self.max_waves(D, n, chunk, recompute)
-        if wmax < 1 and chunk == 0:
-            chunk, recompute = 8, 1
+        wmax = self.max_waves(D, n, chunk, recompute, gamma_global)
+        if wmax < 1 and chunk == 0 and not gamma_global:
+            chunk, recompute = 8, 0
             wmax = self.max_waves(D, n, chunk, recompute)
         if wmax < 1:
             raise ValueError("n=%d too large for the workgroup kernel at D=%d" % (n, D))
@@ -106,7 +103,44 @@ class Context:
             # small ones few, so more instances share a compute unit
             cap = 8 if nsys >= 1024 else 4
             waves = max(1, min(wmax, cap, math.ceil(nsys / 64)))
-        return min(waves, wmax), chunk, recompute
+        return Shape(min(waves, wmax), chunk, recompute, gamma_global)
+
+    def nd_launch(self, tab_ptr, ninst, D, order, nmax, counter_ptr, stream, workspace,
+                  waves=None, groups=0, max_steps=0, chunk=None, recompute=None,
+                  gamma_global=None):
+        """One dh_integrate_nd call.  ``workspace`` is a dict owned by the
+        caller (one per stream: launches that may overlap must not share it)
+        that keeps the gamma workspace between calls."""
+        sh = self.shape(D, nmax, waves, chunk, recompute, gamma_global)
+        cfg = DhLaunch(sh.waves, groups, max_steps, sh.chunk, sh.recompute)
+        if sh.gamma_global:
+            g = self.lib.dh_nd_groups(D, nmax, sh.waves, sh.chunk, sh.recompute, 1, groups, ninst)
+            check(min(g, 0), "dh_nd_groups")
+            need = g * sh.waves * 64 * nmax
+            ws = workspace.get("gamma")
+            if ws is None or ws.numel() < need:
+                ws = torch().empty(need, dtype=torch().float64, device=self.device)
+                workspace["gamma"] = ws
+            cfg.gamma_dev = ws.data_ptr()
+            cfg.gamma_doubles = ws.numel()
+        check(self.lib.dh_integrate_nd(tab_ptr, ninst, D, order, nmax, counter_ptr, cfg, stream),
+              "dh_integrate_nd")
+        return sh
+
+
+class Shape(tuple):
+    """Launch shape of the nD kernel: waves per workgroup, Thomas chunk,
+    recompute flag, gamma columns in a global workspace."""
+    __slots__ = ()
+    _fields = ("waves", "chunk", "recompute", "gamma_global")
+
+    def __new__(cls, waves, chunk, recompute, gamma_global=0):
+        return tuple.__new__(cls, (int(waves), int(chunk), int(recompute), int(gamma_global)))
+
+    waves = property(lambda self: self[0])
+    chunk = property(lambda self: self[1])
+    recompute = property(lambda self: self[2])
+    gamma_global = property(lambda self: self[3])
 
 
 def context(device=0):
@@ -256,6 +290,7 @@ class PackedBatch:
         self.tab = self.tab0.clone()
         self.counter = t.zeros(1, dtype=t.int32, device=ctx.device)
         self.nmax = max(j.n for j in jobs)
+        self.workspace = {}
 
     def restart(self):
         """Reset every instance's progress to the start of its schedule
@@ -263,7 +298,7 @@ class PackedBatch:
         self.tab.copy_(self.tab0)
 
     def launch(self, waves=None, groups=0, max_steps=0, stream=None, chunk=None,
-               recompute=None):
+               recompute=None, gamma_global=None):
         ctx = self.ctx
         stream = ctx.stream() if stream is None else stream
         if self.D == 1:
@@ -272,24 +307,26 @@ class PackedBatch:
             g = groups or max(1, math.ceil(len(self.jobs) / thr))
             nl = g * thr
             need = 6 * self.nmax * nl
-            scr = ctx.scratch(need)
+            scr = self.workspace.get("lanes")
+            if scr is None or scr.numel() < need:
+                scr = torch().empty(need, dtype=torch().float64, device=ctx.device)
+                self.workspace["lanes"] = scr
             cfg = DhLaunch(w, g, max_steps, 0, 0)
             check(ctx.lib.dh_integrate_1d(self.tab.data_ptr(), len(self.jobs), self.order,
                                           self.nmax, self.counter.data_ptr(), scr.data_ptr(),
                                           scr.numel(), cfg, stream), "dh_integrate_1d")
         else:
-            w, ck, ru = ctx.shape(self.D, self.nmax, waves, chunk, recompute)
-            cfg = DhLaunch(w, groups, max_steps, ck, ru)
-            check(ctx.lib.dh_integrate_nd(self.tab.data_ptr(), len(self.jobs), self.D,
-                                          self.order, self.nmax, self.counter.data_ptr(), cfg,
-                                          stream), "dh_integrate_nd")
+            ctx.nd_launch(self.tab.data_ptr(), len(self.jobs), self.D, self.order, self.nmax,
+                          self.counter.data_ptr(), stream, self.workspace, waves, groups,
+                          max_steps, chunk, recompute, gamma_global)
 
     def progress(self):
         raw = self.tab.cpu().numpy().view(INST_DTYPE)
         return raw
 
 
-def run_batch(jobs, ctx=None, waves=None, groups=0, sort=True, chunk=None, recompute=None):
+def run_batch(jobs, ctx=None, waves=None, groups=0, sort=True, chunk=None, recompute=None,
+              gamma_global=None):
     """Run every job to completion; jobs are grouped by (D, order) and each
     group runs as one persistent launch (longest job first)."""
     ctx = ctx or context()
@@ -302,7 +339,8 @@ def run_batch(jobs, ctx=None, waves=None, groups=0, sort=True, chunk=None, recom
         if sort:
             js = sorted(js, key=lambda j: -j.work())
         pb = PackedBatch(ctx, js)
-        pb.launch(waves=waves, groups=groups, chunk=chunk, recompute=recompute)
+        pb.launch(waves=waves, groups=groups, chunk=chunk, recompute=recompute,
+                  gamma_global=gamma_global)
         packs.append(pb)
     torch().cuda.current_stream(ctx.device).synchronize()
     return packs

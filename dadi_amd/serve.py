@@ -226,20 +226,19 @@ class Server:
                 thr = 64
                 g = max(1, math.ceil(len(group) / thr))
                 need = 6 * nmax * g * thr
-                scr = self.scratch.get(key)
+                ws = self.scratch.setdefault(key, {})
+                scr = ws.get("lanes")
                 if scr is None or scr.numel() < need:
                     scr = t.empty(need, dtype=t.float64, device=ctx.device)
-                    self.scratch[key] = scr
+                    ws["lanes"] = scr
                 cfg = _lib.DhLaunch(1, g, self.quantum, 0, 0)
                 _lib.check(ctx.lib.dh_integrate_1d(tabd.data_ptr(), len(group), int(order), nmax,
                                                    counter.data_ptr(), scr.data_ptr(), scr.numel(),
                                                    cfg, h), "dh_integrate_1d")
             else:
-                w, ck, ru = ctx.shape(int(D), nmax)
-                cfg = _lib.DhLaunch(w, 0, self.quantum, ck, ru)
-                _lib.check(ctx.lib.dh_integrate_nd(tabd.data_ptr(), len(group), int(D), int(order),
-                                                   nmax, counter.data_ptr(), cfg, h),
-                           "dh_integrate_nd")
+                ws = self.scratch.setdefault(key, {})
+                ctx.nd_launch(tabd.data_ptr(), len(group), int(D), int(order), nmax,
+                              counter.data_ptr(), h, ws, max_steps=self.quantum)
         self.launches += 1
         return st, tabd, counter
 

@@ -91,6 +91,12 @@ typedef struct dh_launch {
                               back substitution from the untouched phi rows instead of
                               writing u' over phi (24 instead of 32 bytes of phi traffic
                               per point, same bits) */
+    double *gamma_dev;     /* NULL, or (with chunk 0, recompute 0) a device workspace for
+                              the gamma columns: no recomputation and no gamma LDS, at
+                              16 more bytes of memory traffic per point and sweep; it must
+                              hold groups * waves * 64 * nmax doubles (dh_nd_groups) and
+                              may not be shared by launches that overlap */
+    long long gamma_doubles;
 } dh_launch;
 
 /* Library / device set-up; must be called once per process before use.
@@ -103,8 +109,13 @@ int dh_abi_version(void);
 size_t dh_sizeof_segment(void);
 size_t dh_sizeof_instance(void);
 /* LDS footprint per workgroup of the nD kernel for (D, n, waves, chunk,
- * recompute), bytes. */
-size_t dh_lds_bytes(int D, int n, int waves, int chunk, int recompute);
+ * recompute, gamma in a global workspace or not), bytes. */
+size_t dh_lds_bytes(int D, int n, int waves, int chunk, int recompute, int gamma_global);
+/* Number of persistent workgroups dh_integrate_nd launches for this launch
+ * shape (`groups` if > 0, else as many as fit), capped at ninst; sizes the
+ * gamma workspace.  Negative DH_E* code on error. */
+int dh_nd_groups(int D, int nmax, int waves, int chunk, int recompute, int gamma_global,
+                 int groups, int ninst);
 /* Largest LDS a workgroup may use on the initialised device (0 before dh_init). */
 size_t dh_max_lds(void);
 

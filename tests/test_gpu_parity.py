@@ -172,12 +172,13 @@ def test_split_mig_extrapolated_spectrum_and_ll(dadi_amd, oracle):
     assert abs(ll_gpu - ll_ref) <= 1e-9
 
 
-@pytest.mark.parametrize("chunk,recompute", [(0, 0), (8, 0), (8, 1)])
+@pytest.mark.parametrize("chunk,recompute,gg", [(0, 0, 0), (8, 0, 0), (8, 1, 0), (0, 0, 1)])
 @pytest.mark.parametrize("waves", [1, 2])
-def test_batched_variants_bit_exact_on_golden(dadi_amd, chunk, recompute, waves):
+def test_batched_variants_bit_exact_on_golden(dadi_amd, chunk, recompute, gg, waves):
     """Every golden case in one batch per (D, order), with the full gamma
-    column (chunk 0) and with checkpointed gammas (chunk 8), at 1 and 2 waves
-    per workgroup: identical bits to the reference kernels."""
+    column in LDS (chunk 0), checkpointed gammas (chunk 8, with u' stored or
+    recomputed) and gamma columns in a global workspace, at 1 and 2 waves per
+    workgroup: identical bits to the reference kernels."""
     from dadi_amd import engine
     ctx = engine.context()
     jobs, want = [], []
@@ -190,6 +191,6 @@ def test_batched_variants_bit_exact_on_golden(dadi_amd, chunk, recompute, waves)
             seg, order, fz, inj = G.segments(c, xx, which)
             jobs.append(engine.Job(ctx.to_dev(phi0), xx, seg, order, fz, inj))
             want.append(w[which])
-    engine.run_batch(jobs, waves=waves, chunk=chunk, recompute=recompute)
+    engine.run_batch(jobs, waves=waves, chunk=chunk, recompute=recompute, gamma_global=gg)
     for j, w in zip(jobs, want):
         assert np.array_equal(j.phi.cpu().numpy(), w)
