@@ -101,6 +101,8 @@ def main():
     ap.add_argument("--chunk", type=int, default=-1)
     ap.add_argument("--recompute", type=int, default=-1)
     ap.add_argument("--gamma-global", type=int, default=-1)
+    ap.add_argument("--slots", type=int, default=-1,
+                    help="instances per workgroup (1 = unpacked kernel; -1 = automatic)")
     ap.add_argument("--groups", type=int, default=0,
                     help="persistent workgroups (0 = as many as fit)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -151,11 +153,12 @@ def main():
         return None if v < 0 else v
 
     shape = ctx.shape(2, n, args.waves or None, opt(args.chunk), opt(args.recompute),
-                      opt(args.gamma_global))
+                      opt(args.gamma_global), opt(args.slots), packable=pb.packable)
 
     def launch():
         pb.launch(waves=shape.waves, groups=args.groups, chunk=shape.chunk,
-                  recompute=shape.recompute, gamma_global=shape.gamma_global)
+                  recompute=shape.recompute, gamma_global=shape.gamma_global,
+                  slots=shape.slots)
 
     def one_step():
         arena.copy_(phi0d.expand(args.batch, n, n))
@@ -233,10 +236,12 @@ def main():
         "config": {"workload": "2D split_mig two_pops integrations, pts=%d, batch of %d seeded "
                                "draws per persistent launch" % (n, args.batch),
                    "pts": n, "batch": args.batch, "time_steps_per_batch": steps_total,
-                   "parallelism": "single GPU, persistent workgroup per integration",
+                   "parallelism": "single GPU, persistent workgroups, %d integration(s) each"
+                                  % shape.slots,
                    "waves_per_workgroup": shape.waves, "thomas_chunk": shape.chunk,
                    "thomas_recompute": shape.recompute,
-                   "thomas_gamma_global": shape.gamma_global},
+                   "thomas_gamma_global": shape.gamma_global,
+                   "instances_per_workgroup": shape.slots},
         "plain": {"value": plain_rate, "unit": "integrations/s", "calls": len(lat),
                   "time_steps": plain_steps,
                   "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3) if lat else None,
@@ -245,8 +250,10 @@ def main():
         "serving": serving,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": traffic,
-                     "kernel": "k_integrate_nd<2, 1, %d, %d, %d>" % (
-                         shape.chunk, shape.recompute, shape.gamma_global), "kernel_ms_avg": kavg * 1e3,
+                     "kernel": ("k_integrate_packed<2, 1>" if shape.slots >= 2 else
+                                "k_integrate_nd<2, 1, %d, %d, %d>" % (
+                                    shape.chunk, shape.recompute, shape.gamma_global)),
+                     "kernel_ms_avg": kavg * 1e3,
                      "algorithmic_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
     }

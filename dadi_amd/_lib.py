@@ -72,6 +72,9 @@ _SIGS = {
     "dh_max_lds": ([], ctypes.c_size_t),
     "dh_integrate_nd": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp,
                          ctypes.POINTER(DhLaunch), _vp], ctypes.c_int),
+    "dh_integrate_packed": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                             ctypes.c_int, _vp, ctypes.POINTER(DhLaunch), _vp], ctypes.c_int),
+    "dh_packed_lds_bytes": ([ctypes.c_int] * 4, ctypes.c_size_t),
     "dh_integrate_1d": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp,
                          ctypes.c_longlong, ctypes.POINTER(DhLaunch), _vp], ctypes.c_int),
     "dh_sweep_axis": ([ctypes.POINTER(DhInstance), ctypes.c_int, ctypes.c_int, _vp,

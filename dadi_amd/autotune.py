@@ -15,7 +15,9 @@ supports on a synthetic batch that fills the GPU and records the fastest:
   half the writes, more VALU);
 * the gamma columns in a global workspace instead (gamma_global 1: no
   recomputation and no gamma LDS, 16 more bytes of memory traffic per point
-  and sweep, most of it absorbed by the last-level cache).
+  and sweep);
+* instances per workgroup (slots >= 2: the packed kernel, whose lockstep
+  instances fill the lanes a lone instance leaves idle).
 
 The axis order of the sweeps is not a tunable: it is the operator splitting
 of the reference (x, then y, ...) and changing it would change results.
@@ -51,9 +53,10 @@ class Tuning:
             table = {}
         return cls(table, path)
 
-    def lookup(self, D, n):
+    def lookup(self, D, n, packable=True):
         """engine.Shape tuned for (D, n), or for the nearest tuned n of the
-        same D within 25%; None if nothing applies."""
+        same D within 25%; None if nothing applies.  packable=False: the
+        fastest recorded shape with one instance per workgroup."""
         hit = self.table.get("%d:%d" % (D, n))
         if hit is None:
             best = None
@@ -64,20 +67,32 @@ class Tuning:
             hit = best[1] if best else None
         if hit is None:
             return None
-        return engine.Shape(hit["waves"], hit["chunk"], hit.get("recompute", 0),
-                            hit.get("gamma_global", 0))
+        pool = [hit] + list(hit.get("candidates", []))
+        if not packable:
+            pool = [c for c in pool if c.get("slots", 1) < 2]
+        if not pool:
+            return None
+        top = max(pool, key=lambda c: c.get("points_per_s", 0.0)) if len(pool) > 1 else pool[0]
+        return _shape_of(top)
 
     def record(self, D, n, shape, rate, candidates):
-        self.table["%d:%d" % (D, n)] = {"waves": shape.waves, "chunk": shape.chunk,
-                                        "recompute": shape.recompute,
-                                        "gamma_global": shape.gamma_global,
-                                        "points_per_s": float(rate),
-                                        "candidates": candidates}
+        self.table["%d:%d" % (D, n)] = dict(_fields_of(shape), points_per_s=float(rate),
+                                            candidates=candidates)
 
     def save(self, path=None):
         path = path or self.path or DEFAULT_PATH
         with open(path, "w") as f:
             json.dump(self.table, f, indent=1, sort_keys=True)
+
+
+def _fields_of(shape):
+    return {"waves": shape.waves, "chunk": shape.chunk, "recompute": shape.recompute,
+            "gamma_global": shape.gamma_global, "slots": shape.slots}
+
+
+def _shape_of(rec):
+    return engine.Shape(rec["waves"], rec["chunk"], rec.get("recompute", 0),
+                        rec.get("gamma_global", 0), rec.get("slots", 1))
 
 
 def _synthetic_jobs(ctx, D, n, count, steps, rng):
@@ -106,6 +121,15 @@ def candidates(ctx, D, n):
         for w in (1, 2, 4, 8):
             if w <= wmax and (w == 1 or 64 * (w // 2) < nsys):
                 out.append(engine.Shape(w, chunk, ru, gg))
+    if D >= 2:
+        # several instances per workgroup: every slot count that fills 1-8 waves
+        Lp = -(-nsys // 8) * 8
+        seen = set()
+        for w in range(1, 9):
+            S = min(16, (64 * w) // Lp)
+            if S >= 2 and S not in seen and ctx.pack_plan(D, n, w, S) is not None:
+                seen.add(S)
+                out.append(engine.Shape(w, 8, 0, 0, S))
     return out
 
 
@@ -127,14 +151,13 @@ def tune(D, n, ctx=None, steps=30, count=None, reps=2, tuning=None, save=False):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             pb.launch(waves=sh.waves, chunk=sh.chunk, recompute=sh.recompute,
-                      gamma_global=sh.gamma_global)
+                      gamma_global=sh.gamma_global, slots=sh.slots)
             torch.cuda.synchronize()
             el = time.perf_counter() - t0
             best = el if best is None else min(best, el)   # first run warms up
-        results.append({"waves": sh.waves, "chunk": sh.chunk, "recompute": sh.recompute,
-                        "gamma_global": sh.gamma_global, "points_per_s": work / best})
+        results.append(dict(_fields_of(sh), points_per_s=work / best))
     top = max(results, key=lambda r: r["points_per_s"])
-    shape = engine.Shape(top["waves"], top["chunk"], top["recompute"], top["gamma_global"])
+    shape = _shape_of(top)
     tuning.record(D, n, shape, top["points_per_s"], results)
     if save:
         tuning.save()

@@ -426,9 +426,45 @@ __device__ __forceinline__ void wave_sync()
 // one 128-byte run), while each lane runs its own Thomas row out of LDS.
 // Checkpointed gammas (CK); u' written back between the two passes except for
 // the last staging chunk, which stays in LDS.
-template <int NO, int ORDER, int CK, int C>
+// Row addressing of a staged contiguous-axis tile.  Element q of a lane in a
+// staging chunk is row q*(64/C) + lane/C, column lane%C of the chunk.
+//  TileRows:   64 consecutive lines of one phi starting at `tile`.
+//  PackedRows: row groups of 64/C lines, each group inside one instance
+//              (base[q] = its first row; valid[q] = rows of it in use).
+template <int C>
+struct TileRows {
+    double *tile;
+    int rows_active, n;
+    __device__ __forceinline__ bool ok(int q, int lane, int cw) const
+    {
+        const int row = q * (64 / C) + lane / C, col = lane % C;
+        return row < rows_active && col < cw;
+    }
+    __device__ __forceinline__ double *at(int q, int lane, int j0, bool ok) const
+    {
+        const int row = q * (64 / C) + lane / C, col = lane % C;
+        return tile + (ok ? (long long)row * n + j0 + col : 0);
+    }
+};
+
+template <int C>
+struct PackedRows {
+    double *base[C];
+    int valid[C];
+    int n;
+    __device__ __forceinline__ bool ok(int q, int lane, int cw) const
+    {
+        return lane / C < valid[q] && lane % C < cw;
+    }
+    __device__ __forceinline__ double *at(int q, int lane, int j0, bool ok) const
+    {
+        return base[q] + (ok ? (long long)(lane / C) * n + j0 + lane % C : 0);
+    }
+};
+
+template <int NO, int ORDER, int CK, int C, class Rows>
 __device__ __forceinline__ void solve_tile_staged(
-    double *__restrict__ tile, int rows_active, int n,
+    const Rows &R, int n,
     const double *__restrict__ xInt, const double *__restrict__ df,
     const double *__restrict__ W0, const double *__restrict__ W1,
     const double *__restrict__ S,
@@ -448,9 +484,8 @@ __device__ __forceinline__ void solve_tile_staged(
             double v[C];
 #pragma unroll
             for (int q = 0; q < C; ++q) {
-                const int e = q * 64 + lane, row = e / C, col = e - row * C;
-                const bool ok = row < rows_active && col < cw;
-                v[q] = tile[ok ? (long long)row * n + j0 + col : 0];
+                const bool ok = R.ok(q, lane, cw);
+                v[q] = *R.at(q, lane, j0, ok);
                 v[q] = ok ? v[q] : 0.;
             }
 #pragma unroll
@@ -486,8 +521,8 @@ __device__ __forceinline__ void solve_tile_staged(
             }
 #pragma unroll
             for (int q = 0; q < C; ++q) {
-                const int e = q * 64 + lane, row = e / C, col = e - row * C;
-                if (row < rows_active && col < cw) tile[(long long)row * n + j0 + col] = v[q];
+                const bool ok = R.ok(q, lane, cw);
+                if (ok) *R.at(q, lane, j0, ok) = v[q];
             }
         }
         wave_sync();
@@ -499,9 +534,8 @@ __device__ __forceinline__ void solve_tile_staged(
             double v[C];
 #pragma unroll
             for (int q = 0; q < C; ++q) {
-                const int e = q * 64 + lane, row = e / C, col = e - row * C;
-                const bool ok = row < rows_active && col < cw;
-                v[q] = tile[ok ? (long long)row * n + j0 + col : 0];
+                const bool ok = R.ok(q, lane, cw);
+                v[q] = *R.at(q, lane, j0, ok);
                 v[q] = ok ? v[q] : 0.;
             }
 #pragma unroll
@@ -536,8 +570,8 @@ __device__ __forceinline__ void solve_tile_staged(
             }
 #pragma unroll
             for (int q = 0; q < C; ++q) {
-                const int e = q * 64 + lane, row = e / C, col = e - row * C;
-                if (row < rows_active && col < cw) tile[(long long)row * n + j0 + col] = v[q];
+                const bool ok = R.ok(q, lane, cw);
+                if (ok) *R.at(q, lane, j0, ok) = v[q];
             }
         }
         wave_sync();
@@ -1055,7 +1089,7 @@ __device__ __forceinline__ void sweep_tiles(double *__restrict__ phi, int n, int
                         (n - 1) / CK + 1, stage, lane);
                 else
                     solve_tile_staged<NO, ORDER, CK, STAGE_C>(
-                        phi + L0 * n, rows, n, sxi, sdf, W0, W1, S, L.c, mig, dt, inv_dt,
+                        TileRows<STAGE_C>{phi + L0 * n, rows, n}, n, sxi, sdf, W0, W1, S, L.c, mig, dt, inv_dt,
                         L.bcf_on, L.bcf, L.bcl_on, L.bcl, L.inj_row, inj, gamcol, stage, lane);
             }
             return;
@@ -1161,6 +1195,230 @@ k_integrate_nd(dh_instance *__restrict__ tab, int ninst, int *__restrict__ count
             ip->cur_seg = seg;
             ip->cur_step = step;
             ip->status = seg >= nseg ? 1 : 0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Kernel 1b: persistent, several instances ("slots") per workgroup, stepped in
+// lockstep so that their lines fill the waves (a 2-population phi on 100
+// points has 100 lines per sweep: alone it leaves 28 of 128 lanes idle, three
+// slots fill 300 of 320).  All instances of a launch share D, order, n and the
+// grid.  Each slot runs its own schedule; a finished slot takes the next
+// instance from the queue.  Lines of a slot are padded to a multiple of 8 so
+// that every staging row group of the contiguous axis lies in one instance.
+// Checkpointed Thomas scratch (bet every 8 rows) only.
+// LDS: x, xInt, dfactor, dx (4n) | per slot and axis W0,W1,S (3*S*D*n) |
+//      per wave: checkpoints (64*((n-1)/8+1)) and staging (64*(STAGE_C+1)).
+// ---------------------------------------------------------------------------
+constexpr int DH_PACK_MAX = 16;
+
+// Pointers in the slot table are kept as global-address-space pointers so
+// that phi and schedule accesses through them compile to global memory
+// instructions.
+typedef __attribute__((address_space(1))) double *gdptr;
+typedef const __attribute__((address_space(1))) dh_segment *gsptr;
+
+struct PackSlot {
+    gdptr phi;
+    gsptr sp;
+    long long seg, step, nseg, budget;
+    double dt, inv_dt;
+    int id, frozen, injm, first_axis, live, rebuild, done;
+};
+
+// Points slot `ps` at segment `seg` of instance `ip` (skipping empty ones);
+// returns false when the schedule is exhausted.
+__device__ inline bool pack_enter_segment(PackSlot &ps, const dh_instance *ip)
+{
+    const gsptr segs = (gsptr)gbl(ip->seg_dev);
+    while (ps.seg < ps.nseg && segs[ps.seg].nsteps - ps.step <= 0) {
+        ++ps.seg;
+        ps.step = 0;
+    }
+    if (ps.seg >= ps.nseg) return false;
+    ps.sp = segs + ps.seg;
+    ps.dt = ps.sp->dt;
+    ps.inv_dt = 1. / ps.dt;
+    ps.rebuild = 1;
+    return true;
+}
+
+template <int D, int ORDER>
+__device__ __forceinline__ void sweep_packed(int a, const PackSlot *slot, int S, int n,
+                                             long long L, long long Lp, const double *sx,
+                                             const double *sxi, const double *sdf,
+                                             const double *sdx, const double *atab,
+                                             double *gamcol, double *stage, int wave, int nw,
+                                             int lane)
+{
+    constexpr int NO = D - 1, CK = 8, C = STAGE_C;
+    static_assert(8 % (64 / C) == 0, "row groups must not straddle the 8-line slot padding");
+    long long s = 1;
+    for (int q = a + 1; q < D; ++q) s *= n;
+    const long long span = (long long)S * Lp;
+    const int ntiles = (int)((span + 63) / 64);
+    for (int t = wave; t < ntiles; t += nw) {
+        const long long G = (long long)t * 64 + lane;
+        int k = (int)(G / Lp);
+        if (k > S - 1) k = S - 1;
+        const long long line = G - (long long)k * Lp;
+        const PackSlot &ps = slot[k];
+        const bool act = G < span && line < L && ps.live && !((ps.frozen >> a) & 1);
+        const dh_segment *sp = (const dh_segment *)ps.sp;
+        double mig[NO > 0 ? NO : 1];
+#pragma unroll
+        for (int q = 0; q < NO; ++q) mig[q] = sp->mig[a][q];
+        const double nu = sp->nu[a], ga = sp->gamma[a], ha = sp->h[a], inj = sp->inject;
+        const double dt = ps.dt, inv_dt = ps.inv_dt;
+        const double *W0 = atab + ((long long)k * D + a) * LDS_TABLES_PER_AXIS * n;
+        LineSetup<D, ORDER> Ls;
+        line_setup<D, ORDER>(Ls, act ? line : 0, n, a, sx, mig, nu, ga, ha, sdx[0], sdx[n - 2],
+                             a == ps.first_axis, ps.injm);
+        if (!act) Ls.inj_row = -1;
+        if (a == D - 1) {
+            PackedRows<C> R;
+            R.n = n;
+            bool any = false;
+#pragma unroll
+            for (int q = 0; q < C; ++q) {
+                const long long Gq = (long long)t * 64 + q * (64 / C);
+                int kq = (int)(Gq / Lp);
+                if (kq > S - 1) kq = S - 1;
+                const long long lq = Gq - (long long)kq * Lp;
+                const PackSlot &pq = slot[kq];
+                const bool use = Gq < span && lq < L && pq.live && !((pq.frozen >> a) & 1);
+                const long long rows = L - lq < 64 / C ? L - lq : 64 / C;
+                R.valid[q] = use ? (int)rows : 0;
+                R.base[q] = (double *)pq.phi + (use ? lq * n : 0);
+                any = any || use;
+            }
+            if (!any) continue;
+            solve_tile_staged<NO, ORDER, CK, C>(R, n, sxi, sdf, W0, W0 + n, W0 + 2 * n, Ls.c, mig,
+                                                dt, inv_dt, Ls.bcf_on, Ls.bcf, Ls.bcl_on, Ls.bcl,
+                                                Ls.inj_row, inj, gamcol, stage, lane);
+        } else if (act) {
+            const long long hi = line / s, lo = line - hi * s;
+            double *p = (double *)ps.phi + hi * (long long)n * s + lo;
+            solve_line_ck<NO, ORDER, CK>(p, s, n, sxi, sdf, W0, W0 + n, W0 + 2 * n, 1, Ls.c, mig,
+                                         dt, inv_dt, Ls.bcf_on, Ls.bcf, Ls.bcl_on, Ls.bcl,
+                                         Ls.inj_row, inj, gamcol, 64);
+        }
+    }
+}
+
+template <int D, int ORDER>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(DH_WPE, 8)))
+k_integrate_packed(dh_instance *__restrict__ tab, int ninst, int *__restrict__ counter,
+                   long long max_steps, int S, int n)
+{
+    extern __shared__ double lds[];
+    __shared__ PackSlot slot[DH_PACK_MAX];
+    __shared__ int s_live;
+    const int nthr = blockDim.x, tid = threadIdx.x;
+    const int nw = nthr >> 6, wave = tid >> 6, lane = tid & 63;
+    long long L = 1;
+    for (int q = 0; q < D - 1; ++q) L *= n;
+    const long long Lp = (L + 7) & ~7LL;
+    double *sx = lds, *sxi = sx + n, *sdf = sxi + n, *sdx = sdf + n;
+    double *atab = sdx + n;
+    const int col = (n - 1) / 8 + 1;
+    double *scr = atab + (long long)S * D * LDS_TABLES_PER_AXIS * n;
+    double *gamcol = scr + (long long)wave * 64 * col + lane;
+    double *stage = scr + (long long)nw * 64 * col + (long long)wave * 64 * (STAGE_C + 1);
+    build_grid_tables(gbl(tab[0].xx_dev), n, sx, sxi, sdf, sdx, tid, nthr);
+    if (tid < S) {
+        // an idle slot still points at valid memory: its lanes compute on
+        // padding and never store
+        PackSlot &ps = slot[tid];
+        ps.phi = (gdptr)gbl(tab[0].phi_dev);
+        ps.sp = (gsptr)gbl(tab[0].seg_dev);
+        ps.dt = 1.;
+        ps.inv_dt = 1.;
+        ps.live = 0;
+        ps.rebuild = 0;
+        ps.done = 0;
+        ps.id = -1;
+        ps.frozen = 0;
+        ps.injm = 0;
+        ps.first_axis = 0;
+    }
+    const long long budget0 = max_steps > 0 ? max_steps : LLONG_MAX;
+    for (;;) {
+        __syncthreads();
+        if (tid == 0) {
+            int live = 0;
+            for (int k = 0; k < S; ++k) {
+                PackSlot &ps = slot[k];
+                while (!ps.live && !ps.done) {
+                    const int id = atomicAdd(counter, 1);
+                    if (id >= ninst) {
+                        ps.done = 1;
+                        break;
+                    }
+                    dh_instance *ip = tab + id;
+                    ps.id = id;
+                    ps.seg = ip->cur_seg;
+                    ps.step = ip->cur_step;
+                    ps.nseg = ip->nseg;
+                    ps.budget = budget0;
+                    ps.frozen = ip->frozen;
+                    ps.injm = ip->inject;
+                    int fa = 0;
+                    while (fa < D && ((ps.frozen >> fa) & 1)) ++fa;
+                    ps.first_axis = fa;
+                    if (fa == D) {   // every axis frozen: the schedule is the identity
+                        ps.seg = ps.nseg;
+                        ps.step = 0;
+                    }
+                    if (fa < D && pack_enter_segment(ps, ip)) {
+                        ps.phi = (gdptr)gbl(ip->phi_dev);
+                        ps.live = 1;
+                    } else {
+                        ip->cur_seg = ps.seg;
+                        ip->cur_step = ps.step;
+                        ip->status = ps.seg >= ps.nseg ? 1 : 0;
+                    }
+                }
+                live += ps.live;
+            }
+            s_live = live;
+        }
+        __syncthreads();
+        if (s_live == 0) break;
+        for (int k = 0; k < S; ++k) {
+            if (!slot[k].rebuild) continue;
+            const dh_segment *sp = (const dh_segment *)slot[k].sp;
+            for (int a = 0; a < D; ++a) {
+                if ((slot[k].frozen >> a) & 1) continue;
+                double *W0 = atab + ((long long)k * D + a) * LDS_TABLES_PER_AXIS * n;
+                build_axis_tables<D>(sp, a, n, sx, sxi, sdx, W0, W0 + n, W0 + 2 * n, tid, nthr);
+            }
+        }
+        __syncthreads();
+        if (tid < S) slot[tid].rebuild = 0;
+        for (int a = 0; a < D; ++a) {
+            sweep_packed<D, ORDER>(a, slot, S, n, L, Lp, sx, sxi, sdf, sdx, atab, gamcol, stage,
+                                   wave, nw, lane);
+            __syncthreads();
+        }
+        if (tid < S && slot[tid].live) {
+            PackSlot &ps = slot[tid];
+            dh_instance *ip = tab + ps.id;
+            ++ps.step;
+            --ps.budget;
+            bool more = true;
+            if (ps.step >= ps.sp->nsteps) {
+                ++ps.seg;
+                ps.step = 0;
+                more = pack_enter_segment(ps, ip);
+            }
+            if (!more || ps.budget <= 0) {
+                ip->cur_seg = ps.seg;
+                ip->cur_step = ps.step;
+                ip->status = ps.seg >= ps.nseg ? 1 : 0;
+                ps.live = 0;
+            }
         }
     }
 }
@@ -1358,6 +1616,34 @@ const void *pick_nd(int D, int order, int chunk, int recompute, int gg)
 }
 
 template <int D, int ORDER>
+const void *pk_kernel() { return reinterpret_cast<const void *>(&k_integrate_packed<D, ORDER>); }
+
+const void *pick_packed(int D, int order)
+{
+    switch (D * 4 + order) {
+#define DH_CASE(d, o) case d * 4 + o: return pk_kernel<d, o>();
+        DH_CASE(2, 0) DH_CASE(2, 1) DH_CASE(2, 2)
+        DH_CASE(3, 0) DH_CASE(3, 1) DH_CASE(3, 2)
+        DH_CASE(4, 0) DH_CASE(4, 1) DH_CASE(4, 2)
+        DH_CASE(5, 0) DH_CASE(5, 1) DH_CASE(5, 2)
+#undef DH_CASE
+    }
+    return nullptr;
+}
+
+// Statically allocated LDS of the packed kernel (slot table and flag).
+constexpr size_t PACK_STATIC_LDS = sizeof(PackSlot) * DH_PACK_MAX + 16;
+
+// Dynamic LDS of the packed kernel.
+size_t packed_dyn_lds(int D, int n, int waves, int slots)
+{
+    const size_t col = (size_t)((n - 1) / 8 + 1);
+    return sizeof(double) *
+           ((size_t)n * (LDS_GRID_TABLES + LDS_TABLES_PER_AXIS * (size_t)D * (size_t)slots) +
+            (size_t)waves * 64 * (col + STAGE_C + 1));
+}
+
+template <int D, int ORDER>
 const void *sw_kernel() { return reinterpret_cast<const void *>(&k_sweep_axis<D, ORDER>); }
 
 const void *pick_sweep(int D, int order)
@@ -1452,6 +1738,10 @@ int dh_nd_groups(int D, int nmax, int waves, int chunk, int recompute, int gamma
                      ninst);
 }
 size_t dh_max_lds(void) { return g_max_lds; }
+size_t dh_packed_lds_bytes(int D, int n, int waves, int slots)
+{
+    return packed_dyn_lds(D, n, waves, slots) + PACK_STATIC_LDS;
+}
 
 int dh_last_error(char *buf, int len)
 {
@@ -1509,6 +1799,37 @@ int dh_integrate_nd(dh_instance *table_dev, int ninst, int D, int order, int nma
     double *gws = cfg->gamma_dev;
     void *args[] = {&table_dev, &ninst, &counter_dev, &ms, &gws, &nmax};
     DH_HIP(hipLaunchKernel(fn, dim3(groups), dim3(64 * waves), args, lds, st));
+    return DH_OK;
+}
+
+int dh_integrate_packed(dh_instance *table_dev, int ninst, int D, int order, int n, int slots,
+                        int *counter_dev, const dh_launch *cfg, void *stream)
+{
+    if (int e = check_init()) return e;
+    if (D < 2 || D > DH_MAXD) return fail(DH_EINVAL, "packed launches need D = 2..5");
+    if (order < 0 || order > 2) return fail(DH_EINVAL, "bad order");
+    if (ninst <= 0) return DH_OK;
+    if (!table_dev || !counter_dev || !cfg) return fail(DH_EINVAL, "null pointer");
+    if (n < 3) return fail(DH_EINVAL, "n must be >= 3");
+    if (slots < 1 || slots > DH_PACK_MAX)
+        return fail(DH_EINVAL, "slots must be 1.." + std::to_string(DH_PACK_MAX));
+    const int waves = cfg->waves;
+    if (waves < 1 || waves > 8) return fail(DH_EINVAL, "waves must be 1..8");
+    if (cfg->chunk != 8 || cfg->recompute || cfg->gamma_dev)
+        return fail(DH_EINVAL, "packed launches use chunk 8 without recompute or workspace");
+    const void *fn = pick_packed(D, order);
+    const size_t dyn = packed_dyn_lds(D, n, waves, slots);
+    if (dyn + PACK_STATIC_LDS > g_max_lds)
+        return fail(DH_ERANGE, "LDS request of " + std::to_string(dyn + PACK_STATIC_LDS) +
+                                   " bytes exceeds " + std::to_string(g_max_lds));
+    if (int e = set_lds(fn, dyn)) return e;
+    const int groups =
+        nd_groups(dyn + PACK_STATIC_LDS, waves, cfg->groups, (ninst + slots - 1) / slots);
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, counter_dev, 0);
+    long long ms = cfg->max_steps;
+    void *args[] = {&table_dev, &ninst, &counter_dev, &ms, &slots, &n};
+    DH_HIP(hipLaunchKernel(fn, dim3(groups), dim3(64 * waves), args, dyn, st));
     return DH_OK;
 }
 

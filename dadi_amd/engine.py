@@ -64,6 +64,14 @@ class Context:
             self._xx_cache[key] = g
         return g
 
+    def side_streams(self, k):
+        """k streams (default priority) for launches that run side by side."""
+        t = torch()
+        pool = self.__dict__.setdefault("_side", [])
+        while len(pool) < k:
+            pool.append(t.cuda.Stream(self.device))
+        return pool[:k]
+
     def max_waves(self, D, n, chunk=0, recompute=0, gamma_global=0):
         """Largest waves-per-workgroup whose LDS fits (mirrors dh_lds_bytes)."""
         L = self.lib
@@ -71,17 +79,67 @@ class Context:
         per_wave = L.dh_lds_bytes(D, n, 1, chunk, recompute, gamma_global) - fixed
         return max(0, min(8, (self.max_lds - fixed) // per_wave))
 
-    def shape(self, D, n, waves=None, chunk=None, recompute=None, gamma_global=None):
-        """Launch shape of the nD kernel for instances of D populations on n
+    #: resident waves per compute unit the nD kernels reach (register-limited:
+    #: 3 per SIMD at their register budget)
+    RESIDENT_WAVES = 12
+
+    def _occupancy(self, lds, waves):
+        per_cu = min(self.max_lds // max(lds, 1), 16 // waves + 1)
+        return min(per_cu * waves, self.RESIDENT_WAVES)
+
+    def pack_plan(self, D, n, waves=None, slots=None):
+        """(waves, slots, score) of the packed kernel for instances of D
+        populations on n points, maximising busy lanes x resident waves per
+        compute unit; None when nothing fits."""
+        L = n ** (D - 1)
+        Lp = -(-L // 8) * 8
+        best = None
+        for W in ([waves] if waves else range(1, 9)):
+            S = slots if slots else min(16, (64 * W) // Lp)
+            if S < 1:
+                continue
+            lds = self.lib.dh_packed_lds_bytes(D, n, W, S)
+            if lds > self.max_lds:
+                continue
+            tiles = -(-(S * Lp) // 64)
+            rounds = -(-tiles // W)
+            util = S * L / (64.0 * W * rounds)
+            score = util * self._occupancy(lds, W)
+            if best is None or score > best[2] + 1e-9:
+                best = (W, S, score)
+        return best
+
+    def _plain_score(self, D, n, sh):
+        L = n ** (D - 1)
+        lds = self.lib.dh_lds_bytes(D, n, sh.waves, sh.chunk, sh.recompute, sh.gamma_global)
+        util = L / (64.0 * sh.waves * -(-L // (64 * sh.waves)))
+        return util * self._occupancy(lds, sh.waves)
+
+    def shape(self, D, n, waves=None, chunk=None, recompute=None, gamma_global=None,
+              slots=None, packable=False):
+        """Launch shape of the nD kernels for instances of D populations on n
         points: the autotuned choice when one is recorded, else a rule of
-        thumb.  Returns Shape(waves, chunk, recompute, gamma_global)."""
+        thumb.  ``packable``: the instances share n and grid, so several may
+        share a workgroup (slots >= 2, the packed kernel).
+        Returns Shape(waves, chunk, recompute, gamma_global, slots)."""
+        if slots is not None and slots >= 2:
+            if not packable:
+                raise ValueError("packing needs instances that share n and the grid")
+            plan = self.pack_plan(D, n, waves, slots)
+            if plan is None:
+                raise ValueError("%d slots of n=%d do not fit a workgroup" % (slots, n))
+            return Shape(plan[0], 8, 0, 0, plan[1])
+        free = waves is None and chunk is None and recompute is None and gamma_global is None
         if None in (waves, chunk, recompute, gamma_global) and self.tuning is not None:
-            hit = self.tuning.lookup(D, n)
+            hit = self.tuning.lookup(D, n, packable=packable and free and slots is None)
+            if hit is not None and hit.slots >= 2:
+                return self.shape(D, n, hit.waves, slots=hit.slots, packable=True)
             if hit is not None:
                 waves = hit.waves if waves is None else waves
                 chunk = hit.chunk if chunk is None else chunk
                 recompute = hit.recompute if recompute is None else recompute
                 gamma_global = hit.gamma_global if gamma_global is None else gamma_global
+                slots = 1
         gamma_global = int(bool(gamma_global))
         if gamma_global:
             chunk, recompute = 0, 0
@@ -98,21 +156,34 @@ class Context:
             wmax = self.max_waves(D, n, chunk, recompute)
         if wmax < 1:
             raise ValueError("n=%d too large for the workgroup kernel at D=%d" % (n, D))
+        auto_waves = waves is None
         if waves is None:
             # large instances get up to 8 waves (lower latency per step);
             # small ones few, so more instances share a compute unit
             cap = 8 if nsys >= 1024 else 4
             waves = max(1, min(wmax, cap, math.ceil(nsys / 64)))
-        return Shape(min(waves, wmax), chunk, recompute, gamma_global)
+        sh = Shape(min(waves, wmax), chunk, recompute, gamma_global, 1)
+        if packable and slots is None and auto_waves and D >= 2 and chunk == 8 and \
+                not recompute and not gamma_global:
+            plan = self.pack_plan(D, n)
+            if plan is not None and plan[1] >= 2 and plan[2] > 1.05 * self._plain_score(D, n, sh):
+                return Shape(plan[0], 8, 0, 0, plan[1])
+        return sh
 
     def nd_launch(self, tab_ptr, ninst, D, order, nmax, counter_ptr, stream, workspace,
                   waves=None, groups=0, max_steps=0, chunk=None, recompute=None,
-                  gamma_global=None):
-        """One dh_integrate_nd call.  ``workspace`` is a dict owned by the
-        caller (one per stream: launches that may overlap must not share it)
-        that keeps the gamma workspace between calls."""
-        sh = self.shape(D, nmax, waves, chunk, recompute, gamma_global)
+                  gamma_global=None, slots=None, packable=False):
+        """One dh_integrate_nd (or, with slots >= 2, dh_integrate_packed)
+        call.  ``workspace`` is a dict owned by the caller (one per stream:
+        launches that may overlap must not share it) that keeps the gamma
+        workspace between calls.  ``packable``: every instance has n = nmax
+        and the same grid."""
+        sh = self.shape(D, nmax, waves, chunk, recompute, gamma_global, slots, packable)
         cfg = DhLaunch(sh.waves, groups, max_steps, sh.chunk, sh.recompute)
+        if sh.slots >= 2:
+            check(self.lib.dh_integrate_packed(tab_ptr, ninst, D, order, nmax, sh.slots,
+                                               counter_ptr, cfg, stream), "dh_integrate_packed")
+            return sh
         if sh.gamma_global:
             g = self.lib.dh_nd_groups(D, nmax, sh.waves, sh.chunk, sh.recompute, 1, groups, ninst)
             check(min(g, 0), "dh_nd_groups")
@@ -129,18 +200,21 @@ class Context:
 
 
 class Shape(tuple):
-    """Launch shape of the nD kernel: waves per workgroup, Thomas chunk,
-    recompute flag, gamma columns in a global workspace."""
+    """Launch shape of the nD kernels: waves per workgroup, Thomas chunk,
+    recompute flag, gamma columns in a global workspace, instances per
+    workgroup (slots >= 2: the packed kernel)."""
     __slots__ = ()
-    _fields = ("waves", "chunk", "recompute", "gamma_global")
+    _fields = ("waves", "chunk", "recompute", "gamma_global", "slots")
 
-    def __new__(cls, waves, chunk, recompute, gamma_global=0):
-        return tuple.__new__(cls, (int(waves), int(chunk), int(recompute), int(gamma_global)))
+    def __new__(cls, waves, chunk, recompute, gamma_global=0, slots=1):
+        return tuple.__new__(cls, (int(waves), int(chunk), int(recompute), int(gamma_global),
+                                   int(slots)))
 
     waves = property(lambda self: self[0])
     chunk = property(lambda self: self[1])
     recompute = property(lambda self: self[2])
     gamma_global = property(lambda self: self[3])
+    slots = property(lambda self: self[4])
 
 
 def context(device=0):
@@ -291,6 +365,9 @@ class PackedBatch:
         self.counter = t.zeros(1, dtype=t.int32, device=ctx.device)
         self.nmax = max(j.n for j in jobs)
         self.workspace = {}
+        # several instances may share a workgroup when they share n and grid
+        self.packable = self.D >= 2 and len({int(r) for r in tab["xx"]}) == 1 and \
+            all(j.n == self.nmax for j in jobs)
 
     def restart(self):
         """Reset every instance's progress to the start of its schedule
@@ -298,7 +375,10 @@ class PackedBatch:
         self.tab.copy_(self.tab0)
 
     def launch(self, waves=None, groups=0, max_steps=0, stream=None, chunk=None,
-               recompute=None, gamma_global=None):
+               recompute=None, gamma_global=None, slots=None):
+        """One persistent launch over the table (max_steps > 0: at most that
+        many steps per instance).  Shape arguments left None are autotuned /
+        chosen by rule of thumb; slots=1 forbids packing."""
         ctx = self.ctx
         stream = ctx.stream() if stream is None else stream
         if self.D == 1:
@@ -316,9 +396,10 @@ class PackedBatch:
                                           self.nmax, self.counter.data_ptr(), scr.data_ptr(),
                                           scr.numel(), cfg, stream), "dh_integrate_1d")
         else:
-            ctx.nd_launch(self.tab.data_ptr(), len(self.jobs), self.D, self.order, self.nmax,
-                          self.counter.data_ptr(), stream, self.workspace, waves, groups,
-                          max_steps, chunk, recompute, gamma_global)
+            return ctx.nd_launch(self.tab.data_ptr(), len(self.jobs), self.D, self.order,
+                                 self.nmax, self.counter.data_ptr(), stream, self.workspace,
+                                 waves, groups, max_steps, chunk, recompute, gamma_global,
+                                 slots, self.packable)
 
     def progress(self):
         raw = self.tab.cpu().numpy().view(INST_DTYPE)
@@ -326,23 +407,41 @@ class PackedBatch:
 
 
 def run_batch(jobs, ctx=None, waves=None, groups=0, sort=True, chunk=None, recompute=None,
-              gamma_global=None):
-    """Run every job to completion; jobs are grouped by (D, order) and each
-    group runs as one persistent launch (longest job first)."""
+              gamma_global=None, slots=None):
+    """Run every job to completion.  Jobs are grouped by (D, order); a group
+    of several populations is split further by (n, grid) so that each part
+    can pack several instances per workgroup, the parts running side by side
+    on their own streams.  Each part is one persistent launch, longest job
+    first."""
+    t = torch()
     ctx = ctx or context()
     groups_by = {}
     for j in jobs:
         groups_by.setdefault((j.D, j.order), []).append(j)
-    packs = []
+    parts = []
     for key in sorted(groups_by):
         js = groups_by[key]
+        if key[0] >= 2 and slots != 1:
+            by = {}
+            for j in js:
+                by.setdefault((j.n, ctx.grid(j.xx).data_ptr()), []).append(j)
+            parts.extend(by[k] for k in sorted(by))
+        else:
+            parts.append(js)
+    cur = t.cuda.current_stream(ctx.device)
+    streams = ctx.side_streams(len(parts)) if len(parts) > 1 else [cur]
+    packs = []
+    for js, st in zip(parts, streams):
         if sort:
             js = sorted(js, key=lambda j: -j.work())
         pb = PackedBatch(ctx, js)
+        if st is not cur:
+            st.wait_stream(cur)
         pb.launch(waves=waves, groups=groups, chunk=chunk, recompute=recompute,
-                  gamma_global=gamma_global)
+                  gamma_global=gamma_global, slots=slots, stream=st.cuda_stream)
         packs.append(pb)
-    torch().cuda.current_stream(ctx.device).synchronize()
+    for st in streams:
+        st.synchronize()
     return packs
 
 

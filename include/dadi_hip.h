@@ -132,6 +132,21 @@ int dh_integrate_nd(dh_instance *table_dev, int ninst, int D, int order, int nma
                     int *counter_dev, const dh_launch *cfg, void *stream);
 
 /*
+ * Like dh_integrate_nd, with `slots` instances per workgroup stepped in
+ * lockstep so that their tridiagonal lines fill the waves (a 2-population phi
+ * on 100 points has 100 lines per sweep; three of them fill 300 of 320 lanes
+ * of a 5-wave workgroup).  Every instance must have D populations, n points,
+ * the same order and the same grid (xx); each keeps its own schedule, and a
+ * finished slot takes the next instance from the queue.  cfg->chunk must be
+ * 8 (checkpointed gammas) with no recompute and no workspace.  Same results,
+ * bit for bit, as dh_integrate_nd.
+ */
+int dh_integrate_packed(dh_instance *table_dev, int ninst, int D, int order, int n, int slots,
+                        int *counter_dev, const dh_launch *cfg, void *stream);
+/* LDS per workgroup of dh_integrate_packed, bytes. */
+size_t dh_packed_lds_bytes(int D, int n, int waves, int slots);
+
+/*
  * Runs ninst 1-population instances, one lane per instance, lanes pulling
  * instances from a device-side queue independently (lane-level continuous
  * batching).  scratch_dev must hold 6 * nmax * lanes doubles where

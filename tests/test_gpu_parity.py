@@ -172,13 +172,15 @@ def test_split_mig_extrapolated_spectrum_and_ll(dadi_amd, oracle):
     assert abs(ll_gpu - ll_ref) <= 1e-9
 
 
-@pytest.mark.parametrize("chunk,recompute,gg", [(0, 0, 0), (8, 0, 0), (8, 1, 0), (0, 0, 1)])
+@pytest.mark.parametrize("chunk,recompute,gg,slots", [(0, 0, 0, 1), (8, 0, 0, 1), (8, 1, 0, 1),
+                                                     (0, 0, 1, 1), (8, 0, 0, 2), (8, 0, 0, 3)])
 @pytest.mark.parametrize("waves", [1, 2])
-def test_batched_variants_bit_exact_on_golden(dadi_amd, chunk, recompute, gg, waves):
+def test_batched_variants_bit_exact_on_golden(dadi_amd, chunk, recompute, gg, slots, waves):
     """Every golden case in one batch per (D, order), with the full gamma
     column in LDS (chunk 0), checkpointed gammas (chunk 8, with u' stored or
-    recomputed) and gamma columns in a global workspace, at 1 and 2 waves per
-    workgroup: identical bits to the reference kernels."""
+    recomputed), gamma columns in a global workspace, and 2 or 3 instances
+    per workgroup (packed kernel), at 1 and 2 waves per workgroup: identical
+    bits to the reference kernels."""
     from dadi_amd import engine
     ctx = engine.context()
     jobs, want = [], []
@@ -191,6 +193,82 @@ def test_batched_variants_bit_exact_on_golden(dadi_amd, chunk, recompute, gg, wa
             seg, order, fz, inj = G.segments(c, xx, which)
             jobs.append(engine.Job(ctx.to_dev(phi0), xx, seg, order, fz, inj))
             want.append(w[which])
-    engine.run_batch(jobs, waves=waves, chunk=chunk, recompute=recompute, gamma_global=gg)
+    if slots >= 2:
+        engine.run_batch(jobs, waves=waves, slots=slots)
+    else:
+        engine.run_batch(jobs, waves=waves, chunk=chunk, recompute=recompute, gamma_global=gg,
+                         slots=1)
     for j, w in zip(jobs, want):
         assert np.array_equal(j.phi.cpu().numpy(), w)
+
+
+@pytest.mark.parametrize("D,n,order", [(2, 40, 1), (2, 100, 1), (2, 45, 0), (3, 12, 1)])
+def test_packed_slots_refill_and_quanta(dadi_amd, oracle, D, n, order):
+    """The packed kernel (several instances per workgroup in lockstep, each on
+    its own schedule, slots refilled from the queue) gives the unpacked
+    kernel's bits, also when run in quanta of a few steps; spot checks
+    against the oracle.  Some instances have a frozen population, step counts
+    differ by two orders of magnitude."""
+    import torch
+    from dadi_amd import engine, schedule as S, _lib
+    from dadi_amd.Misc import ensure_1arg_func as f1
+    ctx = engine.context()
+    rng = np.random.default_rng(100 * D + n)
+    xx = oracle.default_grid(n)
+    phi0 = oracle.phi_1D(xx)
+    for _ in range(D - 1):
+        phi0 = oracle.phi_1D_to_2D(xx, phi0) if phi0.ndim == 1 else \
+            oracle.phi_2D_to_3D_split_2(xx, phi0)
+    specs = []
+    for i in range(37):
+        nus = list(np.exp(rng.uniform(np.log(0.2), np.log(5), D)))
+        migs = [list(rng.uniform(0, 3, D - 1)) for _ in range(D)]
+        gammas = list(rng.uniform(-3, 3, D))
+        T = float(np.exp(rng.uniform(np.log(0.002), np.log(0.3))))
+        frozen = [False] * D
+        if i % 5 == 2:
+            frozen[0] = True
+            for k in range(D):
+                migs[k] = [0.0] * (D - 1)
+        fz, inj = S.masks(D, frozen, [False] * D)
+        if order == _lib.ORDER_TV:
+            seg = S.tv_segments(D, xx, 0.0, T, [f1(v) for v in nus],
+                                [[f1(m) for m in ms] for ms in migs], [f1(g) for g in gammas],
+                                [f1(0.5)] * D, f1(1.0), None)
+        else:
+            dt = min(S.compute_dt(nus[k], migs[k], gammas[k], 0.5) for k in range(D))
+            seg = S.const_segments(D, xx, 0.0, T, dt, nus, migs, gammas, [0.5] * D, 1.0)
+        specs.append((seg, fz, inj, (T, nus, migs, gammas, frozen)))
+
+    def batch():
+        return [engine.Job(ctx.to_dev(phi0), xx, seg, order, fz, inj)
+                for seg, fz, inj, _ in specs]
+
+    plain = batch()
+    engine.run_batch(plain, slots=1)
+    want = [j.phi.cpu().numpy() for j in plain]
+    plan = ctx.pack_plan(D, n)
+    assert plan is not None
+    W, Smax = plan[0], max(2, plan[1])
+    for slots in sorted({2, Smax}):
+        packed = batch()
+        engine.run_batch(packed, waves=W, slots=slots)
+        for i, (j, w) in enumerate(zip(packed, want)):
+            assert np.array_equal(j.phi.cpu().numpy(), w), (slots, i)
+    # quanta: at most 23 steps per instance per launch until every one is done
+    jobs = batch()
+    pb = engine.PackedBatch(ctx, jobs)
+    for _ in range(100000):
+        pb.launch(waves=W, slots=Smax, max_steps=23)
+        torch.cuda.synchronize()
+        if np.all(pb.progress()["status"] == 1):
+            break
+    for i, (j, w) in enumerate(zip(jobs, want)):
+        assert np.array_equal(j.phi.cpu().numpy(), w), ("quanta", i)
+    # the unpacked result itself against the oracle, one free and one frozen
+    if order == _lib.ORDER_PRE:
+        for i in (0, 2):
+            T, nus, migs, gammas, frozen = specs[i][3]
+            ref = oracle.integrate_const(phi0, xx, T, nus, migs, gammas, [0.5] * D,
+                                         frozen=frozen)
+            assert np.array_equal(want[i], ref), i
