@@ -75,16 +75,18 @@ def cpu_baseline(params, pts, budget_s):
                       % (done, pts, steps, el)}
 
 
-def measured_traffic(pts, batch, seed):
-    """HBM bytes per launch of this exact workload, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (each counter scaled by what it
-    reads for a device copy of known size); None when no summary matches."""
-    path = os.path.join(ROOT, "profiles", "traffic_k_integrate_nd_2d.json")
+def measured_traffic(pts, batch, seed, kernel):
+    """HBM bytes per launch of this exact workload and kernel, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (each counter scaled
+    by what it reads for a device copy of known size); None when no summary
+    matches."""
+    path = os.path.join(ROOT, "profiles", "traffic_split_mig_2d_pts%d.json" % pts)
     try:
         rec = json.load(open(path))
     except (OSError, ValueError):
         return None
-    if rec.get("pts") != pts or rec.get("batch") != batch or rec.get("seed") != seed:
+    if rec.get("pts") != pts or rec.get("batch") != batch or rec.get("seed") != seed or \
+            rec.get("kernel") != kernel:
         return None
     return (rec["fetch_kib"] * rec["fetch_scale"] + rec["write_kib"] * rec["write_scale"]) * 1024
 
@@ -103,6 +105,8 @@ def main():
     ap.add_argument("--gamma-global", type=int, default=-1)
     ap.add_argument("--slots", type=int, default=-1,
                     help="instances per workgroup (1 = unpacked kernel; -1 = automatic)")
+    ap.add_argument("--dense", action="store_true",
+                    help="store phi rows unpadded (default: rows padded to 64-byte multiples)")
     ap.add_argument("--groups", type=int, default=0,
                     help="persistent workgroups (0 = as many as fit)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -141,7 +145,11 @@ def main():
     xx, phi0, segs = build_jobs(dadi_amd, params, args.pts)
     n = args.pts
     phi0d = ctx.to_dev(phi0)
-    arena = torch.empty((args.batch, n, n), dtype=torch.float64, device=ctx.device)
+    # phi rows padded to a multiple of 8 doubles (64-byte aligned rows)
+    if args.dense:
+        arena = torch.empty((args.batch, n, n), dtype=torch.float64, device=ctx.device)
+    else:
+        arena = ctx.empty_phi((n, n), batch=args.batch)
     arena.copy_(phi0d.expand(args.batch, n, n))
     jobs = [engine.Job(arena[i], xx, segs[i], _lib.ORDER_PRE, 0, 3) for i in range(args.batch)]
     order = sorted(range(args.batch), key=lambda i: -jobs[i].work())
@@ -219,7 +227,10 @@ def main():
         cpu = cpu_baseline(params, n, args.cpu_seconds)
 
     achieved = alg_bytes / kavg
-    traffic = measured_traffic(n, args.batch, args.seed)
+    kname = ("k_integrate_packed<2, 1>" if shape.slots >= 2 else
+             "k_integrate_nd<2, 1, %d, %d, %d>" % (shape.chunk, shape.recompute,
+                                                  shape.gamma_global))
+    traffic = None if args.dense else measured_traffic(n, args.batch, args.seed, kname)
     out = {
         "metric": "2D split_mig integrations/s at pts=100 FP64, batched vs plain; p99 req latency",
         "value": value,
@@ -241,7 +252,8 @@ def main():
                    "waves_per_workgroup": shape.waves, "thomas_chunk": shape.chunk,
                    "thomas_recompute": shape.recompute,
                    "thomas_gamma_global": shape.gamma_global,
-                   "instances_per_workgroup": shape.slots},
+                   "instances_per_workgroup": shape.slots,
+                   "row_pitch": engine.pitch_of(arena[0]) or n},
         "plain": {"value": plain_rate, "unit": "integrations/s", "calls": len(lat),
                   "time_steps": plain_steps,
                   "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3) if lat else None,
@@ -250,10 +262,7 @@ def main():
         "serving": serving,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": traffic,
-                     "kernel": ("k_integrate_packed<2, 1>" if shape.slots >= 2 else
-                                "k_integrate_nd<2, 1, %d, %d, %d>" % (
-                                    shape.chunk, shape.recompute, shape.gamma_global)),
-                     "kernel_ms_avg": kavg * 1e3,
+                     "kernel": kname, "kernel_ms_avg": kavg * 1e3,
                      "algorithmic_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
     }

@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("DADI_AMD_LIB") or os.path.join(HERE, "libdadi_hip.so")
 
 MAXD = 5
-ABI_VERSION = 2
+ABI_VERSION = 3
 ORDER_TV, ORDER_PRE, ORDER_PRE1 = 0, 1, 2
 
 #: numpy mirror of ``dh_segment``
@@ -33,7 +33,7 @@ INST_DTYPE = np.dtype([
     ("phi", "<u8"), ("xx", "<u8"), ("seg", "<u8"),
     ("nseg", "<i8"), ("cur_seg", "<i8"), ("cur_step", "<i8"),
     ("D", "<i4"), ("n", "<i4"), ("order", "<i4"), ("frozen", "<i4"),
-    ("inject", "<i4"), ("status", "<i4"),
+    ("inject", "<i4"), ("status", "<i4"), ("ld", "<i4"), ("reserved", "<i4"),
 ])
 
 
@@ -50,7 +50,8 @@ class DhInstance(ctypes.Structure):
                 ("seg_dev", ctypes.c_void_p), ("nseg", ctypes.c_longlong),
                 ("cur_seg", ctypes.c_longlong), ("cur_step", ctypes.c_longlong),
                 ("D", ctypes.c_int), ("n", ctypes.c_int), ("order", ctypes.c_int),
-                ("frozen", ctypes.c_int), ("inject", ctypes.c_int), ("status", ctypes.c_int)]
+                ("frozen", ctypes.c_int), ("inject", ctypes.c_int), ("status", ctypes.c_int),
+                ("ld", ctypes.c_int), ("reserved", ctypes.c_int)]
 
 
 class DhLaunch(ctypes.Structure):

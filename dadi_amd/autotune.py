@@ -99,7 +99,7 @@ def _synthetic_jobs(ctx, D, n, count, steps, rng):
     import torch
     xx = np.asarray(__import__("dadi_amd").Numerics.default_grid(n))
     order = _lib.ORDER_PRE if 2 <= D <= 3 else (_lib.ORDER_PRE1 if D == 1 else _lib.ORDER_TV)
-    arena = torch.empty((count,) + (n,) * D, dtype=torch.float64, device=ctx.device)
+    arena = ctx.empty_phi((n,) * D, batch=count)
     arena.uniform_(0.5, 1.5)
     jobs = []
     for i in range(count):
@@ -139,7 +139,7 @@ def tune(D, n, ctx=None, steps=30, count=None, reps=2, tuning=None, save=False):
     ctx = ctx or engine.context()
     tuning = tuning or ctx.tuning or Tuning.load()
     rng = np.random.default_rng(1000 * D + n)
-    count = count or 4 * ctx.num_cu
+    count = count or 12 * ctx.num_cu   # enough to fill the GPU with packed workgroups too
     arena, jobs = _synthetic_jobs(ctx, D, n, count, steps, rng)
     pb = engine.PackedBatch(ctx, jobs)
     work = sum(int(np.sum(j.seg["nsteps"])) for j in jobs) * (n ** D) * D

@@ -434,7 +434,8 @@ __device__ __forceinline__ void wave_sync()
 template <int C>
 struct TileRows {
     double *tile;
-    int rows_active, n;
+    int rows_active;
+    long long ld;   // row pitch
     __device__ __forceinline__ bool ok(int q, int lane, int cw) const
     {
         const int row = q * (64 / C) + lane / C, col = lane % C;
@@ -443,7 +444,7 @@ struct TileRows {
     __device__ __forceinline__ double *at(int q, int lane, int j0, bool ok) const
     {
         const int row = q * (64 / C) + lane / C, col = lane % C;
-        return tile + (ok ? (long long)row * n + j0 + col : 0);
+        return tile + (ok ? (long long)row * ld + j0 + col : 0);
     }
 };
 
@@ -451,14 +452,14 @@ template <int C>
 struct PackedRows {
     double *base[C];
     int valid[C];
-    int n;
+    long long ld;   // row pitch
     __device__ __forceinline__ bool ok(int q, int lane, int cw) const
     {
         return lane / C < valid[q] && lane % C < cw;
     }
     __device__ __forceinline__ double *at(int q, int lane, int j0, bool ok) const
     {
-        return base[q] + (ok ? (long long)(lane / C) * n + j0 + lane % C : 0);
+        return base[q] + (ok ? (long long)(lane / C) * ld + j0 + lane % C : 0);
     }
 };
 
@@ -584,7 +585,7 @@ __device__ __forceinline__ void solve_tile_staged(
 // The last staging chunk stays in LDS between the two passes.
 template <int NO, int ORDER, int C>
 __device__ __forceinline__ void solve_tile_staged_gg(
-    double *__restrict__ tile, int rows_active, int n,
+    double *__restrict__ tile, int rows_active, int n, long long ld,
     const double *__restrict__ xInt, const double *__restrict__ df,
     const double *__restrict__ W0, const double *__restrict__ W1,
     const double *__restrict__ S,
@@ -605,7 +606,7 @@ __device__ __forceinline__ void solve_tile_staged_gg(
             for (int q = 0; q < C; ++q) {
                 const int e = q * 64 + lane, row = e / C, col = e - row * C;
                 const bool ok = row < rows_active && col < cw;
-                v[q] = tile[ok ? (long long)row * n + j0 + col : 0];
+                v[q] = tile[ok ? (long long)row * ld + j0 + col : 0];
                 v[q] = ok ? v[q] : 0.;
             }
 #pragma unroll
@@ -640,7 +641,7 @@ __device__ __forceinline__ void solve_tile_staged_gg(
 #pragma unroll
             for (int q = 0; q < C; ++q) {
                 const int e = q * 64 + lane, row = e / C, col = e - row * C;
-                if (row < rows_active && col < cw) tile[(long long)row * n + j0 + col] = v[q];
+                if (row < rows_active && col < cw) tile[(long long)row * ld + j0 + col] = v[q];
             }
         }
         wave_sync();
@@ -660,7 +661,7 @@ __device__ __forceinline__ void solve_tile_staged_gg(
             for (int q = 0; q < C; ++q) {
                 const int e = q * 64 + lane, row = e / C, col = e - row * C;
                 const bool ok = row < rows_active && col < cw;
-                v[q] = tile[ok ? (long long)row * n + j0 + col : 0];
+                v[q] = tile[ok ? (long long)row * ld + j0 + col : 0];
                 v[q] = ok ? v[q] : 0.;
             }
 #pragma unroll
@@ -690,7 +691,7 @@ __device__ __forceinline__ void solve_tile_staged_gg(
 #pragma unroll
             for (int q = 0; q < C; ++q) {
                 const int e = q * 64 + lane, row = e / C, col = e - row * C;
-                if (row < rows_active && col < cw) tile[(long long)row * n + j0 + col] = v[q];
+                if (row < rows_active && col < cw) tile[(long long)row * ld + j0 + col] = v[q];
             }
         }
         wave_sync();
@@ -813,7 +814,7 @@ __device__ __forceinline__ void solve_line_ru(
 // columns; forward loads only, backward loads, recomputes and stores.
 template <int NO, int ORDER, int CK, int C>
 __device__ __forceinline__ void solve_tile_staged_ru(
-    double *__restrict__ tile, int rows_active, int n,
+    double *__restrict__ tile, int rows_active, int n, long long ld,
     const double *__restrict__ xInt, const double *__restrict__ df,
     const double *__restrict__ W0, const double *__restrict__ W1,
     const double *__restrict__ S,
@@ -836,7 +837,7 @@ __device__ __forceinline__ void solve_tile_staged_ru(
             for (int q = 0; q < C; ++q) {
                 const int e = q * 64 + lane, row = e / C, cl = e - row * C;
                 const bool ok = row < rows_active && cl < cw;
-                v[q] = tile[ok ? (long long)row * n + j0 + cl : 0];
+                v[q] = tile[ok ? (long long)row * ld + j0 + cl : 0];
                 v[q] = ok ? v[q] : 0.;
             }
 #pragma unroll
@@ -876,7 +877,7 @@ __device__ __forceinline__ void solve_tile_staged_ru(
             for (int q = 0; q < C; ++q) {
                 const int e = q * 64 + lane, row = e / C, cl = e - row * C;
                 const bool ok = row < rows_active && cl < cw;
-                v[q] = tile[ok ? (long long)row * n + j0 + cl : 0];
+                v[q] = tile[ok ? (long long)row * ld + j0 + cl : 0];
                 v[q] = ok ? v[q] : 0.;
             }
 #pragma unroll
@@ -917,7 +918,7 @@ __device__ __forceinline__ void solve_tile_staged_ru(
 #pragma unroll
             for (int q = 0; q < C; ++q) {
                 const int e = q * 64 + lane, row = e / C, cl = e - row * C;
-                if (row < rows_active && cl < cw) tile[(long long)row * n + j0 + cl] = v[q];
+                if (row < rows_active && cl < cw) tile[(long long)row * ld + j0 + cl] = v[q];
             }
         }
         wave_sync();
@@ -1046,10 +1047,31 @@ __device__ __forceinline__ void build_grid_tables(const double *__restrict__ xx,
     }
 }
 
+// First element and row stride of line `line` of axis `a` of a phi whose last
+// axis is stored with row pitch ld >= n (C order otherwise).  Lines enumerate
+// the other axes in C order.
+template <int D>
+__device__ __forceinline__ void line_addr(int n, long long ld, int a, long long line,
+                                          long long &off, long long &stride)
+{
+    if (a == D - 1) {
+        off = line * ld;
+        stride = 1;
+        return;
+    }
+    long long mid = 1;   // lines per index of axis a among the axes a+1 .. D-2
+    for (int q = a + 1; q < D - 1; ++q) mid *= n;
+    const long long cnt = mid * n, hi = line / cnt, lo = line - hi * cnt;
+    const long long lq = lo / n;
+    stride = mid * ld;
+    off = hi * n * stride + lq * ld + (lo - lq * n);
+}
+
 // All lines of axis `a`, tiles of 64 lines per wave.  GG: gamcol is this
 // lane's gamma column in the global workspace (rows 64 doubles apart).
 template <int D, int ORDER, int CK, int RU, int GG>
-__device__ __forceinline__ void sweep_tiles(double *__restrict__ phi, int n, int a, bool first_axis,
+__device__ __forceinline__ void sweep_tiles(double *__restrict__ phi, int n, long long ld, int a,
+                                            bool first_axis,
                                             int injm, const dh_segment *sp,
                                             const double *sx, const double *sxi,
                                             const double *sdf, const double *sdx,
@@ -1059,8 +1081,7 @@ __device__ __forceinline__ void sweep_tiles(double *__restrict__ phi, int n, int
                                             int lane)
 {
     constexpr int NO = D - 1;
-    long long s = 1, nsys = 1;
-    for (int q = a + 1; q < D; ++q) s *= n;
+    long long nsys = 1;
     for (int q = 0; q < NO; ++q) nsys *= n;
     double mig[NO > 0 ? NO : 1];
 #pragma unroll
@@ -1080,16 +1101,16 @@ __device__ __forceinline__ void sweep_tiles(double *__restrict__ phi, int n, int
                 if (line >= nsys) L.inj_row = -1;
                 if constexpr (GG)
                     solve_tile_staged_gg<NO, ORDER, STAGE_C>(
-                        phi + L0 * n, rows, n, sxi, sdf, W0, W1, S, L.c, mig, dt, inv_dt,
+                        phi + L0 * ld, rows, n, ld, sxi, sdf, W0, W1, S, L.c, mig, dt, inv_dt,
                         L.bcf_on, L.bcf, L.bcl_on, L.bcl, L.inj_row, inj, gamcol, stage, lane);
                 else if constexpr (RU)
                     solve_tile_staged_ru<NO, ORDER, CK, CK>(
-                        phi + L0 * n, rows, n, sxi, sdf, W0, W1, S, L.c, mig, dt, inv_dt,
+                        phi + L0 * ld, rows, n, ld, sxi, sdf, W0, W1, S, L.c, mig, dt, inv_dt,
                         L.bcf_on, L.bcf, L.bcl_on, L.bcl, L.inj_row, inj, gamcol,
                         (n - 1) / CK + 1, stage, lane);
                 else
                     solve_tile_staged<NO, ORDER, CK, STAGE_C>(
-                        TileRows<STAGE_C>{phi + L0 * n, rows, n}, n, sxi, sdf, W0, W1, S, L.c, mig, dt, inv_dt,
+                        TileRows<STAGE_C>{phi + L0 * ld, rows, ld}, n, sxi, sdf, W0, W1, S, L.c, mig, dt, inv_dt,
                         L.bcf_on, L.bcf, L.bcl_on, L.bcl, L.inj_row, inj, gamcol, stage, lane);
             }
             return;
@@ -1101,8 +1122,9 @@ __device__ __forceinline__ void sweep_tiles(double *__restrict__ phi, int n, int
             LineSetup<D, ORDER> L;
             line_setup<D, ORDER>(L, line, n, a, sx, mig, nu, ga, ha, sdx[0], sdx[n - 2],
                                  first_axis, injm);
-            const long long hi = line / s, lo = line - hi * s;
-            double *p = phi + hi * (long long)n * s + lo;
+            long long off, s;
+            line_addr<D>(n, ld, a, line, off, s);
+            double *p = phi + off;
             if constexpr (CK == 0)
                 solve_line<NO, ORDER>(p, s, n, sxi, sdf, W0, W1, S, 1, L.c, mig, dt, inv_dt,
                                       L.bcf_on, L.bcf, L.bcl_on, L.bcl, L.inj_row, inj, gamcol,
@@ -1160,6 +1182,7 @@ k_integrate_nd(dh_instance *__restrict__ tab, int ninst, int *__restrict__ count
         long long budget = max_steps > 0 ? max_steps : LLONG_MAX;
         const int frozen = ip->frozen, injm = ip->inject;
         double *phi = gbl(ip->phi_dev);
+        const long long ld = ip->ld > 0 ? ip->ld : n;
         while (seg < nseg && budget > 0) {
             const dh_segment *sp = gbl(ip->seg_dev) + seg;
             __syncthreads();
@@ -1177,7 +1200,7 @@ k_integrate_nd(dh_instance *__restrict__ tab, int ninst, int *__restrict__ count
                 for (int a = 0; a < D; ++a) {
                     if ((frozen >> a) & 1) continue;
                     const double *W0 = tabs + LDS_TABLES_PER_AXIS * a * n;
-                    sweep_tiles<D, ORDER, CK, RU, GG>(phi, n, a, first, injm, sp, sx, sxi, sdf, sdx, W0,
+                    sweep_tiles<D, ORDER, CK, RU, GG>(phi, n, ld, a, first, injm, sp, sx, sxi, sdf, sdx, W0,
                                           W0 + n, W0 + 2 * n, dt, inv_dt, gamcol, stage, wave, nw,
                                           lane);
                     __syncthreads();
@@ -1246,7 +1269,8 @@ __device__ inline bool pack_enter_segment(PackSlot &ps, const dh_instance *ip)
 
 template <int D, int ORDER>
 __device__ __forceinline__ void sweep_packed(int a, const PackSlot *slot, int S, int n,
-                                             long long L, long long Lp, const double *sx,
+                                             long long ld, long long L, long long Lp,
+                                             const double *sx,
                                              const double *sxi, const double *sdf,
                                              const double *sdx, const double *atab,
                                              double *gamcol, double *stage, int wave, int nw,
@@ -1254,8 +1278,6 @@ __device__ __forceinline__ void sweep_packed(int a, const PackSlot *slot, int S,
 {
     constexpr int NO = D - 1, CK = 8, C = STAGE_C;
     static_assert(8 % (64 / C) == 0, "row groups must not straddle the 8-line slot padding");
-    long long s = 1;
-    for (int q = a + 1; q < D; ++q) s *= n;
     const long long span = (long long)S * Lp;
     const int ntiles = (int)((span + 63) / 64);
     for (int t = wave; t < ntiles; t += nw) {
@@ -1278,7 +1300,7 @@ __device__ __forceinline__ void sweep_packed(int a, const PackSlot *slot, int S,
         if (!act) Ls.inj_row = -1;
         if (a == D - 1) {
             PackedRows<C> R;
-            R.n = n;
+            R.ld = ld;
             bool any = false;
 #pragma unroll
             for (int q = 0; q < C; ++q) {
@@ -1290,7 +1312,7 @@ __device__ __forceinline__ void sweep_packed(int a, const PackSlot *slot, int S,
                 const bool use = Gq < span && lq < L && pq.live && !((pq.frozen >> a) & 1);
                 const long long rows = L - lq < 64 / C ? L - lq : 64 / C;
                 R.valid[q] = use ? (int)rows : 0;
-                R.base[q] = (double *)pq.phi + (use ? lq * n : 0);
+                R.base[q] = (double *)pq.phi + (use ? lq * ld : 0);
                 any = any || use;
             }
             if (!any) continue;
@@ -1298,8 +1320,9 @@ __device__ __forceinline__ void sweep_packed(int a, const PackSlot *slot, int S,
                                                 dt, inv_dt, Ls.bcf_on, Ls.bcf, Ls.bcl_on, Ls.bcl,
                                                 Ls.inj_row, inj, gamcol, stage, lane);
         } else if (act) {
-            const long long hi = line / s, lo = line - hi * s;
-            double *p = (double *)ps.phi + hi * (long long)n * s + lo;
+            long long off, s;
+            line_addr<D>(n, ld, a, line, off, s);
+            double *p = (double *)ps.phi + off;
             solve_line_ck<NO, ORDER, CK>(p, s, n, sxi, sdf, W0, W0 + n, W0 + 2 * n, 1, Ls.c, mig,
                                          dt, inv_dt, Ls.bcf_on, Ls.bcf, Ls.bcl_on, Ls.bcl,
                                          Ls.inj_row, inj, gamcol, 64);
@@ -1320,6 +1343,7 @@ k_integrate_packed(dh_instance *__restrict__ tab, int ninst, int *__restrict__ c
     long long L = 1;
     for (int q = 0; q < D - 1; ++q) L *= n;
     const long long Lp = (L + 7) & ~7LL;
+    const long long ld = tab[0].ld > 0 ? tab[0].ld : n;   // shared by every instance
     double *sx = lds, *sxi = sx + n, *sdf = sxi + n, *sdx = sdf + n;
     double *atab = sdx + n;
     const int col = (n - 1) / 8 + 1;
@@ -1398,7 +1422,7 @@ k_integrate_packed(dh_instance *__restrict__ tab, int ninst, int *__restrict__ c
         __syncthreads();
         if (tid < S) slot[tid].rebuild = 0;
         for (int a = 0; a < D; ++a) {
-            sweep_packed<D, ORDER>(a, slot, S, n, L, Lp, sx, sxi, sdf, sdx, atab, gamcol, stage,
+            sweep_packed<D, ORDER>(a, slot, S, n, ld, L, Lp, sx, sxi, sdf, sdx, atab, gamcol, stage,
                                    wave, nw, lane);
             __syncthreads();
         }
@@ -1528,7 +1552,7 @@ __global__ void __launch_bounds__(256) k_sweep_axis(dh_instance inst, int a, int
     build_axis_tables<D>(sp, a, n, sx, sxi, sdx, W0, W1, S, tid, nthr);
     __syncthreads();
     const double dt = sp->dt, inv_dt = 1. / dt;
-    sweep_tiles<D, ORDER, 0, 0, 0>(gbl(inst.phi_dev), n, a, first_axis != 0, inst.inject, sp, sx, sxi, sdf,
+    sweep_tiles<D, ORDER, 0, 0, 0>(gbl(inst.phi_dev), n, inst.ld > 0 ? inst.ld : n, a, first_axis != 0, inst.inject, sp, sx, sxi, sdf,
                           sdx, W0, W1, S, dt, inv_dt, gamcol, nullptr,
                           (long long)blockIdx.x * nw + wave,
                           (long long)gridDim.x * nw, lane);
@@ -1720,7 +1744,7 @@ std::map<std::string, hipGraphExec_t> g_graphs;
 
 extern "C" {
 
-int dh_abi_version(void) { return 2; }
+int dh_abi_version(void) { return 3; }
 size_t dh_sizeof_segment(void) { return sizeof(dh_segment); }
 size_t dh_sizeof_instance(void) { return sizeof(dh_instance); }
 size_t dh_lds_bytes(int D, int n, int waves, int chunk, int recompute, int gamma_global)

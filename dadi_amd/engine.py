@@ -64,6 +64,17 @@ class Context:
             self._xx_cache[key] = g
         return g
 
+    def empty_phi(self, shape, batch=None):
+        """Uninitialised device phi of ``shape`` (all axes of equal length),
+        stored with its last axis padded to row_pitch(n) when D >= 2; with
+        ``batch``, a (batch,) + shape arena of such phis."""
+        t = torch()
+        shape = tuple(int(v) for v in shape)
+        n = shape[-1]
+        ld = row_pitch(n) if len(shape) >= 2 else n
+        full = ((batch,) if batch is not None else ()) + shape[:-1] + (ld,)
+        return t.empty(full, dtype=t.float64, device=self.device)[..., :n]
+
     def side_streams(self, k):
         """k streams (default priority) for launches that run side by side."""
         t = torch()
@@ -231,8 +242,35 @@ def _lines(D, n):
     return n ** (D - 1)
 
 
-def _inst_record(phi_ptr, xx_ptr, seg_ptr, nseg, D, n, order, frozen, inject):
+def pitch_of(phi):
+    """Row pitch (in doubles) of a device phi stored in C order with at most
+    its last axis padded; 0 when it is dense.  ValueError for other layouts."""
+    D = phi.dim()
+    st, sh = phi.stride(), tuple(phi.shape)
+    if D <= 1:
+        if D == 1 and st[0] != 1:
+            raise ValueError("phi must be C-ordered")
+        return 0
+    ld = st[-2]
+    ok = st[-1] == 1 and ld >= sh[-1]
+    expect = ld
+    for k in range(D - 2, -1, -1):
+        ok = ok and st[k] == expect
+        expect *= sh[k]
+    if not ok:
+        raise ValueError("phi must be C-ordered with at most its last axis padded")
+    return 0 if ld == sh[-1] else int(ld)
+
+
+def row_pitch(n):
+    """Row pitch the package allocates for phi on n points: a multiple of 8
+    doubles, so that every row of the last axis starts 64-byte aligned."""
+    return -(-n // 8) * 8
+
+
+def _inst_record(phi_ptr, xx_ptr, seg_ptr, nseg, D, n, order, frozen, inject, ld=0):
     rec = np.zeros(1, dtype=INST_DTYPE)
+    rec["ld"] = ld
     rec["phi"] = phi_ptr
     rec["xx"] = xx_ptr
     rec["seg"] = seg_ptr
@@ -255,14 +293,20 @@ def run_single(phi, xx, seg, order, frozen, inject, ctx=None, mode="auto", waves
     t = torch()
     ctx = ctx or context()
     host = isinstance(phi, np.ndarray)
-    phid = ctx.to_dev(np.ascontiguousarray(phi, dtype=np.float64)) if host else phi
+    if host:
+        src = np.ascontiguousarray(phi, dtype=np.float64)
+        phid = ctx.empty_phi(src.shape)
+        phid.copy_(t.from_numpy(src))
+    else:
+        phid = phi
     co = getattr(_local, "coalescer", None)
     if co is not None and len(seg) and frozen != (1 << phid.dim()) - 1:
         job = Job(phid, xx, seg, order, frozen, inject)
         co.submit(job)
         return phid.cpu().numpy() if host else phid
-    if not (phid.is_cuda and phid.dtype == t.float64 and phid.is_contiguous()):
-        raise ValueError("phi must be a contiguous float64 device tensor")
+    if not (phid.is_cuda and phid.dtype == t.float64):
+        raise ValueError("phi must be a float64 device tensor")
+    ld = pitch_of(phid)
     D = phid.dim()
     n = int(phid.shape[0])
     if n < 3:
@@ -281,7 +325,7 @@ def run_single(phi, xx, seg, order, frozen, inject, ctx=None, mode="auto", waves
         ck = 0 if ctx.max_waves(D, n, 0) >= min(8, math.ceil(nsys / 64)) else 8
         w = waves or max(1, min(ctx.max_waves(D, n, ck), math.ceil(nsys / 64)))
         rec = _inst_record(phid.data_ptr(), xxd.data_ptr(), segd.data_ptr(), len(seg), D, n,
-                           order, frozen, inject)
+                           order, frozen, inject, ld)
         tab = ctx.to_dev(rec.view(np.uint8))
         counter = t.zeros(1, dtype=t.int32, device=ctx.device)
         cfg = DhLaunch(w, 1, 0, ck, 0)   # one instance: full gamma column when it fits
@@ -294,7 +338,7 @@ def run_single(phi, xx, seg, order, frozen, inject, ctx=None, mode="auto", waves
         sosd = ctx.to_dev(seg_of_step)
         base = t.zeros(1, dtype=t.int64, device=ctx.device)
         inst = DhInstance(phid.data_ptr(), xxd.data_ptr(), segd.data_ptr(), len(seg), 0, 0,
-                          D, n, order, frozen, inject, 0)
+                          D, n, order, frozen, inject, 0, ld)
         w = waves or 2
         check(ctx.lib.dh_integrate_graph(inst, base.data_ptr(), sosd.data_ptr(), nst,
                                          steps_per_graph, w, stream), "dh_integrate_graph")
@@ -358,7 +402,7 @@ class PackedBatch:
             self.xxs.append(xxd)
             tab[i] = _inst_record(j.phi.data_ptr(), xxd.data_ptr(),
                                   base + int(offs[i]) * SEG_DTYPE.itemsize, len(j.seg), j.D, j.n,
-                                  j.order, j.frozen, j.inject)[0]
+                                  j.order, j.frozen, j.inject, pitch_of(j.phi))[0]
         self.host_tab = tab
         self.tab0 = ctx.to_dev(tab.view(np.uint8))
         self.tab = self.tab0.clone()
@@ -367,7 +411,7 @@ class PackedBatch:
         self.workspace = {}
         # several instances may share a workgroup when they share n and grid
         self.packable = self.D >= 2 and len({int(r) for r in tab["xx"]}) == 1 and \
-            all(j.n == self.nmax for j in jobs)
+            len({int(r) for r in tab["ld"]}) == 1 and all(j.n == self.nmax for j in jobs)
 
     def restart(self):
         """Reset every instance's progress to the start of its schedule
@@ -409,7 +453,7 @@ class PackedBatch:
 def run_batch(jobs, ctx=None, waves=None, groups=0, sort=True, chunk=None, recompute=None,
               gamma_global=None, slots=None):
     """Run every job to completion.  Jobs are grouped by (D, order); a group
-    of several populations is split further by (n, grid) so that each part
+    of several populations is split further by (n, grid, row pitch) so that each part
     can pack several instances per workgroup, the parts running side by side
     on their own streams.  Each part is one persistent launch, longest job
     first."""
@@ -424,7 +468,7 @@ def run_batch(jobs, ctx=None, waves=None, groups=0, sort=True, chunk=None, recom
         if key[0] >= 2 and slots != 1:
             by = {}
             for j in js:
-                by.setdefault((j.n, ctx.grid(j.xx).data_ptr()), []).append(j)
+                by.setdefault((j.n, ctx.grid(j.xx).data_ptr(), pitch_of(j.phi)), []).append(j)
             parts.extend(by[k] for k in sorted(by))
         else:
             parts.append(js)

@@ -195,7 +195,7 @@ class Server:
                 xxd = self.ctx.grid(r.prog.xx)
                 r.rec = engine._inst_record(r.phi.data_ptr(), xxd.data_ptr(), r.segd.data_ptr(),
                                             len(ep.seg), ep.D, r.prog.pts, ep.order, ep.frozen,
-                                            ep.inject)
+                                            ep.inject, engine.pitch_of(r.phi))
                 return "epoch"
             if r.rec["status"][0] == 1:
                 r.rec = None

@@ -61,9 +61,12 @@ typedef struct dh_segment {
     double mig[DH_MAXD][DH_MAXD - 1];
 } dh_segment;
 
-/* One integration ("instance").  phi_dev holds n^D doubles in C order and is
- * updated in place.  cur_seg/cur_step record progress: a launch resumes from
- * them and writes them back, so a schedule can be run in several quanta. */
+/* One integration ("instance").  phi_dev holds phi in C order, updated in
+ * place; its last axis may be padded: consecutive rows of the last axis are
+ * ld doubles apart (ld >= n; 0 means n, i.e. dense).  A pitch that is a
+ * multiple of 8 keeps every row 64-byte aligned.  cur_seg/cur_step record
+ * progress: a launch resumes from them and writes them back, so a schedule
+ * can be run in several quanta. */
 typedef struct dh_instance {
     double *phi_dev;
     const double *xx_dev;
@@ -77,6 +80,8 @@ typedef struct dh_instance {
     int frozen;  /* bit k set: population k frozen (its axis is not swept) */
     int inject;  /* bit k set: population k receives new mutations */
     int status;  /* 1 when the schedule has completed */
+    int ld;      /* row pitch of the last axis in doubles (0 = n) */
+    int reserved;
 } dh_instance;
 
 /* Launch shape of the workgroup-per-instance kernel. */
@@ -136,7 +141,7 @@ int dh_integrate_nd(dh_instance *table_dev, int ninst, int D, int order, int nma
  * lockstep so that their tridiagonal lines fill the waves (a 2-population phi
  * on 100 points has 100 lines per sweep; three of them fill 300 of 320 lanes
  * of a 5-wave workgroup).  Every instance must have D populations, n points,
- * the same order and the same grid (xx); each keeps its own schedule, and a
+ * the same order, grid (xx) and row pitch; each keeps its own schedule, and a
  * finished slot takes the next instance from the queue.  cfg->chunk must be
  * 8 (checkpointed gammas) with no recompute and no workspace.  Same results,
  * bit for bit, as dh_integrate_nd.

@@ -205,9 +205,9 @@ def test_batched_variants_bit_exact_on_golden(dadi_amd, chunk, recompute, gg, sl
 @pytest.mark.parametrize("D,n,order", [(2, 40, 1), (2, 100, 1), (2, 45, 0), (3, 12, 1)])
 def test_packed_slots_refill_and_quanta(dadi_amd, oracle, D, n, order):
     """The packed kernel (several instances per workgroup in lockstep, each on
-    its own schedule, slots refilled from the queue) gives the unpacked
-    kernel's bits, also when run in quanta of a few steps; spot checks
-    against the oracle.  Some instances have a frozen population, step counts
+    its own schedule, slots refilled from the queue) on row-padded phi gives
+    the unpacked kernel's bits on dense phi, also when run in quanta of a few
+    steps; spot checks against the oracle.  Some instances have a frozen population, step counts
     differ by two orders of magnitude."""
     import torch
     from dadi_amd import engine, schedule as S, _lib
@@ -240,11 +240,19 @@ def test_packed_slots_refill_and_quanta(dadi_amd, oracle, D, n, order):
             seg = S.const_segments(D, xx, 0.0, T, dt, nus, migs, gammas, [0.5] * D, 1.0)
         specs.append((seg, fz, inj, (T, nus, migs, gammas, frozen)))
 
-    def batch():
-        return [engine.Job(ctx.to_dev(phi0), xx, seg, order, fz, inj)
-                for seg, fz, inj, _ in specs]
+    def batch(padded=True):
+        jobs = []
+        for seg, fz, inj, _ in specs:
+            if padded:   # rows of the last axis padded to a multiple of 8
+                phi = ctx.empty_phi(phi0.shape)
+                phi.copy_(torch.from_numpy(phi0))
+                assert engine.pitch_of(phi) == engine.row_pitch(n) or n % 8 == 0
+            else:
+                phi = ctx.to_dev(phi0)
+            jobs.append(engine.Job(phi, xx, seg, order, fz, inj))
+        return jobs
 
-    plain = batch()
+    plain = batch(padded=False)
     engine.run_batch(plain, slots=1)
     want = [j.phi.cpu().numpy() for j in plain]
     plan = ctx.pack_plan(D, n)
