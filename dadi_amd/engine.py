@@ -127,12 +127,30 @@ class Context:
         return util * self._occupancy(lds, sh.waves)
 
     def shape(self, D, n, waves=None, chunk=None, recompute=None, gamma_global=None,
-              slots=None, packable=False):
+              slots=None, packable=False, ninst=None):
         """Launch shape of the nD kernels for instances of D populations on n
         points: the autotuned choice when one is recorded, else a rule of
         thumb.  ``packable``: the instances share n and grid, so several may
-        share a workgroup (slots >= 2, the packed kernel).
+        share a workgroup (slots >= 2, the packed kernel).  ``ninst``: number
+        of instances in the launch; too few to fill the GPU at the tuned
+        shape (tuned on full batches) and each instance gets as many waves as
+        it has 64-line tiles, for the shortest time per step.
         Returns Shape(waves, chunk, recompute, gamma_global, slots)."""
+        sh = self._shape(D, n, waves, chunk, recompute, gamma_global, slots, packable)
+        if ninst is None or waves is not None or slots is not None or D < 2:
+            return sh
+        tiles = -(-(n ** (D - 1)) // 64)
+        if -(-ninst // sh.slots) * sh.waves >= 4 * self.num_cu or \
+                (sh.slots < 2 and sh.waves >= min(8, tiles)):
+            return sh
+        ck = 8 if sh.slots >= 2 else sh.chunk
+        ru = 0 if sh.slots >= 2 else sh.recompute
+        gg = 0 if sh.slots >= 2 else sh.gamma_global
+        wmax = self.max_waves(D, n, ck, ru, gg)
+        w = max(sh.waves if sh.slots < 2 else 1, min(8, tiles, wmax))
+        return Shape(w, ck, ru, gg, 1)
+
+    def _shape(self, D, n, waves, chunk, recompute, gamma_global, slots, packable):
         if slots is not None and slots >= 2:
             if not packable:
                 raise ValueError("packing needs instances that share n and the grid")
@@ -144,7 +162,7 @@ class Context:
         if None in (waves, chunk, recompute, gamma_global) and self.tuning is not None:
             hit = self.tuning.lookup(D, n, packable=packable and free and slots is None)
             if hit is not None and hit.slots >= 2:
-                return self.shape(D, n, hit.waves, slots=hit.slots, packable=True)
+                return self._shape(D, n, hit.waves, None, None, None, hit.slots, True)
             if hit is not None:
                 waves = hit.waves if waves is None else waves
                 chunk = hit.chunk if chunk is None else chunk
@@ -189,7 +207,7 @@ class Context:
         launches that may overlap must not share it) that keeps the gamma
         workspace between calls.  ``packable``: every instance has n = nmax
         and the same grid."""
-        sh = self.shape(D, nmax, waves, chunk, recompute, gamma_global, slots, packable)
+        sh = self.shape(D, nmax, waves, chunk, recompute, gamma_global, slots, packable, ninst)
         cfg = DhLaunch(sh.waves, groups, max_steps, sh.chunk, sh.recompute)
         if sh.slots >= 2:
             check(self.lib.dh_integrate_packed(tab_ptr, ninst, D, order, nmax, sh.slots,
