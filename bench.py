@@ -16,6 +16,7 @@ Prints one JSON line.  --gpus > 1 is refused: this build uses one GPU.
 """
 import argparse
 import json
+import multiprocessing
 import os
 import sys
 import time
@@ -48,31 +49,108 @@ def build_jobs(dadi_amd, params, pts):
     return xx, phi0, segs
 
 
-def cpu_baseline(params, pts, budget_s):
-    """The reference's own C kernels (oracle/_ref, compiled from dadi's
-    sources) driven by a restatement of dadi's constant-parameter Python loop,
-    one core; falls back to the oracle's C restatement when _ref is absent."""
+def _cpu_share(task):
+    """One core's share of the CPU baseline: integrate draws until the
+    deadline (time.monotonic, shared by the processes)."""
+    params, pts, deadline = task
     from oracle import dadi_oracle as O
-    kind = "reference" if O.ref_available() else "port"
-    backend = "ref" if kind == "reference" else "oracle"
+    backend = "ref" if O.ref_available() else "oracle"
     xx = O.default_grid(pts)
     phi0 = O.phi_1D_to_2D(xx, O.phi_1D(xx))
-    done = 0
-    steps = 0
-    t0 = time.perf_counter()
+    done = steps = 0
+    t0 = time.monotonic()
     for nu1, nu2, T, m in params:
+        if time.monotonic() >= deadline:
+            break
         O.integrate_const(phi0, xx, T, [nu1, nu2], [[m], [m]], [0, 0], [0.5, 0.5],
                           backend=backend)
         dt = min(O.compute_dt(nu1, [m], 0, 0.5), O.compute_dt(nu2, [m], 0, 0.5))
         steps += int(np.ceil(T / dt))
         done += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
-    el = time.perf_counter() - t0
-    return {"value": done / el, "unit": "integrations/s", "cores": 1, "kind": kind,
-            "sample": "first %d draws of the same seeded split_mig distribution at pts=%d "
-                      "(%d time steps), full two_pops integrations, single thread, %.1f s"
-                      % (done, pts, steps, el)}
+    return done, steps, time.monotonic() - t0
+
+
+def cpu_baseline(params, pts, budget_s, procs):
+    """The reference's own C kernels (oracle/_ref, compiled from dadi's
+    sources) driven by a restatement of dadi's constant-parameter Python loop,
+    one process per core over disjoint shares of the draws (the oracle's C
+    restatement when _ref is absent).  Runs before the GPU is initialised:
+    the processes are forked."""
+    from oracle import dadi_oracle as O
+    kind = "reference" if O.ref_available() else "port"
+    procs = max(1, int(procs))
+    t0 = time.monotonic()
+    tasks = [(params[i::procs], pts, t0 + budget_s) for i in range(procs)]
+    if procs == 1:
+        res = [_cpu_share(tasks[0])]
+    else:
+        with multiprocessing.get_context("fork").Pool(procs) as pool:
+            res = pool.map(_cpu_share, tasks)
+    el = time.monotonic() - t0
+    done = sum(r[0] for r in res)
+    steps = sum(r[1] for r in res)
+    per_core = [r[0] / r[2] for r in res if r[0]]
+    return {"value": done / el, "unit": "integrations/s", "cores": procs, "kind": kind,
+            "single_core": float(np.mean(per_core)) if per_core else None,
+            "sample": "%d draws of the same seeded split_mig distribution at pts=%d (%d time "
+                      "steps), full two_pops integrations, %d processes x 1 thread, %.1f s"
+                      % (done, pts, steps, procs, el)}
+
+
+def model_workloads(args, rng):
+    """Whole model evaluations through the public API, many in flight at once
+    (engine.map_batched coalesces their integrations into shared launches):
+    split_mig and the 3-population chain extrapolated over three grids, and
+    the DFE caches (Cache1D of two_epoch_sel; optionally Cache2D of
+    split_mig_sel)."""
+    from dadi_amd import engine, Numerics, Demographics2D, Demographics3D
+    from dadi_amd.DFE import DemogSelModels, Cache1D_mod, Cache2D_mod
+    out = {}
+
+    def timed(fn):
+        t0 = time.perf_counter()
+        r = fn()
+        return r, time.perf_counter() - t0
+
+    if args.evals > 0:
+        ex = Numerics.make_extrap_log_func(Demographics2D.split_mig)
+        ps = draws(rng, args.evals)
+        res, el = timed(lambda: engine.map_batched(
+            lambda p: ex(p, (20, 20), [80, 90, 100]), [(p,) for p in ps], workers=args.workers))
+        out["split_mig_evaluations"] = {
+            "value": len(ps) / el, "unit": "evaluations/s", "evaluations": len(ps),
+            "failed": sum(isinstance(r, Exception) for r in res), "wall_s": el,
+            "model": "split_mig, pts (80, 90, 100) log-extrapolated, ns (20, 20)"}
+    if args.evals3d > 0:
+        ex3 = Numerics.make_extrap_log_func(Demographics3D.three_pop_chain)
+        lu = lambda: float(np.exp(rng.uniform(np.log(0.1), np.log(10))))
+        ps = [(lu(), rng.uniform(0.01, 1), lu(), lu(), rng.uniform(0.01, 1), rng.uniform(0, 5),
+               lu(), lu(), lu(), rng.uniform(0.01, 1), rng.uniform(0, 5))
+              for _ in range(args.evals3d)]
+        res, el = timed(lambda: engine.map_batched(
+            lambda p: ex3(p, (10, 10, 10), [40, 50, 60]), [(p,) for p in ps],
+            workers=args.workers))
+        out["chain3d_evaluations"] = {
+            "value": len(ps) / el, "unit": "evaluations/s", "evaluations": len(ps),
+            "failed": sum(isinstance(r, Exception) for r in res), "wall_s": el,
+            "model": "1D -> 2D -> 3D chain, pts (40, 50, 60) log-extrapolated, ns (10, 10, 10)"}
+    if args.dfe1d:
+        c1, el = timed(lambda: Cache1D_mod.Cache1D((2.0, 0.1), (50,),
+                                                   DemogSelModels.two_epoch_sel,
+                                                   (200, 220, 240), gamma_pts=500))
+        out["dfe_cache1d"] = {"value": el, "unit": "s", "spectra": len(c1.gammas) + 1,
+                              "model": "two_epoch_sel nu=2 T=0.1, 500 gammas, pts (200, 220, "
+                                       "240), ns 50"}
+    if args.dfe2d:
+        nu1, nu2, T, m = draws(rng, 1)[0]
+        c2, el = timed(lambda: Cache2D_mod.Cache2D((nu1, nu2, T, m), (20, 20),
+                                                   DemogSelModels.split_mig_sel, (60, 70, 80),
+                                                   gamma_pts=10))
+        out["dfe_cache2d"] = {"value": el, "unit": "s", "spectra": 100,
+                              "model": "split_mig_sel nu1=%.4g nu2=%.4g T=%.4g m=%.4g, 10x10 "
+                                       "gammas, pts (60, 70, 80), ns (20, 20)"
+                                       % (nu1, nu2, T, m)}
+    return out
 
 
 def measured_traffic(pts, batch, seed, kernel):
@@ -110,6 +188,17 @@ def main():
     ap.add_argument("--groups", type=int, default=0,
                     help="persistent workgroups (0 = as many as fit)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-procs", type=int, default=0,
+                    help="CPU baseline processes (0 = min(cores, 16))")
+    ap.add_argument("--evals", type=int, default=512,
+                    help="split_mig model evaluations (3 grids each) through the public API")
+    ap.add_argument("--evals3d", type=int, default=96,
+                    help="3-population chain evaluations (3 grids each)")
+    ap.add_argument("--workers", type=int, default=512,
+                    help="threads evaluating models concurrently")
+    ap.add_argument("--dfe1d", type=int, default=1, help="time a Cache1D build (500 gammas)")
+    ap.add_argument("--dfe2d", type=int, default=0,
+                    help="time a Cache2D build (10x10 gammas; long: up to ~5e5*T steps)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=200)
@@ -134,14 +223,20 @@ def main():
               "build runs on a single MI355X; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    rng = np.random.default_rng(args.seed)
+    params = draws(rng, args.batch)
+    cpu = None
+    if not args.no_cpu:
+        # before anything initialises the GPU: the baseline forks its processes
+        procs = args.cpu_procs or min(os.cpu_count() or 1, 16)
+        cpu = cpu_baseline(params, args.pts, args.cpu_seconds, procs)
+
     import torch
     import dadi_amd
     from dadi_amd import engine, _lib
     if not dadi_amd.cuda_enabled(True):
         raise SystemExit("HIP backend unavailable")
     ctx = engine.context()
-    rng = np.random.default_rng(args.seed)
-    params = draws(rng, args.batch)
     xx, phi0, segs = build_jobs(dadi_amd, params, args.pts)
     n = args.pts
     phi0d = ctx.to_dev(phi0)
@@ -222,9 +317,7 @@ def main():
                         "30% 1D three_epoch pts 100-600, 12% 3D chain pts 30-60, "
                         "3% 4D chain pts 16-24; Poisson arrivals, seeded"})
 
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(params, n, args.cpu_seconds)
+    workloads = model_workloads(args, np.random.default_rng(args.seed + 2))
 
     achieved = alg_bytes / kavg
     kname = ("k_integrate_packed<2, 1>" if shape.slots >= 2 else
@@ -260,6 +353,7 @@ def main():
                                  "p99": float(np.percentile(lat, 99) * 1e3) if lat else None}},
         "batched_over_plain": value / plain_rate if plain_rate else None,
         "serving": serving,
+        "model_workloads": workloads,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": traffic,
                      "kernel": kname, "kernel_ms_avg": kavg * 1e3,
