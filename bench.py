@@ -134,6 +134,28 @@ def model_workloads(args, rng):
             "value": len(ps) / el, "unit": "evaluations/s", "evaluations": len(ps),
             "failed": sum(isinstance(r, Exception) for r in res), "wall_s": el,
             "model": "1D -> 2D -> 3D chain, pts (40, 50, 60) log-extrapolated, ns (10, 10, 10)"}
+    if args.opt:
+        # BFGS in log(params) on a seeded Poisson sample of a split_mig model:
+        # one evaluation at a time (as dadi) and with each point's
+        # finite-difference stencil evaluated as one batch (same trajectory)
+        from dadi_amd import Inference, Spectrum
+        ex = Numerics.make_extrap_log_func(Demographics2D.split_mig)
+        grids, ns = [40, 50, 60], (20, 20)
+        truth = [1.8, 0.6, 0.4, 1.5]
+        model = ex(truth, ns, grids)
+        data = Spectrum(rng.poisson(5000 * np.maximum(model.data, 0)).astype(float))
+        data.mask = model.mask
+        kw = dict(lower_bound=[1e-2, 1e-2, 1e-3, 0], upper_bound=[20, 20, 5, 10],
+                  maxiter=args.opt, full_output=True)
+        p0 = [1.0, 1.0, 0.5, 1.0]
+        ref, el_s = timed(lambda: Inference.optimize_log(p0, data, ex, grids, batch=False, **kw))
+        got, el_b = timed(lambda: Inference.optimize_log(p0, data, ex, grids, batch=True, **kw))
+        out["optimize_log"] = {
+            "value": el_s / el_b, "unit": "x (serial time / batched time)",
+            "serial_s": el_s, "batched_s": el_b, "iterations": args.opt,
+            "evaluations": Inference.last_model_cache.batched,
+            "same_result": bool(np.array_equal(ref[0], got[0]) and ref[1] == got[1]),
+            "model": "split_mig, pts (40, 50, 60) log-extrapolated, ns (20, 20), 4 parameters"}
     if args.dfe1d:
         c1, el = timed(lambda: Cache1D_mod.Cache1D((2.0, 0.1), (50,),
                                                    DemogSelModels.two_epoch_sel,
@@ -197,6 +219,8 @@ def main():
     ap.add_argument("--workers", type=int, default=512,
                     help="threads evaluating models concurrently")
     ap.add_argument("--dfe1d", type=int, default=1, help="time a Cache1D build (500 gammas)")
+    ap.add_argument("--opt", type=int, default=4,
+                    help="BFGS iterations of the optimiser comparison (0 = skip)")
     ap.add_argument("--dfe2d", type=int, default=0,
                     help="time a Cache2D build (10x10 gammas; long: up to ~5e5*T steps)")
     ap.add_argument("--no-cpu", action="store_true")

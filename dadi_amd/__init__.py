@@ -14,6 +14,7 @@ from . import Demes, Misc, Numerics, PhiManip, Integration, Inference
 from . import Spectrum_mod
 from .Spectrum_mod import Spectrum
 from . import Demographics1D, Demographics2D, Demographics3D
+from . import Godambe
 from . import DFE
 
 # dadi ignores floating-point warnings globally (dadi/__init__.py:28)

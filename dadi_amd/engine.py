@@ -514,7 +514,7 @@ class Coalescer:
     Integration API thereby batch across evaluations without changes."""
 
     def __init__(self, ctx=None):
-        self.ctx = ctx or context()
+        self._ctx = ctx          # resolved at the first flush: no GPU needed until then
         self.cv = threading.Condition()
         self.waiting = []
         self.live = 0
@@ -542,7 +542,9 @@ class Coalescer:
         if self.waiting and len(self.waiting) >= self.live:
             jobs, self.waiting = self.waiting, []
             try:
-                run_batch(jobs, self.ctx)
+                if self._ctx is None:
+                    self._ctx = context()
+                run_batch(jobs, self._ctx)
             finally:
                 for j in jobs:
                     j.done = True
