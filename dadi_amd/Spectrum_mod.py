@@ -8,11 +8,17 @@ binomial sampling probabilities, whose interval integrals are differences of
 regularised incomplete beta functions.  The beta tables are built on the host
 with scipy (as dadi does, cached per sample size and grid); the contractions,
 one axis at a time, run on the device (``dh_sample_axis``).
+
+With ``het_ascertained`` or ``force_direct`` the direct samplers of
+dadi/Spectrum_mod.py:1263-1279, 1350-1380, 1550-1592, 1594-1643 apply: the
+trapezoid rule against the binomial sampling factors (times x(1-x) along a
+heterozygote-ascertained axis), again one device contraction per axis
+(``dh_contract_axis``) against host-built tables of factor x trapezoid weight.
 """
 import logging
 
 import numpy
-from scipy.special import betainc
+from scipy.special import betainc, comb
 
 from . import _lib
 from . import engine
@@ -52,6 +58,69 @@ def _device_dbeta(ctx, m, xx):
             _dev_dbeta.clear()
         _dev_dbeta[key] = hit
     return hit
+
+
+_direct_tables = {}
+
+
+def direct_table(m, xx, het=False):
+    """(m+1, len(xx)) table: comb(m, k) x^k (1-x)^(m-k) [x (1-x)] (the factor of
+    dadi/Spectrum_mod.py:1272-1274) times the trapezoid weight of each grid
+    point, so that contracting phi with row k is the trapezoid integral of
+    factor_k * phi."""
+    xx = numpy.asarray(xx, dtype=numpy.float64)
+    key = (m, xx.tobytes(), bool(het))
+    hit = _direct_tables.get(key)
+    if hit is None:
+        k = numpy.arange(m + 1)[:, None]
+        F = comb(m, k) * xx[None, :] ** k * (1 - xx[None, :]) ** (m - k)
+        if het:
+            F = F * (xx * (1 - xx))[None, :]
+        d = numpy.diff(xx)
+        w = numpy.zeros_like(xx)
+        w[:-1] += d / 2.0
+        w[1:] += d / 2.0
+        hit = numpy.ascontiguousarray(F * w[None, :])
+        if len(_direct_tables) > 256:
+            _direct_tables.clear()
+        _direct_tables[key] = hit
+    return hit
+
+
+def sample_device_direct(phi, ns, xxs, het_ascertained=None, ctx=None):
+    """Direct (trapezoid) sample spectrum data of a 1-4D phi on the device;
+    every axis may have its own grid.  Returns a device tensor of shape
+    (n0+1, ..., n_{D-1}+1)."""
+    import torch
+    ctx = ctx or engine.context()
+    if isinstance(phi, numpy.ndarray):
+        cur = ctx.to_dev(numpy.ascontiguousarray(phi, dtype=numpy.float64))
+    else:
+        cur = phi.to(dtype=torch.float64).contiguous()
+    D = cur.dim()
+    if D > 4:
+        raise NotImplementedError('direct sampling is provided for 1-4 populations, as in '
+                                  'the reference')
+    names = ('xx', 'yy', 'zz', 'aa')
+    stream = ctx.stream()
+    keep = []
+    for k in range(D - 1, -1, -1):
+        xx = numpy.asarray(xxs[k], dtype=numpy.float64)
+        n = int(cur.shape[-1])
+        if n != len(xx):
+            raise ValueError('phi and grid sizes differ')
+        m = int(ns[k])
+        A = ctx.to_dev(direct_table(m, xx, het_ascertained == names[k]))
+        P = cur.numel() // n
+        out = torch.empty((m + 1,) + tuple(cur.shape[:-1]), dtype=torch.float64,
+                          device=ctx.device)
+        _lib.check(ctx.lib.dh_contract_axis(cur.data_ptr(), P, n, A.data_ptr(), m + 1,
+                                            out.data_ptr(), stream), "dh_contract_axis")
+        keep.append((cur, A))
+        cur = out
+    torch.cuda.current_stream(ctx.device).synchronize()
+    del keep
+    return cur
 
 
 def sample_device(phi, ns, xx, ctx=None):
@@ -141,16 +210,25 @@ class Spectrum(numpy.ma.masked_array):
     def from_phi(phi, ns, xxs, mask_corners=True, pop_ids=None, admix_props=None,
                  het_ascertained=None, force_direct=False):
         """Expected spectrum of sample sizes ``ns`` from the density ``phi`` on
-        grids ``xxs`` (dadi/Spectrum_mod.py:1938-2062).  The analytic
-        (piecewise-linear phi) samplers only; every grid must be the same."""
-        if admix_props or het_ascertained or force_direct:
-            raise NotImplementedError('admix_props, het_ascertained and force_direct '
-                                      'samplers are not provided by the GPU backend')
+        grids ``xxs`` (dadi/Spectrum_mod.py:1938-2062): the analytic
+        (piecewise-linear phi) samplers, which need one grid for every axis,
+        or with het_ascertained / force_direct the direct trapezoid samplers
+        (1-4 populations).  admix_props is not provided."""
+        if admix_props:
+            raise NotImplementedError('the admix_props samplers are not provided by the GPU '
+                                      'backend; implement admixture with PhiManip')
         ndim = phi.ndim if hasattr(phi, 'ndim') else phi.dim()
         if not ndim == len(ns) == len(xxs):
             raise ValueError('Dimensionality of phi and lengths of ns and xxs do not all agree.')
+        if het_ascertained and het_ascertained not in ['xx', 'yy', 'zz']:
+            raise ValueError("If used, het_ascertained must be 'xx', 'yy', or 'zz'.")
         if ndim > 5:
             raise ValueError('Only implemented for dimensions 1-5.')
+        if het_ascertained or force_direct:
+            data = sample_device_direct(phi, ns, xxs, het_ascertained).cpu().numpy()
+            fs = Spectrum(data, mask_corners=mask_corners, pop_ids=pop_ids)
+            fs.extrap_x = xxs[0][1]
+            return fs
         xx = numpy.asarray(xxs[0], dtype=numpy.float64)
         for other in xxs[1:]:
             if not (len(other) == len(xx) and numpy.allclose(other, xx)):

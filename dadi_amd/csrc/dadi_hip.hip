@@ -1610,6 +1610,22 @@ __global__ void k_sample_contract(const double *__restrict__ sT, const double *_
     out[e] = t1 + t2 * fac;
 }
 
+// Direct (trapezoid) sampling: out[k][p] = sum_i A[k][i] * in[p][i], where
+// A[k][i] holds the binomial sampling factor of entry k at grid point i times
+// the trapezoid weight of that point (dadi/Spectrum_mod.py:1263-1279,
+// 1350-1380, 1550-1592, 1594-1643).
+__global__ void k_contract_axis(const double *__restrict__ in, long long P, int n,
+                                const double *__restrict__ A, int m1, double *__restrict__ out)
+{
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long long)m1 * P) return;
+    const long long k = e / P, p = e - k * P;
+    const double *a = A + k * n, *row = in + p * n;
+    double t = 0.;
+    for (int i = 0; i < n; ++i) t = t + a[i] * row[i];
+    out[e] = t;
+}
+
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
@@ -2000,6 +2016,21 @@ int dh_sample_axis(const double *in_dev, long long P, int n, const double *xx_de
     const long long outn = (long long)(m + 1) * P;
     hipLaunchKernelGGL(k_sample_contract, dim3((unsigned)((outn + B - 1) / B)), dim3(B), 0, st,
                        sT, cT, P, n, m, db1_dev, db2_dev, out_dev);
+    DH_HIP(hipGetLastError());
+    return DH_OK;
+}
+
+int dh_contract_axis(const double *in_dev, long long P, int n, const double *A_dev, int m1,
+                     double *out_dev, void *stream)
+{
+    if (int e = check_init()) return e;
+    if (!in_dev || !A_dev || !out_dev) return fail(DH_EINVAL, "null pointer");
+    if (P <= 0 || n < 1 || m1 < 1) return fail(DH_EINVAL, "bad contraction shape");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const long long outn = (long long)m1 * P;
+    const int B = 256;
+    hipLaunchKernelGGL(k_contract_axis, dim3((unsigned)((outn + B - 1) / B)), dim3(B), 0, st,
+                       in_dev, P, n, A_dev, m1, out_dev);
     DH_HIP(hipGetLastError());
     return DH_OK;
 }

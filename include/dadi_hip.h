@@ -201,6 +201,19 @@ int dh_sample_axis(const double *in_dev, long long P, int n, const double *xx_de
                    int m, const double *db1_dev, const double *db2_dev,
                    double *work_dev, double *out_dev, void *stream);
 
+/*
+ * Direct (trapezoid-rule) sampling along the last axis: out (m1, P) =
+ * A (m1, n) contracted with in (P, n) over n, i.e. out[k][p] = sum_i A[k][i] *
+ * in[p][i].  A[k][i] is the binomial sampling factor of entry k at grid point
+ * i (times x(1-x) for a heterozygote-ascertained axis) times the trapezoid
+ * weight of point i.  Replaces the direct samplers
+ * dadi/Spectrum_mod.py:1263-1279, 1350-1380, 1550-1592, 1594-1643 (used for
+ * het_ascertained and force_direct); calling it D times, last axis first,
+ * yields the spectrum.
+ */
+int dh_contract_axis(const double *in_dev, long long P, int n, const double *A_dev, int m1,
+                     double *out_dev, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,6 +18,8 @@ Contents, each a restatement of the reference function it cites:
 * ``integrate_tv``            dadi/Integration.py:178-937 (time-varying loops)
 * ``from_phi``                dadi/Spectrum_mod.py:25-41, 1307-1348, 1509-1548,
                               1774-1871, 1938-2062
+* ``from_phi_direct``         dadi/Spectrum_mod.py:1263-1279, 1350-1380, 1550-1592,
+                              1594-1643 (trapezoid samplers, heterozygote ascertainment)
 * ``quadratic_extrap``/``linear_extrap``/``extrapolate``
                               dadi/Numerics.py:334-475, 644-669
 * ``ll``/``ll_multinom``      dadi/Inference.py:466-575, 668-685
@@ -32,7 +34,12 @@ import os
 
 import numpy as np
 from numpy import newaxis as nuax
-from scipy.special import betainc, gammaln
+from scipy.special import betainc, comb, gammaln
+
+try:
+    _trapezoid = np.trapezoid
+except AttributeError:  # numpy < 2
+    _trapezoid = np.trapz
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORC_TV, ORC_PRE, ORC_PRE1 = 0, 1, 2
@@ -527,6 +534,78 @@ def from_phi(phi, ns, xx):
         out = np.moveaxis(out, -1, 0)
     # after D contractions the sample axes are in order (n0, n1, ..., n_{D-1})
     return out
+
+
+def _binom_factor(n, ii, xx, het):
+    """comb(n, ii) x^ii (1-x)^(n-ii) [* x (1-x)], dadi/Spectrum_mod.py:1272-1274."""
+    f = comb(n, ii) * xx ** ii * (1 - xx) ** (n - ii)
+    if het:
+        f *= xx * (1 - xx)
+    return f
+
+
+def from_phi_direct(phi, ns, xxs, het_ascertained=None):
+    """Sampled spectrum data of a 1-4D phi by the trapezoid rule against the
+    binomial sampling factors, following the loops of the reference's direct
+    samplers (dadi/Spectrum_mod.py:1263-1279, 1350-1380, 1550-1592,
+    1594-1643); het_ascertained 'xx'/'yy'/'zz' weights that axis by x(1-x)."""
+    phi = np.asarray(phi, dtype=np.float64)
+    D = phi.ndim
+    xxs = [np.asarray(x, dtype=np.float64) for x in xxs]
+    het = [het_ascertained == name for name in ('xx', 'yy', 'zz', 'aa')[:D]]
+    if D == 1:
+        n, xx = ns[0], xxs[0]
+        data = np.zeros(n + 1)
+        for ii in range(n + 1):
+            data[ii] = _trapezoid(_binom_factor(n, ii, xx, het[0]) * phi, xx)
+        return data
+    if D == 2:
+        (nx, ny), (xx, yy) = ns, xxs
+        data = np.zeros((nx + 1, ny + 1))
+        fx = {ii: _binom_factor(nx, ii, xx, het[0]) for ii in range(nx + 1)}
+        dx, dy = np.diff(xx), np.diff(yy)
+        for jj in range(ny + 1):
+            fy = _binom_factor(ny, jj, yy, het[1])
+            over_y = _trapezoid(fy[nuax, :] * phi, dx=dy)
+            for ii in range(nx + 1):
+                data[ii, jj] = _trapezoid(fx[ii] * over_y, dx=dx)
+        return data
+    if D == 3:
+        (nx, ny, nz), (xx, yy, zz) = ns, xxs
+        data = np.zeros((nx + 1, ny + 1, nz + 1))
+        dy, dz = np.diff(yy), np.diff(zz)
+        half_dx = np.diff(xx) / 2.0
+        fx = {ii: _binom_factor(nx, ii, xx, het[0]) for ii in range(nx + 1)}
+        fy = {jj: _binom_factor(ny, jj, yy, het[1])[nuax, :] for jj in range(ny + 1)}
+        for kk in range(nz + 1):
+            fz = _binom_factor(nz, kk, zz, het[2])
+            over_z = _trapezoid(fz[nuax, nuax, :] * phi, dx=dz)
+            for jj in range(ny + 1):
+                over_y = _trapezoid(fy[jj] * over_z, dx=dy)
+                for ii in range(nx + 1):
+                    integrand = fx[ii] * over_y
+                    data[ii, jj, kk] = np.sum(half_dx * (integrand[1:] + integrand[:-1]))
+        return data
+    if D == 4:
+        (nx, ny, nz, na), (xx, yy, zz, aa) = ns, xxs
+        data = np.zeros((nx + 1, ny + 1, nz + 1, na + 1))
+        dy, dz, da = np.diff(yy), np.diff(zz), np.diff(aa)
+        half_dx = np.diff(xx) / 2.0
+        fx = {ii: _binom_factor(nx, ii, xx, het[0]) for ii in range(nx + 1)}
+        fy = {jj: _binom_factor(ny, jj, yy, het[1])[nuax, :] for jj in range(ny + 1)}
+        fz = {kk: _binom_factor(nz, kk, zz, het[2])[nuax, nuax, :] for kk in range(nz + 1)}
+        for ll in range(na + 1):
+            fa = _binom_factor(na, ll, aa, het[3])
+            over_a = _trapezoid(fa[nuax, nuax, nuax, :] * phi, dx=da)
+            for kk in range(nz + 1):
+                over_z = _trapezoid(fz[kk] * over_a, dx=dz)
+                for jj in range(ny + 1):
+                    over_y = _trapezoid(fy[jj] * over_z, dx=dy)
+                    for ii in range(nx + 1):
+                        integrand = fx[ii] * over_y
+                        data[ii, jj, kk, ll] = np.sum(half_dx * (integrand[1:] + integrand[:-1]))
+        return data
+    raise ValueError("direct sampling is defined for 1-4 populations")
 
 
 def corner_mask(shape):

@@ -280,3 +280,18 @@ def test_packed_slots_refill_and_quanta(dadi_amd, oracle, D, n, order):
             ref = oracle.integrate_const(phi0, xx, T, nus, migs, gammas, [0.5] * D,
                                          frozen=frozen)
             assert np.array_equal(want[i], ref), i
+
+
+@pytest.mark.parametrize("D,n,ns,het", [(1, 60, (12,), None), (1, 60, (12,), "xx"),
+                                        (2, 30, (8, 10), "yy"), (2, 25, (6, 6), None),
+                                        (3, 14, (4, 5, 6), "zz"), (4, 10, (3, 4, 3, 2), "xx")])
+def test_direct_sampler_matches_oracle(dadi_amd, oracle, D, n, ns, het):
+    """het_ascertained / force_direct sampling on the device against the
+    oracle's restatement of dadi's trapezoid samplers; grids differ between
+    axes where the reference allows it."""
+    rng = np.random.default_rng(D * 100 + n)
+    xxs = [oracle.default_grid(n + (k if D <= 2 else 0)) for k in range(D)]
+    phi = rng.uniform(0.1, 2.0, tuple(len(x) for x in xxs))
+    want = oracle.from_phi_direct(phi, ns, xxs, het)
+    fs = dadi_amd.Spectrum.from_phi(phi, ns, xxs, het_ascertained=het, force_direct=True)
+    assert np.allclose(fs.data, want, rtol=1e-12, atol=0)

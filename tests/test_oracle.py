@@ -153,3 +153,43 @@ def test_frozen1_reference_golden(oracle):
     fs = oracle.from_phi(phi, [10, 10], xx)
     rel = np.abs(fs - data)[~mask] / np.abs(data[~mask])
     assert rel.max() < 1e-6
+
+
+def test_direct_sampler_matches_beta_integrals(oracle):
+    """The oracle's trapezoid samplers (dadi/Spectrum_mod.py:1263-1279,
+    1350-1380, 1550-1592) integrate comb(n,k) x^k (1-x)^(n-k) phi(x): for
+    polynomial phi on a fine grid they reach the closed-form Beta integrals,
+    with and without the x(1-x) weight of heterozygote ascertainment."""
+    from scipy.special import beta as B, comb
+    xx = np.linspace(0, 1, 4001)
+    n = 9
+
+    def exact(a, het):
+        k = np.arange(n + 1)
+        e = het * 1
+        return comb(n, k) * B(k + a + 1 + e, n - k + 1 + e)
+
+    for a in (0, 1, 3):
+        for het in (False, True):
+            got = oracle.from_phi_direct(xx ** a, [n], [xx], 'xx' if het else None)
+            assert np.allclose(got, exact(a, het), rtol=1e-5, atol=1e-12)
+    # 2D and 3D separable densities are products of 1D integrals; the
+    # ascertainment weight applies to the named axis only
+    m = 6
+    x2 = np.linspace(0, 1, 1201)
+    phi2 = (x2 ** 2)[:, None] * (1 - x2)[None, :]
+    got = oracle.from_phi_direct(phi2, [m, m], [x2, x2], 'yy')
+    k = np.arange(m + 1)
+    ex = comb(m, k) * B(k + 3, m - k + 1)
+    ey = comb(m, k) * B(k + 2, m - k + 3)   # (1-y) * y(1-y)
+    assert np.allclose(got, np.outer(ex, ey), rtol=1e-5, atol=1e-12)
+    x3 = np.linspace(0, 1, 61)   # coarse: the trapezoid error is ~1e-3 here
+    phi3 = ((x3 ** 2)[:, None] * (1 - x3)[None, :])[:, :, None] * np.ones(61)[None, None, :]
+    got3 = oracle.from_phi_direct(phi3, [m, m, 4], [x3, x3, x3])
+    ez = comb(4, np.arange(5)) * B(np.arange(5) + 1, 4 - np.arange(5) + 1)
+    ey0 = comb(m, k) * B(k + 1, m - k + 2)
+    assert np.allclose(got3, ex[:, None, None] * ey0[None, :, None] * ez[None, None, :],
+                       rtol=5e-3, atol=1e-6)
+    # a planted mistake (dropping the binomial coefficient) does not pass
+    bad = got / comb(m, k)[None, :]
+    assert not np.allclose(bad, np.outer(ex, ey), rtol=1e-3)
