@@ -145,6 +145,40 @@ def one_pop(phi, xx, T, nu=1, gamma=0, h=0.5, theta0=1.0, initial_t=0,
                 beta, deme_ids, const_ok=True)
 
 
+def one_pop_X(phi, xx, T, nu=1, gamma=0, h=0.5, beta=1, alpha=1, theta0=1.0, initial_t=0,
+              frozen=False):
+    """Integrate a 1-population phi of X-chromosome alleles forward
+    (dadi/Integration.py:1256-1352): V = 1/nu x(1-x) (2 beta+4)(beta+1)/(9 beta),
+    M = gamma 4/3 (0.5+h+x(1-2h)) x(1-x), injection scaled by
+    2/(1+2 beta) (1/(alpha+1) + beta).  beta = Nf/Nm, alpha = mu_m/mu_f.
+    Constant parameters only, as in the reference."""
+    phi = _prep_phi(phi, 1, copy=True)
+    if frozen:
+        return phi
+    if T - initial_t == 0:
+        return phi
+    _check_T(T, initial_t)
+    if not _scalar_all((nu, gamma, h, theta0, beta, alpha)):
+        raise NotImplementedError('X chromosome integration currently only '
+                                  'implemented for constant parameters.')
+    if numpy.any(numpy.less([T, nu, theta0, beta, alpha], 0)):
+        raise ValueError('A time, population size, migration rate, theta0, '
+                         'beta, or alpha is < 0. Has the model been '
+                         'mis-specified?')
+    if numpy.any(numpy.equal([nu, beta], 0)):
+        raise ValueError('A population size or beta is 0. Has the model been '
+                         'mis-specified?')
+    _backend_ok()
+    xx = numpy.asarray(xx, dtype=numpy.float64)
+    if len(xx) != phi.shape[0]:
+        raise ValueError("phi and xx sizes differ")
+    dx = numpy.diff(xx)
+    dt = _sched.compute_dt(nu, [0], gamma, h, timescale_factor, use_old_timestep,
+                           old_timescale_factor, dx)
+    seg = _sched.xchrom_segments(xx, initial_t, T, dt, nu, gamma, h, beta, alpha, theta0)
+    return engine.run_single(phi, xx, seg, _lib.ORDER_PRE1, 0, 1)
+
+
 def two_pops(phi, xx, T, nu1=1, nu2=1, m12=0, m21=0, gamma1=0, gamma2=0,
              h1=0.5, h2=0.5, theta0=1, initial_t=0, frozen1=False,
              frozen2=False, nomut1=False, nomut2=False, enable_cuda_cached=False,

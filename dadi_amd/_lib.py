@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("DADI_AMD_LIB") or os.path.join(HERE, "libdadi_hip.so")
 
 MAXD = 5
-ABI_VERSION = 3
+ABI_VERSION = 4
 ORDER_TV, ORDER_PRE, ORDER_PRE1 = 0, 1, 2
 
 #: numpy mirror of ``dh_segment``
@@ -26,6 +26,7 @@ SEG_DTYPE = np.dtype([
     ("beta_p", "<f8"), ("beta_q", "<f8"),
     ("nu", "<f8", (MAXD,)), ("gamma", "<f8", (MAXD,)), ("h", "<f8", (MAXD,)),
     ("mig", "<f8", (MAXD, MAXD - 1)),
+    ("beta_r", "<f8"), ("chrom", "<i4"), ("reserved", "<i4"),
 ])
 
 #: numpy mirror of ``dh_instance``

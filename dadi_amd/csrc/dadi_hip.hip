@@ -66,9 +66,12 @@ __device__ __forceinline__ double d_vfunc(double x, double nu)
     return 1. / nu * x * (1. - x);
 }
 
-// selection part of M, dadi/integration_shared.c:12-30
-__device__ __forceinline__ double d_sel(double x, double gamma, double h)
+// selection part of M, dadi/integration_shared.c:12-30; for an X chromosome
+// (chrom = 1) gamma * 4/3 * (0.5 + h + x (1-2h)) x (1-x),
+// dadi/Integration.py:1246-1247
+__device__ __forceinline__ double d_sel(double x, double gamma, double h, int chrom = 0)
 {
+    if (chrom) return gamma * 4. / 3. * (0.5 + h + x * (1. - 2 * h)) * x * (1. - x);
     return gamma * 2 * (h + (1. - 2 * h) * x) * x * (1. - x);
 }
 
@@ -938,7 +941,7 @@ template <int D, int ORDER>
 __device__ __forceinline__ void line_setup(LineSetup<D, ORDER> &L, long long line, int n, int a,
                                            const double *__restrict__ sx, const double *mig,
                                            double nu, double gam_a, double h_a, double dx0,
-                                           double dxl, bool first_axis, int injm)
+                                           double dxl, bool first_axis, int injm, int chrom = 0)
 {
     constexpr int NO = D - 1;
     int idx[NO > 0 ? NO : 1];
@@ -967,11 +970,11 @@ __device__ __forceinline__ void line_setup(LineSetup<D, ORDER> &L, long long lin
     if constexpr (ORDER == DH_ORDER_TV) {
         // dadi/integration2D.c:67-70 and its 3-5D/1D twins
         if (all0) {
-            const double Mf = d_M<NO>(L.c, mig, sx[0], d_sel(sx[0], gam_a, h_a));
+            const double Mf = d_M<NO>(L.c, mig, sx[0], d_sel(sx[0], gam_a, h_a, chrom));
             if (Mf <= 0) { L.bcf_on = true; L.bcf = (0.5 / nu - Mf) * 2. / dx0; }
         }
         if (all1) {
-            const double Ml = d_M<NO>(L.c, mig, sx[n - 1], d_sel(sx[n - 1], gam_a, h_a));
+            const double Ml = d_M<NO>(L.c, mig, sx[n - 1], d_sel(sx[n - 1], gam_a, h_a, chrom));
             if (Ml >= 0) { L.bcl_on = true; L.bcl = -(-0.5 / nu - Ml) * 2. / dxl; }
         }
     } else {
@@ -980,14 +983,14 @@ __device__ __forceinline__ void line_setup(LineSetup<D, ORDER> &L, long long lin
             double cc[NO > 0 ? NO : 1];
 #pragma unroll
             for (int q = 0; q < NO; ++q) cc[q] = sx[0];
-            const double M0 = d_M<NO>(cc, mig, sx[0], d_sel(sx[0], gam_a, h_a));
+            const double M0 = d_M<NO>(cc, mig, sx[0], d_sel(sx[0], gam_a, h_a, chrom));
             if (M0 <= 0) { L.bcf_on = true; L.bcf = (0.5 / nu - M0) * 2 / dx0; }
         }
         if (last) {
             double cc[NO > 0 ? NO : 1];
 #pragma unroll
             for (int q = 0; q < NO; ++q) cc[q] = sx[n - 1];
-            const double M1 = d_M<NO>(cc, mig, sx[n - 1], d_sel(sx[n - 1], gam_a, h_a));
+            const double M1 = d_M<NO>(cc, mig, sx[n - 1], d_sel(sx[n - 1], gam_a, h_a, chrom));
             if (M1 >= 0) { L.bcl_on = true; L.bcl = -(-0.5 / nu - M1) * 2 / dxl; }
         }
     }
@@ -1013,16 +1016,18 @@ __device__ __forceinline__ void build_axis_tables(const dh_segment *sp, int a, i
                                                   double *S, int tid, int nthr)
 {
     const double nu = sp->nu[a], g = sp->gamma[a], h = sp->h[a];
-    const double bp = sp->beta_p, bq = sp->beta_q;
+    const double bp = sp->beta_p, br = sp->beta_r, bq = sp->beta_q;
+    const int chrom = sp->chrom;
     for (int i = tid; i < n - 1; i += nthr) {
         double vi = d_vfunc(sx[i], nu), vj = d_vfunc(sx[i + 1], nu);
-        if (D == 1) {  // Vfunc_beta, dadi/integration_shared.c:8-10
-            vi = vi * bp / bq;
-            vj = vj * bp / bq;
+        if (D == 1) {  // Vfunc_beta, dadi/integration_shared.c:8-10 (br = 1), or
+                       // the X-chromosome V, dadi/Integration.py:1244-1245
+            vi = vi * bp * br / bq;
+            vj = vj * bp * br / bq;
         }
         W0[i] = vi / (2 * sdx[i]);
         W1[i] = vj / (2 * sdx[i]);
-        S[i] = d_sel(sxi[i], g, h);
+        S[i] = d_sel(sxi[i], g, h, chrom);
     }
 }
 
@@ -1097,7 +1102,7 @@ __device__ __forceinline__ void sweep_tiles(double *__restrict__ phi, int n, lon
                 const long long line = L0 + lane;
                 LineSetup<D, ORDER> L;
                 line_setup<D, ORDER>(L, line < nsys ? line : nsys - 1, n, a, sx, mig, nu, ga, ha,
-                                     sdx[0], sdx[n - 2], first_axis, injm);
+                                     sdx[0], sdx[n - 2], first_axis, injm, sp->chrom);
                 if (line >= nsys) L.inj_row = -1;
                 if constexpr (GG)
                     solve_tile_staged_gg<NO, ORDER, STAGE_C>(
@@ -1121,7 +1126,7 @@ __device__ __forceinline__ void sweep_tiles(double *__restrict__ phi, int n, lon
         if (line < nsys) {
             LineSetup<D, ORDER> L;
             line_setup<D, ORDER>(L, line, n, a, sx, mig, nu, ga, ha, sdx[0], sdx[n - 2],
-                                 first_axis, injm);
+                                 first_axis, injm, sp->chrom);
             long long off, s;
             line_addr<D>(n, ld, a, line, off, s);
             double *p = phi + off;
@@ -1296,7 +1301,7 @@ __device__ __forceinline__ void sweep_packed(int a, const PackSlot *slot, int S,
         const double *W0 = atab + ((long long)k * D + a) * LDS_TABLES_PER_AXIS * n;
         LineSetup<D, ORDER> Ls;
         line_setup<D, ORDER>(Ls, act ? line : 0, n, a, sx, mig, nu, ga, ha, sdx[0], sdx[n - 2],
-                             a == ps.first_axis, ps.injm);
+                             a == ps.first_axis, ps.injm, sp->chrom);
         if (!act) Ls.inj_row = -1;
         if (a == D - 1) {
             PackedRows<C> R;
@@ -1490,16 +1495,17 @@ __global__ void __launch_bounds__(256) k_integrate_1d(dh_instance *__restrict__ 
         while (seg < nseg && budget > 0) {
             const dh_segment *sp = gbl(ip->seg_dev) + seg;
             const double nu = sp->nu[0], g = sp->gamma[0], h = sp->h[0];
-            const double bp = sp->beta_p, bq = sp->beta_q;
+            const double bp = sp->beta_p, br = sp->beta_r, bq = sp->beta_q;
+            const int chrom = sp->chrom;
             for (int i = 0; i < n - 1; ++i) {
                 const double dxi = xx[i + 1] - xx[i];
-                const double vi = d_vfunc(xx[i], nu) * bp / bq;
-                const double vj = d_vfunc(xx[i + 1], nu) * bp / bq;
+                const double vi = d_vfunc(xx[i], nu) * bp * br / bq;
+                const double vj = d_vfunc(xx[i + 1], nu) * bp * br / bq;
                 tW0[i * ts] = vi / (2 * dxi);
                 tW1[i * ts] = vj / (2 * dxi);
-                tS[i * ts] = d_sel(txi[i * ts], g, h);
+                tS[i * ts] = d_sel(txi[i * ts], g, h, chrom);
             }
-            const double sel0 = d_sel(xx[0], g, h), sel1 = d_sel(xx[n - 1], g, h);
+            const double sel0 = d_sel(xx[0], g, h, chrom), sel1 = d_sel(xx[n - 1], g, h, chrom);
             // 1D: the line is the whole axis, both boundary terms always apply
             // when M has the right sign (dadi/integration1D.c:50-53).
             const bool bf = sel0 <= 0, bl = sel1 >= 0;
@@ -1760,7 +1766,7 @@ std::map<std::string, hipGraphExec_t> g_graphs;
 
 extern "C" {
 
-int dh_abi_version(void) { return 3; }
+int dh_abi_version(void) { return 4; }
 size_t dh_sizeof_segment(void) { return sizeof(dh_segment); }
 size_t dh_sizeof_instance(void) { return sizeof(dh_instance); }
 size_t dh_lds_bytes(int D, int n, int waves, int chunk, int recompute, int gamma_global)

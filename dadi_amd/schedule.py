@@ -82,6 +82,7 @@ def const_steps(initial_t, T, dt):
 def _blank(count):
     seg = np.zeros(count, dtype=SEG_DTYPE)
     seg["beta_p"] = 1.0
+    seg["beta_r"] = 1.0
     seg["beta_q"] = 1.0
     seg["nu"] = 1.0
     return seg
@@ -116,6 +117,41 @@ def const_segments(D, xx, initial_t, T, dt, nus, migs, gammas, hs, theta0, beta=
         seg["nsteps"][i] = 1
         seg["dt"][i] = td
         seg["inject"][i] = inject_value(D, td, xx, theta0)
+        i += 1
+    return seg
+
+
+def inject_value_X(dt, xx, theta0, beta, alpha):
+    """Mass added at index 1 per step of an X-chromosome integration,
+    dadi/Integration.py:1248-1254:
+    dt/x1 * theta0/2 * factor * 2/(x2 - x0), factor = 2/(1+2 beta) (1/(alpha+1) + beta),
+    left to right."""
+    factor = 2. / (1. + 2. * beta) * (1. / (alpha + 1.) + beta)
+    return dt / xx[1] * theta0 / 2. * factor * 2. / (xx[2] - xx[0])
+
+
+def xchrom_segments(xx, initial_t, T, dt, nu, gamma, h, beta, alpha, theta0):
+    """Segments of a constant-parameter X-chromosome integration
+    (dadi/Integration.py:1301-1352): V = 1/nu x(1-x) (2 beta+4)(beta+1)/(9 beta)
+    and the X selection term (the kernel's chrom = 1), the X injection."""
+    nfull, tails = const_steps(initial_t, T, dt)
+    count = (1 if nfull else 0) + len(tails)
+    seg = _blank(count)
+    _fill_params(seg, 1, [nu], [[]], [gamma], [h], 1)
+    seg["beta_p"] = 2 * beta + 4.
+    seg["beta_r"] = beta + 1.
+    seg["beta_q"] = 9. * beta
+    seg["chrom"] = 1
+    i = 0
+    if nfull:
+        seg["nsteps"][0] = nfull
+        seg["dt"][0] = dt
+        seg["inject"][0] = inject_value_X(dt, xx, theta0, beta, alpha)
+        i = 1
+    for td in tails:
+        seg["nsteps"][i] = 1
+        seg["dt"][i] = td
+        seg["inject"][i] = inject_value_X(td, xx, theta0, beta, alpha)
         i += 1
     return seg
 

@@ -47,9 +47,13 @@ extern "C" {
  * then sweeps every non-frozen axis in order with step `dt`.  Parameters are
  * per population k; mig[k][j] is the migration rate into population k from
  * the j-th other population counted in increasing order (axis 1 of a
- * 3-population phi: m21, m23). beta_p/beta_q scale V in 1D only:
- * V = x(1-x)/nu * beta_p / beta_q with beta_p = pow(beta+1,2), beta_q = 4*beta
- * (dadi/integration_shared.c:8-10). */
+ * 3-population phi: m21, m23). beta_p, beta_r, beta_q scale V in 1D only:
+ * V = 1/nu x (1-x) * beta_p * beta_r / beta_q, evaluated left to right:
+ *   autosome: beta_p = pow(beta+1,2), beta_r = 1, beta_q = 4*beta
+ *             (dadi/integration_shared.c:8-10);
+ *   X (chrom = 1): beta_p = 2*beta+4, beta_r = beta+1, beta_q = 9*beta, and the
+ *             selection term is gamma*4/3*(0.5+h+x(1-2h))*x(1-x)
+ *             (dadi/Integration.py:1244-1247). */
 typedef struct dh_segment {
     long long nsteps;
     double dt;
@@ -59,6 +63,9 @@ typedef struct dh_segment {
     double gamma[DH_MAXD];
     double h[DH_MAXD];
     double mig[DH_MAXD][DH_MAXD - 1];
+    double beta_r;
+    int chrom;      /* 0: autosome; 1: X chromosome (1 population only) */
+    int reserved;
 } dh_segment;
 
 /* One integration ("instance").  phi_dev holds phi in C order, updated in

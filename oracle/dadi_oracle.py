@@ -16,6 +16,7 @@ Contents, each a restatement of the reference function it cites:
 * ``inject_value``            dadi/Integration.py:70-145
 * ``integrate_const``         dadi/Integration.py:986-1242 (constant parameters)
 * ``integrate_tv``            dadi/Integration.py:178-937 (time-varying loops)
+* ``one_pop_X``               dadi/Integration.py:1244-1352 (X chromosome, constant)
 * ``from_phi``                dadi/Spectrum_mod.py:25-41, 1307-1348, 1509-1548,
                               1774-1871, 1938-2062
 * ``from_phi_direct``         dadi/Spectrum_mod.py:1263-1279, 1350-1380, 1550-1592,
@@ -50,6 +51,16 @@ _ip = ctypes.POINTER(ctypes.c_int)
 
 def _ptr(a):
     return a.ctypes.data_as(_dp)
+
+
+def tridiag(a, b, c, r):
+    """Thomas solve through the C restatement of dadi/tridiag.c:14-40."""
+    n = len(r)
+    a, b, c, r = (np.ascontiguousarray(v, dtype=np.float64) for v in (a, b, c, r))
+    u = np.empty(n)
+    gam = np.empty(n)
+    lib().orc_tridiag(_ptr(a), _ptr(b), _ptr(c), _ptr(r), _ptr(u), n, _ptr(gam))
+    return u
 
 
 _lib = None
@@ -380,6 +391,58 @@ def ref_sweep_pre(phi, axis, arrays, dt):
 def _check(T, initial_t):
     if T - initial_t < 0:
         raise ValueError("Final integration time T is less than initial_time.")
+
+
+def _Vfunc_X(x, nu, beta):
+    """dadi/Integration.py:1244-1245."""
+    return 1. / nu * x * (1 - x) * (2 * beta + 4.) * (beta + 1.) / (9. * beta)
+
+
+def _Mfunc1D_X(x, gamma, h, beta):
+    """dadi/Integration.py:1246-1247."""
+    return gamma * 4. / 3. * (0.5 + h + x * (1 - 2 * h)) * x * (1 - x)
+
+
+def one_pop_X(phi, xx, T, nu=1, gamma=0, h=0.5, beta=1, alpha=1, theta0=1.0, initial_t=0,
+              dt=None):
+    """Constant-parameter X-chromosome integration, dadi/Integration.py:1256-1352
+    (numpy coefficient arrays as the reference forms them, the C Thomas
+    solver).  dt: the time step (default: the reference's _compute_dt)."""
+    phi = np.array(phi, dtype=np.float64, copy=True)
+    if T - initial_t == 0:
+        return phi
+    xx = np.asarray(xx, dtype=np.float64)
+    M = _Mfunc1D_X(xx, gamma, h, beta)
+    MInt = _Mfunc1D_X((xx[:-1] + xx[1:]) / 2, gamma, h, beta)
+    V = _Vfunc_X(xx, nu, beta)
+    dx = np.diff(xx)
+    dfactor = np.zeros(len(dx) + 1)
+    dfactor[1:-1] = 2 / (dx[:-1] + dx[1:])
+    dfactor[0] = 2 / dx[0]
+    dfactor[-1] = 2 / dx[-1]
+    delj = 0.5
+    a = np.zeros(phi.shape)
+    a[1:] += dfactor[1:] * (-MInt * delj - V[:-1] / (2 * dx))
+    c = np.zeros(phi.shape)
+    c[:-1] += -dfactor[:-1] * (-MInt * (1 - delj) + V[1:] / (2 * dx))
+    b = np.zeros(phi.shape)
+    b[:-1] += -dfactor[:-1] * (-MInt * delj - V[:-1] / (2 * dx))
+    b[1:] += dfactor[1:] * (-MInt * (1 - delj) + V[1:] / (2 * dx))
+    if M[0] <= 0:
+        b[0] += (0.5 / nu - M[0]) * 2 / dx[0]
+    if M[-1] >= 0:
+        b[-1] += -(-0.5 / nu - M[-1]) * 2 / dx[-1]
+    if dt is None:
+        dt = compute_dt(nu, [0], gamma, h)
+    factor = 2. / (1. + 2. * beta) * (1. / (alpha + 1.) + beta)
+    current_t = initial_t
+    while current_t < T:
+        this_dt = min(dt, T - current_t)
+        phi[1] += this_dt / xx[1] * theta0 / 2. * factor * 2. / (xx[2] - xx[0])
+        r = phi / this_dt
+        phi = tridiag(a, b + 1. / this_dt, c, r)
+        current_t += this_dt
+    return phi
 
 
 def integrate_const(phi, xx, T, nus, migs, gammas, hs, theta0=1.0, initial_t=0.0,

@@ -37,9 +37,9 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layout_and_errors_without_gpu():
     from dadi_amd import _lib
     L = _lib.load()
-    assert L.dh_sizeof_segment() == _lib.SEG_DTYPE.itemsize == 320
+    assert L.dh_sizeof_segment() == _lib.SEG_DTYPE.itemsize == 336
     assert L.dh_sizeof_instance() == _lib.INST_DTYPE.itemsize
-    assert L.dh_abi_version() == _lib.ABI_VERSION == 3
+    assert L.dh_abi_version() == _lib.ABI_VERSION == 4
     # calls before dh_init fail cleanly with a message (no GPU touched)
     cfg = _lib.DhLaunch(1, 1, 0, 0, 0)
     rc = L.dh_integrate_nd(None, 1, 2, 0, 10, None, cfg, None)
@@ -161,3 +161,20 @@ def test_product_package_never_imports_oracle():
             if fn.endswith((".py", ".hip", ".h", ".c", ".cpp")):
                 text = open(os.path.join(dirpath, fn)).read()
                 assert "oracle" not in text.replace("the oracle", ""), fn
+
+
+def test_one_pop_x_argument_errors():
+    """dadi/Integration.py:1285-1318: errors come before any GPU work."""
+    import dadi_amd
+    xx = np.linspace(0, 1, 20)
+    phi = np.ones(20)
+    with pytest.raises(NotImplementedError):
+        dadi_amd.Integration.one_pop_X(phi, xx, 0.1, nu=lambda t: 1.0)
+    with pytest.raises(ValueError):
+        dadi_amd.Integration.one_pop_X(phi, xx, 0.1, beta=0)
+    with pytest.raises(ValueError):
+        dadi_amd.Integration.one_pop_X(phi, xx, 0.1, alpha=-1)
+    with pytest.raises(ValueError):
+        dadi_amd.Integration.one_pop_X(phi, xx, 0.1, initial_t=0.2)
+    out = dadi_amd.Integration.one_pop_X(phi, xx, 0.1, frozen=True)
+    assert np.array_equal(out, phi) and out is not phi

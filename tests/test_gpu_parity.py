@@ -295,3 +295,25 @@ def test_direct_sampler_matches_oracle(dadi_amd, oracle, D, n, ns, het):
     want = oracle.from_phi_direct(phi, ns, xxs, het)
     fs = dadi_amd.Spectrum.from_phi(phi, ns, xxs, het_ascertained=het, force_direct=True)
     assert np.allclose(fs.data, want, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("nu,gamma,h,beta,alpha,T", [(1.0, 0.0, 0.5, 1.0, 1.0, 0.1),
+                                                     (0.5, -4.0, 0.3, 2.0, 1.5, 0.2),
+                                                     (2.5, 6.0, 0.9, 0.5, 0.7, 0.05)])
+def test_one_pop_x_bit_exact(dadi_amd, oracle, nu, gamma, h, beta, alpha, T):
+    """X-chromosome integration (one_pop_X) through the public API and through
+    the batched lane kernel, bit for bit against the oracle's restatement of
+    dadi/Integration.py:1256-1352."""
+    from dadi_amd import engine, schedule as S, _lib
+    xx = oracle.default_grid(80)
+    phi0 = oracle.phi_1D(xx)
+    want = oracle.one_pop_X(phi0, xx, T, nu, gamma, h, beta, alpha)
+    got = dadi_amd.Integration.one_pop_X(phi0, xx, T, nu, gamma, h, beta, alpha)
+    assert np.array_equal(got, want)
+    ctx = engine.context()
+    dt = S.compute_dt(nu, [0], gamma, h)
+    seg = S.xchrom_segments(xx, 0.0, T, dt, nu, gamma, h, beta, alpha, 1.0)
+    jobs = [engine.Job(ctx.to_dev(phi0), xx, seg, _lib.ORDER_PRE1, 0, 1) for _ in range(3)]
+    engine.run_batch(jobs)
+    for j in jobs:
+        assert np.array_equal(j.phi.cpu().numpy(), want)

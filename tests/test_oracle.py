@@ -193,3 +193,42 @@ def test_direct_sampler_matches_beta_integrals(oracle):
     # a planted mistake (dropping the binomial coefficient) does not pass
     bad = got / comb(m, k)[None, :]
     assert not np.allclose(bad, np.outer(ex, ey), rtol=1e-3)
+
+
+@pytest.mark.parametrize("nu,gamma,h,beta", [(1.0, 0.0, 0.5, 1.0), (0.6, -3.0, 0.2, 2.5),
+                                             (2.0, 4.0, 0.8, 0.4)])
+def test_x_chromosome_step_equals_reparametrised_autosome(oracle, nu, gamma, h, beta):
+    """One X-chromosome step (dadi/Integration.py:1244-1352) is the autosomal
+    1D step (dadi/Integration.py:986-1035) with the breeding-ratio factor
+    (b+1)^2/(4b) = (2 beta+4)(beta+1)/(9 beta), gamma' = 4 gamma/3 and
+    h' = (0.5+h)/2: V and M agree as functions of x and both boundary terms
+    use 0.5/nu.  So the oracle's X restatement must agree with the pinned
+    autosomal sweep up to rounding; without injection (theta0 = 0) to isolate
+    the operator."""
+    xx = oracle.default_grid(60)
+    rng = np.random.default_rng(7)
+    phi = rng.uniform(0.5, 2.0, 60)
+    dt = 1e-3
+    x_step = oracle.one_pop_X(phi, xx, dt, nu, gamma, h, beta, 1.0, theta0=0.0, dt=dt)
+    K = (2 * beta + 4) * (beta + 1) / (9 * beta)      # >= 1.29 for every beta > 0
+    b_a = ((4 * K - 2) + np.sqrt((4 * K - 2) ** 2 - 4)) / 2
+    assert np.isclose((b_a + 1) ** 2 / (4 * b_a), K, rtol=1e-14)
+    auto = phi.copy()
+    oracle.sweep(oracle.ORC_PRE1, auto, 0, xx, nu, [], 4 * gamma / 3, (0.5 + h) / 2, b_a, dt)
+    assert np.allclose(x_step, auto, rtol=1e-11, atol=0)
+    # a planted mistake in the selection term is caught
+    wrong = phi.copy()
+    oracle.sweep(oracle.ORC_PRE1, wrong, 0, xx, nu, [], 2 * gamma, (0.5 + h) / 2, b_a, dt)
+    assert gamma == 0 or not np.allclose(x_step, wrong, rtol=1e-8, atol=0)
+
+
+def test_x_chromosome_injection_factor(oracle):
+    """2/(1+2 beta) (1/(alpha+1) + beta) is 1 at beta = alpha = 1 (the X
+    injection then equals the autosomal one) and 4/5 at beta = 2, alpha = 1 ...
+    checked through the package's schedule against the oracle loop."""
+    from dadi_amd import schedule as S
+    xx = oracle.default_grid(30)
+    for beta, alpha, want in ((1.0, 1.0, 1.0), (2.0, 1.0, 2 / 5 * 2.5), (0.5, 3.0, 1.0 * 0.75)):
+        auto = oracle.inject_value(1, 1e-3, xx, 1.0)
+        assert np.isclose(S.inject_value_X(1e-3, xx, 1.0, beta, alpha), auto * want,
+                          rtol=1e-14)
