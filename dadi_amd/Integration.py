@@ -48,8 +48,6 @@ def _backend_ok():
         raise _lib.BackendUnavailable(
             "the HIP backend is disabled (dadi_amd.cuda_enabled(False)); this package has no "
             "CPU integrator -- use dadi's own CPU path")
-    if use_delj_trick:
-        raise ValueError("delj trick not currently supported in GPU execution")
 
 
 def _is_dev(phi):
@@ -114,7 +112,7 @@ def _run(D, phi, xx, T, nus, migs, gammas, hs, theta0, initial_t, frozen, nomut,
                                  history=hist)
         Demes.cache.append(Demes.IntegrationNonConst(history=hist, deme_ids=deme_ids))
         order = _lib.ORDER_TV
-    return engine.run_single(phi, xx, seg, order, fz, inj)
+    return engine.run_single(phi, xx, seg, order, fz, inj, delj=bool(use_delj_trick))
 
 
 def _check_T(T, initial_t):
@@ -176,7 +174,7 @@ def one_pop_X(phi, xx, T, nu=1, gamma=0, h=0.5, beta=1, alpha=1, theta0=1.0, ini
     dt = _sched.compute_dt(nu, [0], gamma, h, timescale_factor, use_old_timestep,
                            old_timescale_factor, dx)
     seg = _sched.xchrom_segments(xx, initial_t, T, dt, nu, gamma, h, beta, alpha, theta0)
-    return engine.run_single(phi, xx, seg, _lib.ORDER_PRE1, 0, 1)
+    return engine.run_single(phi, xx, seg, _lib.ORDER_PRE1, 0, 1, delj=bool(use_delj_trick))
 
 
 def two_pops(phi, xx, T, nu1=1, nu2=1, m12=0, m21=0, gamma1=0, gamma2=0,

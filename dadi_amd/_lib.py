@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("DADI_AMD_LIB") or os.path.join(HERE, "libdadi_hip.so")
 
 MAXD = 5
-ABI_VERSION = 4
+ABI_VERSION = 5
 ORDER_TV, ORDER_PRE, ORDER_PRE1 = 0, 1, 2
 
 #: numpy mirror of ``dh_segment``
@@ -59,7 +59,8 @@ class DhLaunch(ctypes.Structure):
     _fields_ = [("waves", ctypes.c_int), ("groups", ctypes.c_int),
                 ("max_steps", ctypes.c_longlong), ("chunk", ctypes.c_int),
                 ("recompute", ctypes.c_int), ("gamma_dev", ctypes.c_void_p),
-                ("gamma_doubles", ctypes.c_longlong)]
+                ("gamma_doubles", ctypes.c_longlong), ("delj", ctypes.c_int),
+                ("reserved", ctypes.c_int)]
 
 
 _vp = ctypes.c_void_p
@@ -69,7 +70,7 @@ _SIGS = {
     "dh_abi_version": ([], ctypes.c_int),
     "dh_sizeof_segment": ([], ctypes.c_size_t),
     "dh_sizeof_instance": ([], ctypes.c_size_t),
-    "dh_lds_bytes": ([ctypes.c_int] * 6, ctypes.c_size_t),
+    "dh_lds_bytes": ([ctypes.c_int] * 7, ctypes.c_size_t),
     "dh_nd_groups": ([ctypes.c_int] * 8, ctypes.c_int),
     "dh_max_lds": ([], ctypes.c_size_t),
     "dh_integrate_nd": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp,

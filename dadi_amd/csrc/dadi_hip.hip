@@ -100,6 +100,34 @@ __device__ __forceinline__ T *gbl(T *const &field)
     return (T *)*(const GP *)&field;
 }
 
+// Chang and Cooper's delta of one interval (use_delj_trick): wj = 2 M dx,
+// epsj = exp(wj/VInt), delta = (-epsj wj + epsj VInt - VInt)/(wj - epsj wj).
+// The C sweeps (time-varying order) fall back to 1/2 when epsj == 1 or
+// wj == 0 (dadi/integration_shared.c:52-70); the numpy coefficient arrays of
+// the constant-parameter paths replace NaN and inf by 1/2
+// (dadi/Integration.py:960-984).  exp is the device's, so results agree
+// with the reference to rounding, not bit for bit.
+template <int ORDER>
+__device__ __forceinline__ double d_delj(double M, double VI, double dx)
+{
+    const double wj = 2 * M * dx;
+    const double epsj = exp(wj / VI);
+    if constexpr (ORDER == DH_ORDER_TV) {
+        if ((epsj != 1.0) && (wj != 0)) return (-epsj * wj + epsj * VI - VI) / (wj - epsj * wj);
+        return 0.5;
+    } else {
+        const double d = (-epsj * wj + epsj * VI - VI) / (wj - epsj * wj);
+        return (__builtin_isnan(d) || __builtin_isinf(d)) ? 0.5 : d;
+    }
+}
+
+// Per-interval tables of the delta trick: V at the interval midpoints and
+// the interval widths, with the stride of the other coefficient tables.
+struct DeljTabs {
+    const double *VI = nullptr;
+    const double *DX = nullptr;
+};
+
 // Compile-time tunables (defaults measured best on MI355X):
 //  DH_PF       rows fetched ahead of use in the Thomas sweeps.  The store of
 //              row j and the load of row j+PF go to different addresses, so
@@ -124,7 +152,7 @@ constexpr int STAGE_C = DH_STAGE_C;
 // Coefficients a_j, b_j and c_{j-1} of row j >= 1 (compute_abc_nobc,
 // dadi/integration_shared.c:72-93, with the boundary term of the last row);
 // pa/pc carry atemp/ctemp of interface j-1 and are advanced to interface j.
-template <int NO, int ORDER>
+template <int NO, int ORDER, int DJ = 0>
 __device__ __forceinline__ void row_coeffs(int j, int n, const double *__restrict__ xInt,
                                            const double *__restrict__ df,
                                            const double *__restrict__ W0,
@@ -132,7 +160,7 @@ __device__ __forceinline__ void row_coeffs(int j, int n, const double *__restric
                                            const double *__restrict__ S, long long ts,
                                            const double *c, const double *m, double inv_dt,
                                            bool bcl_on, double bcl, double &pa, double &pc,
-                                           double &a, double &b, double &cm)
+                                           double &a, double &b, double &cm, DeljTabs dj = {})
 {
     const double dfj = df[j * ts];
     a = -dfj * pa;
@@ -140,8 +168,9 @@ __device__ __forceinline__ void row_coeffs(int j, int n, const double *__restric
     double na = 0., nc = 0.;
     if (j < n - 1) {
         const double Mj = d_M<NO>(c, m, xInt[j * ts], S[j * ts]);
-        na = Mj * 0.5 + W0[j * ts];
-        nc = -Mj * (1 - 0.5) + W1[j * ts];
+        const double dl = DJ ? d_delj<ORDER>(Mj, dj.VI[j * ts], dj.DX[j * ts]) : 0.5;
+        na = Mj * dl + W0[j * ts];
+        nc = -Mj * (1 - dl) + W1[j * ts];
         if constexpr (ORDER == DH_ORDER_TV)
             b = (inv_dt + dfj * pc) + dfj * na;
         else
@@ -161,18 +190,20 @@ __device__ __forceinline__ void row_coeffs(int j, int n, const double *__restric
 }
 
 // Row 0: coefficients, boundary term and the first Thomas step.
-template <int NO, int ORDER>
+template <int NO, int ORDER, int DJ = 0>
 __device__ __forceinline__ void row0(double p0, const double *__restrict__ xInt,
                                      const double *__restrict__ df,
                                      const double *__restrict__ W0,
                                      const double *__restrict__ W1,
                                      const double *__restrict__ S, const double *c,
                                      const double *m, double dt, double inv_dt, bool bcf_on,
-                                     double bcf, double &bet, double &u, double &pa, double &pc)
+                                     double bcf, double &bet, double &u, double &pa, double &pc,
+                                     DeljTabs dj = {})
 {
     const double M = d_M<NO>(c, m, xInt[0], S[0]);
-    pa = M * 0.5 + W0[0];
-    pc = -M * (1 - 0.5) + W1[0];
+    const double dl = DJ ? d_delj<ORDER>(M, dj.VI[0], dj.DX[0]) : 0.5;
+    pa = M * dl + W0[0];
+    pc = -M * (1 - dl) + W1[0];
     double b;
     if constexpr (ORDER == DH_ORDER_TV) {
         b = inv_dt + df[0] * pa;
@@ -189,7 +220,7 @@ __device__ __forceinline__ void row0(double p0, const double *__restrict__ xInt,
 
 // One forward-elimination row j >= 1 (dadi/tridiag.c:25-29); pj is phi at
 // row j.  Returns gamma_j; updates bet, u (= u'_j) and the coefficient carry.
-template <int NO, int ORDER>
+template <int NO, int ORDER, int DJ = 0>
 __device__ __forceinline__ double fwd_core(int j, int n, double pj,
                                            const double *__restrict__ xInt,
                                            const double *__restrict__ df,
@@ -198,11 +229,12 @@ __device__ __forceinline__ double fwd_core(int j, int n, double pj,
                                            const double *__restrict__ S, long long ts,
                                            const double *c, const double *m, double dt,
                                            double inv_dt, bool bcl_on, double bcl,
-                                           double &bet, double &u, double &pa, double &pc)
+                                           double &bet, double &u, double &pa, double &pc,
+                                           DeljTabs dj = {})
 {
     double a, b, cm;
-    row_coeffs<NO, ORDER>(j, n, xInt, df, W0, W1, S, ts, c, m, inv_dt, bcl_on, bcl, pa, pc, a,
-                          b, cm);
+    row_coeffs<NO, ORDER, DJ>(j, n, xInt, df, W0, W1, S, ts, c, m, inv_dt, bcl_on, bcl, pa, pc,
+                              a, b, cm, dj);
     const double g = cm / bet;
     bet = b - a * g;
     const double r = (ORDER == DH_ORDER_PRE) ? inv_dt * pj : pj / dt;
@@ -210,7 +242,7 @@ __device__ __forceinline__ double fwd_core(int j, int n, double pj,
     return g;
 }
 
-template <int NO, int ORDER>
+template <int NO, int ORDER, int DJ = 0>
 __device__ __forceinline__ void fwd_row(int j, int n, double pj,
                                         const double *__restrict__ xInt,
                                         const double *__restrict__ df,
@@ -221,10 +253,11 @@ __device__ __forceinline__ void fwd_row(int j, int n, double pj,
                                         double inv_dt, bool bcl_on, double bcl,
                                         double &bet, double &u, double &pa, double &pc,
                                         double *__restrict__ p, long long s,
-                                        double *__restrict__ gam, long long gs)
+                                        double *__restrict__ gam, long long gs,
+                                        DeljTabs dj = {})
 {
-    const double g = fwd_core<NO, ORDER>(j, n, pj, xInt, df, W0, W1, S, ts, c, m, dt, inv_dt,
-                                         bcl_on, bcl, bet, u, pa, pc);
+    const double g = fwd_core<NO, ORDER, DJ>(j, n, pj, xInt, df, W0, W1, S, ts, c, m, dt,
+                                             inv_dt, bcl_on, bcl, bet, u, pa, pc, dj);
     gam[j * gs] = g;
     p[j * s] = u;
 }
@@ -261,7 +294,7 @@ __device__ __forceinline__ void regamma(int cb, int n, double bt,
 // One line: inject (optional), build a/b/c on the fly, Thomas forward with u'
 // over phi in place and gamma_j in `gam`, then back substitution.
 //   ts: stride of the 1D tables; gs: stride of the gamma column.
-template <int NO, int ORDER>
+template <int NO, int ORDER, int DJ = 0>
 __device__ __forceinline__ void solve_line(
     double *__restrict__ p, long long s, int n,
     const double *__restrict__ xInt, const double *__restrict__ df,
@@ -271,7 +304,7 @@ __device__ __forceinline__ void solve_line(
     double dt, double inv_dt,
     bool bcf_on, double bcf, bool bcl_on, double bcl,
     int inj_row, double inj,
-    double *__restrict__ gam, long long gs)
+    double *__restrict__ gam, long long gs, DeljTabs dj = {})
 {
     if (inj_row >= 0) p[inj_row * s] = p[inj_row * s] + inj;
 
@@ -285,7 +318,8 @@ __device__ __forceinline__ void solve_line(
 
     // row 0 (compute_abc_nobc, dadi/integration_shared.c:72-93)
     double bet, u, pa, pc;
-    row0<NO, ORDER>(p[0], xInt, df, W0, W1, S, c, m, dt, inv_dt, bcf_on, bcf, bet, u, pa, pc);
+    row0<NO, ORDER, DJ>(p[0], xInt, df, W0, W1, S, c, m, dt, inv_dt, bcf_on, bcf, bet, u, pa,
+                        pc, dj);
     p[0] = u;
 
     // forward elimination, PF rows per block, the next block's rows loading
@@ -302,8 +336,8 @@ __device__ __forceinline__ void solve_line(
         for (int q = 0; q < PF; ++q) {
             const int j = j0 + q;
             if (j < n)
-                fwd_row<NO, ORDER>(j, n, cur[q], xInt, df, W0, W1, S, ts, c, m, dt, inv_dt,
-                                   bcl_on, bcl, bet, u, pa, pc, p, s, gam, gs);
+                fwd_row<NO, ORDER, DJ>(j, n, cur[q], xInt, df, W0, W1, S, ts, c, m, dt, inv_dt,
+                                       bcl_on, bcl, bet, u, pa, pc, p, s, gam, gs, dj);
         }
     }
 
@@ -1013,7 +1047,8 @@ template <int D>
 __device__ __forceinline__ void build_axis_tables(const dh_segment *sp, int a, int n,
                                                   const double *sx, const double *sxi,
                                                   const double *sdx, double *W0, double *W1,
-                                                  double *S, int tid, int nthr)
+                                                  double *S, int tid, int nthr,
+                                                  double *VI = nullptr)
 {
     const double nu = sp->nu[a], g = sp->gamma[a], h = sp->h[a];
     const double bp = sp->beta_p, br = sp->beta_r, bq = sp->beta_q;
@@ -1028,6 +1063,11 @@ __device__ __forceinline__ void build_axis_tables(const dh_segment *sp, int a, i
         W0[i] = vi / (2 * sdx[i]);
         W1[i] = vj / (2 * sdx[i]);
         S[i] = d_sel(sxi[i], g, h, chrom);
+        if (VI) {   // V at the interval midpoint, for the delta trick
+            double v = d_vfunc(sxi[i], nu);
+            if (D == 1) v = v * bp * br / bq;
+            VI[i] = v;
+        }
     }
 }
 
@@ -1074,7 +1114,7 @@ __device__ __forceinline__ void line_addr(int n, long long ld, int a, long long 
 
 // All lines of axis `a`, tiles of 64 lines per wave.  GG: gamcol is this
 // lane's gamma column in the global workspace (rows 64 doubles apart).
-template <int D, int ORDER, int CK, int RU, int GG>
+template <int D, int ORDER, int CK, int RU, int GG, int DJ = 0>
 __device__ __forceinline__ void sweep_tiles(double *__restrict__ phi, int n, long long ld, int a,
                                             bool first_axis,
                                             int injm, const dh_segment *sp,
@@ -1083,9 +1123,10 @@ __device__ __forceinline__ void sweep_tiles(double *__restrict__ phi, int n, lon
                                             const double *W0, const double *W1, const double *S,
                                             double dt, double inv_dt, double *gamcol,
                                             double *stage, long long tile0, long long tstep,
-                                            int lane)
+                                            int lane, DeljTabs dj = {})
 {
     constexpr int NO = D - 1;
+    static_assert(!DJ || (CK == 0 && !RU && !GG), "the delta trick uses the plain Thomas path");
     long long nsys = 1;
     for (int q = 0; q < NO; ++q) nsys *= n;
     double mig[NO > 0 ? NO : 1];
@@ -1131,9 +1172,9 @@ __device__ __forceinline__ void sweep_tiles(double *__restrict__ phi, int n, lon
             line_addr<D>(n, ld, a, line, off, s);
             double *p = phi + off;
             if constexpr (CK == 0)
-                solve_line<NO, ORDER>(p, s, n, sxi, sdf, W0, W1, S, 1, L.c, mig, dt, inv_dt,
-                                      L.bcf_on, L.bcf, L.bcl_on, L.bcl, L.inj_row, inj, gamcol,
-                                      64);
+                solve_line<NO, ORDER, DJ>(p, s, n, sxi, sdf, W0, W1, S, 1, L.c, mig, dt, inv_dt,
+                                          L.bcf_on, L.bcf, L.bcl_on, L.bcl, L.inj_row, inj,
+                                          gamcol, 64, dj);
             else if constexpr (RU)
                 solve_line_ru<NO, ORDER, CK>(p, s, n, sxi, sdf, W0, W1, S, 1, L.c, mig, dt,
                                              inv_dt, L.bcf_on, L.bcf, L.bcl_on, L.bcl, L.inj_row,
@@ -1150,7 +1191,7 @@ __device__ __forceinline__ void sweep_tiles(double *__restrict__ phi, int n, lon
 // Kernel 1: persistent, one workgroup per instance (any D), whole schedule.
 // LDS: x, xInt, dfactor, dx (4n) | per axis W0,W1,S (3Dn) | gamma (waves*64*n)
 // ---------------------------------------------------------------------------
-template <int D, int ORDER, int CK, int RU, int GG>
+template <int D, int ORDER, int CK, int RU, int GG, int DJ = 0>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(DH_WPE, 8)))
 k_integrate_nd(dh_instance *__restrict__ tab, int ninst, int *__restrict__ counter,
                long long max_steps, double *__restrict__ gws, int nmax)
@@ -1169,16 +1210,17 @@ k_integrate_nd(dh_instance *__restrict__ tab, int ninst, int *__restrict__ count
         const int n = ip->n;
         double *sx = lds, *sxi = sx + n, *sdf = sxi + n, *sdx = sdf + n;
         double *tabs = sdx + n;
+        constexpr int TPA = LDS_TABLES_PER_AXIS + DJ;   // + V at midpoints (delta trick)
         // per-lane scratch doubles: the gamma column, or (bet[, u']) checkpoints
         const int col = CK ? ((n - 1) / (CK ? CK : 1) + 1) * (RU ? 2 : 1) : n;
         const int sc = RU ? CK : STAGE_C;
         double *gamcol, *stage;
         if constexpr (GG) {
             gamcol = gws + ((long long)blockIdx.x * nw + wave) * 64 * nmax + lane;
-            stage = tabs + LDS_TABLES_PER_AXIS * D * n + (long long)wave * 64 * (STAGE_C + 1);
+            stage = tabs + TPA * D * n + (long long)wave * 64 * (STAGE_C + 1);
         } else {
-            gamcol = tabs + LDS_TABLES_PER_AXIS * D * n + (long long)wave * 64 * col + lane;
-            stage = tabs + LDS_TABLES_PER_AXIS * D * n + (long long)nw * 64 * col +
+            gamcol = tabs + TPA * D * n + (long long)wave * 64 * col + lane;
+            stage = tabs + TPA * D * n + (long long)nw * 64 * col +
                     (long long)wave * 64 * (sc + 1);
         }
         build_grid_tables(gbl(ip->xx_dev), n, sx, sxi, sdf, sdx, tid, nthr);
@@ -1193,8 +1235,9 @@ k_integrate_nd(dh_instance *__restrict__ tab, int ninst, int *__restrict__ count
             __syncthreads();
             for (int a = 0; a < D; ++a) {
                 if ((frozen >> a) & 1) continue;
-                double *W0 = tabs + LDS_TABLES_PER_AXIS * a * n;
-                build_axis_tables<D>(sp, a, n, sx, sxi, sdx, W0, W0 + n, W0 + 2 * n, tid, nthr);
+                double *W0 = tabs + TPA * a * n;
+                build_axis_tables<D>(sp, a, n, sx, sxi, sdx, W0, W0 + n, W0 + 2 * n, tid, nthr,
+                                     DJ ? W0 + 3 * n : nullptr);
             }
             __syncthreads();
             const double dt = sp->dt, inv_dt = 1. / dt;
@@ -1204,10 +1247,13 @@ k_integrate_nd(dh_instance *__restrict__ tab, int ninst, int *__restrict__ count
                 bool first = true;
                 for (int a = 0; a < D; ++a) {
                     if ((frozen >> a) & 1) continue;
-                    const double *W0 = tabs + LDS_TABLES_PER_AXIS * a * n;
-                    sweep_tiles<D, ORDER, CK, RU, GG>(phi, n, ld, a, first, injm, sp, sx, sxi, sdf, sdx, W0,
-                                          W0 + n, W0 + 2 * n, dt, inv_dt, gamcol, stage, wave, nw,
-                                          lane);
+                    const double *W0 = tabs + TPA * a * n;
+                    DeljTabs dj;
+                    if constexpr (DJ) dj = DeljTabs{W0 + 3 * n, sdx};
+                    sweep_tiles<D, ORDER, CK, RU, GG, DJ>(phi, n, ld, a, first, injm, sp, sx, sxi,
+                                                          sdf, sdx, W0, W0 + n, W0 + 2 * n, dt,
+                                                          inv_dt, gamcol, stage, wave, nw, lane,
+                                                          dj);
                     __syncthreads();
                     first = false;
                 }
@@ -1457,7 +1503,7 @@ k_integrate_packed(dh_instance *__restrict__ tab, int ninst, int *__restrict__ c
 // Scratch layout (coalesced over lanes): [q][i][slot], q = xInt, df, W0, W1,
 // S, gamma.
 // ---------------------------------------------------------------------------
-template <int ORDER>
+template <int ORDER, int DJ = 0>
 __global__ void __launch_bounds__(256) k_integrate_1d(dh_instance *__restrict__ tab, int ninst,
                                                       int *__restrict__ counter,
                                                       double *__restrict__ scratch,
@@ -1470,6 +1516,7 @@ __global__ void __launch_bounds__(256) k_integrate_1d(dh_instance *__restrict__ 
     const long long qs = (long long)nmax * nlanes;  // stride between tables
     double *txi = T, *tdf = T + qs, *tW0 = T + 2 * qs, *tW1 = T + 3 * qs, *tS = T + 4 * qs;
     double *tg = T + 5 * qs;
+    double *tVI = T + 6 * qs, *tDX = T + 7 * qs;   // delta trick only
     const long long ts = nlanes;
     for (;;) {
         const int id = atomicAdd(counter, 1);
@@ -1485,6 +1532,8 @@ __global__ void __launch_bounds__(256) k_integrate_1d(dh_instance *__restrict__ 
             else v = 2. / ((xx[i + 1] - xx[i]) + (xx[i] - xx[i - 1]));
             tdf[i * ts] = v;
         }
+        if constexpr (DJ)
+            for (int i = 0; i < n - 1; ++i) tDX[i * ts] = xx[i + 1] - xx[i];
         const double dx0 = xx[1] - xx[0], dxl = xx[n - 1] - xx[n - 2];
         long long seg = ip->cur_seg, step = ip->cur_step;
         const long long nseg = ip->nseg;
@@ -1504,6 +1553,7 @@ __global__ void __launch_bounds__(256) k_integrate_1d(dh_instance *__restrict__ 
                 tW0[i * ts] = vi / (2 * dxi);
                 tW1[i * ts] = vj / (2 * dxi);
                 tS[i * ts] = d_sel(txi[i * ts], g, h, chrom);
+                if constexpr (DJ) tVI[i * ts] = d_vfunc(txi[i * ts], nu) * bp * br / bq;
             }
             const double sel0 = d_sel(xx[0], g, h, chrom), sel1 = d_sel(xx[n - 1], g, h, chrom);
             // 1D: the line is the whole axis, both boundary terms always apply
@@ -1516,9 +1566,10 @@ __global__ void __launch_bounds__(256) k_integrate_1d(dh_instance *__restrict__ 
             if (nst > budget) nst = budget;
             if (!frz) {
                 for (long long k = 0; k < nst; ++k)
-                    solve_line<0, ORDER>(phi, 1, n, txi, tdf, tW0, tW1, tS, ts, nullptr, nullptr,
-                                         dt, inv_dt, bf, bcf, bl, bcl, injon ? 1 : -1,
-                                         sp->inject, tg, ts);
+                    solve_line<0, ORDER, DJ>(phi, 1, n, txi, tdf, tW0, tW1, tS, ts, nullptr,
+                                             nullptr, dt, inv_dt, bf, bcf, bl, bcl,
+                                             injon ? 1 : -1, sp->inject, tg, ts,
+                                             DeljTabs{tVI, tDX});
             }
             budget -= nst;
             step += nst;
@@ -1638,8 +1689,9 @@ __global__ void k_contract_axis(const double *__restrict__ in, long long P, int 
 namespace {
 
 template <int D, int ORDER>
-const void *nd_kernel(int chunk, int recompute, int gg)
+const void *nd_kernel(int chunk, int recompute, int gg, int dj)
 {
+    if (dj) return reinterpret_cast<const void *>(&k_integrate_nd<D, ORDER, 0, 0, 0, 1>);
     if (gg) return reinterpret_cast<const void *>(&k_integrate_nd<D, ORDER, 0, 0, 1>);
     if (chunk == 8 && recompute)
         return reinterpret_cast<const void *>(&k_integrate_nd<D, ORDER, 8, 1, 0>);
@@ -1647,10 +1699,10 @@ const void *nd_kernel(int chunk, int recompute, int gg)
     return reinterpret_cast<const void *>(&k_integrate_nd<D, ORDER, 0, 0, 0>);
 }
 
-const void *pick_nd(int D, int order, int chunk, int recompute, int gg)
+const void *pick_nd(int D, int order, int chunk, int recompute, int gg, int dj)
 {
     switch (D * 4 + order) {
-#define DH_CASE(d, o) case d * 4 + o: return nd_kernel<d, o>(chunk, recompute, gg);
+#define DH_CASE(d, o) case d * 4 + o: return nd_kernel<d, o>(chunk, recompute, gg, dj);
         DH_CASE(1, 0) DH_CASE(1, 1) DH_CASE(1, 2)
         DH_CASE(2, 0) DH_CASE(2, 1) DH_CASE(2, 2)
         DH_CASE(3, 0) DH_CASE(3, 1) DH_CASE(3, 2)
@@ -1706,22 +1758,26 @@ const void *pick_sweep(int D, int order)
     return nullptr;
 }
 
-const void *pick_1d(int order)
+const void *pick_1d(int order, int dj)
 {
-    switch (order) {
-    case 0: return reinterpret_cast<const void *>(&k_integrate_1d<0>);
-    case 1: return reinterpret_cast<const void *>(&k_integrate_1d<1>);
-    case 2: return reinterpret_cast<const void *>(&k_integrate_1d<2>);
+    switch (order * 2 + (dj ? 1 : 0)) {
+    case 0: return reinterpret_cast<const void *>(&k_integrate_1d<0, 0>);
+    case 1: return reinterpret_cast<const void *>(&k_integrate_1d<0, 1>);
+    case 2: return reinterpret_cast<const void *>(&k_integrate_1d<1, 0>);
+    case 3: return reinterpret_cast<const void *>(&k_integrate_1d<1, 1>);
+    case 4: return reinterpret_cast<const void *>(&k_integrate_1d<2, 0>);
+    case 5: return reinterpret_cast<const void *>(&k_integrate_1d<2, 1>);
     }
     return nullptr;
 }
 
-size_t nd_lds(int D, int n, int waves, int chunk, int recompute, int gg)
+size_t nd_lds(int D, int n, int waves, int chunk, int recompute, int gg, int dj = 0)
 {
     const size_t col = gg ? 0
                           : chunk ? (size_t)((n - 1) / chunk + 1) * (recompute ? 2 : 1) : (size_t)n;
     const size_t stage = (chunk || gg) ? 64 * (size_t)((recompute ? chunk : STAGE_C) + 1) : 0;
-    return sizeof(double) * ((size_t)n * (LDS_GRID_TABLES + LDS_TABLES_PER_AXIS * (size_t)D) +
+    const size_t tpa = LDS_TABLES_PER_AXIS + (dj ? 1 : 0);
+    return sizeof(double) * ((size_t)n * (LDS_GRID_TABLES + tpa * (size_t)D) +
                              (size_t)waves * (64 * col + stage));
 }
 
@@ -1766,12 +1822,13 @@ std::map<std::string, hipGraphExec_t> g_graphs;
 
 extern "C" {
 
-int dh_abi_version(void) { return 4; }
+int dh_abi_version(void) { return 5; }
 size_t dh_sizeof_segment(void) { return sizeof(dh_segment); }
 size_t dh_sizeof_instance(void) { return sizeof(dh_instance); }
-size_t dh_lds_bytes(int D, int n, int waves, int chunk, int recompute, int gamma_global)
+size_t dh_lds_bytes(int D, int n, int waves, int chunk, int recompute, int gamma_global,
+                    int delj)
 {
-    return nd_lds(D, n, waves, chunk, recompute, gamma_global != 0);
+    return nd_lds(D, n, waves, chunk, recompute, gamma_global != 0, delj != 0);
 }
 
 int dh_nd_groups(int D, int nmax, int waves, int chunk, int recompute, int gamma_global,
@@ -1826,13 +1883,15 @@ int dh_integrate_nd(dh_instance *table_dev, int ninst, int D, int order, int nma
     const int waves = cfg->waves;
     if (waves < 1 || waves > 8) return fail(DH_EINVAL, "waves must be 1..8");
     const int chunk = cfg->chunk, recompute = cfg->recompute;
-    const int gg = cfg->gamma_dev != nullptr;
+    const int gg = cfg->gamma_dev != nullptr, dj = cfg->delj != 0;
     if (chunk != 0 && chunk != 8) return fail(DH_EINVAL, "chunk must be 0 or 8");
     if (recompute && chunk != 8) return fail(DH_EINVAL, "recompute needs chunk 8");
     if (gg && (chunk || recompute))
         return fail(DH_EINVAL, "a global gamma workspace needs chunk 0 and recompute 0");
-    const void *fn = pick_nd(D, order, chunk, recompute, gg);
-    const size_t lds = nd_lds(D, nmax, waves, chunk, recompute, gg);
+    if (dj && (chunk || recompute || gg))
+        return fail(DH_EINVAL, "the delta trick needs chunk 0 without recompute or workspace");
+    const void *fn = pick_nd(D, order, chunk, recompute, gg, dj);
+    const size_t lds = nd_lds(D, nmax, waves, chunk, recompute, gg, dj);
     if (int e = set_lds(fn, lds)) return e;
     const int groups = nd_groups(lds, waves, cfg->groups, ninst);
     if (gg && cfg->gamma_doubles < (long long)groups * waves * 64 * nmax)
@@ -1861,8 +1920,9 @@ int dh_integrate_packed(dh_instance *table_dev, int ninst, int D, int order, int
         return fail(DH_EINVAL, "slots must be 1.." + std::to_string(DH_PACK_MAX));
     const int waves = cfg->waves;
     if (waves < 1 || waves > 8) return fail(DH_EINVAL, "waves must be 1..8");
-    if (cfg->chunk != 8 || cfg->recompute || cfg->gamma_dev)
-        return fail(DH_EINVAL, "packed launches use chunk 8 without recompute or workspace");
+    if (cfg->chunk != 8 || cfg->recompute || cfg->gamma_dev || cfg->delj)
+        return fail(DH_EINVAL, "packed launches use chunk 8 without recompute, workspace or "
+                               "delta trick");
     const void *fn = pick_packed(D, order);
     const size_t dyn = packed_dyn_lds(D, n, waves, slots);
     if (dyn + PACK_STATIC_LDS > g_max_lds)
@@ -1894,14 +1954,16 @@ int dh_integrate_1d(dh_instance *table_dev, int ninst, int order, int nmax, int 
     long long groups = cfg->groups;
     if (groups <= 0) groups = (ninst + thr - 1) / thr;
     const long long nlanes = groups * thr;
-    if (6LL * nmax * nlanes > scratch_doubles)
-        return fail(DH_ERANGE, "1D scratch too small: need " + std::to_string(6LL * nmax * nlanes) +
-                                   " doubles");
+    const long long tables = cfg->delj ? 8 : 6;
+    if (tables * nmax * nlanes > scratch_doubles)
+        return fail(DH_ERANGE, "1D scratch too small: need " +
+                                   std::to_string(tables * nmax * nlanes) + " doubles");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, counter_dev, 0);
     long long ms = cfg->max_steps;
     void *args[] = {&table_dev, &ninst, &counter_dev, &scratch_dev, (void *)&nlanes, &nmax, &ms};
-    DH_HIP(hipLaunchKernel(pick_1d(order), dim3((unsigned)groups), dim3(thr), args, 0, st));
+    DH_HIP(hipLaunchKernel(pick_1d(order, cfg->delj), dim3((unsigned)groups), dim3(thr), args, 0,
+                           st));
     return DH_OK;
 }
 

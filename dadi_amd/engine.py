@@ -83,11 +83,11 @@ class Context:
             pool.append(t.cuda.Stream(self.device))
         return pool[:k]
 
-    def max_waves(self, D, n, chunk=0, recompute=0, gamma_global=0):
+    def max_waves(self, D, n, chunk=0, recompute=0, gamma_global=0, delj=0):
         """Largest waves-per-workgroup whose LDS fits (mirrors dh_lds_bytes)."""
         L = self.lib
-        fixed = L.dh_lds_bytes(D, n, 0, chunk, recompute, gamma_global)
-        per_wave = L.dh_lds_bytes(D, n, 1, chunk, recompute, gamma_global) - fixed
+        fixed = L.dh_lds_bytes(D, n, 0, chunk, recompute, gamma_global, delj)
+        per_wave = L.dh_lds_bytes(D, n, 1, chunk, recompute, gamma_global, delj) - fixed
         return max(0, min(8, (self.max_lds - fixed) // per_wave))
 
     #: resident waves per compute unit the nD kernels reach (register-limited:
@@ -122,7 +122,7 @@ class Context:
 
     def _plain_score(self, D, n, sh):
         L = n ** (D - 1)
-        lds = self.lib.dh_lds_bytes(D, n, sh.waves, sh.chunk, sh.recompute, sh.gamma_global)
+        lds = self.lib.dh_lds_bytes(D, n, sh.waves, sh.chunk, sh.recompute, sh.gamma_global, 0)
         util = L / (64.0 * sh.waves * -(-L // (64 * sh.waves)))
         return util * self._occupancy(lds, sh.waves)
 
@@ -201,12 +201,23 @@ class Context:
 
     def nd_launch(self, tab_ptr, ninst, D, order, nmax, counter_ptr, stream, workspace,
                   waves=None, groups=0, max_steps=0, chunk=None, recompute=None,
-                  gamma_global=None, slots=None, packable=False):
+                  gamma_global=None, slots=None, packable=False, delj=False):
         """One dh_integrate_nd (or, with slots >= 2, dh_integrate_packed)
         call.  ``workspace`` is a dict owned by the caller (one per stream:
         launches that may overlap must not share it) that keeps the gamma
         workspace between calls.  ``packable``: every instance has n = nmax
         and the same grid."""
+        if delj:   # the delta trick runs on the plain Thomas path only
+            tiles = -(-(nmax ** (D - 1)) // 64)
+            w = waves or max(1, min(8, tiles, self.max_waves(D, nmax, 0, 0, 0, 1)))
+            if self.max_waves(D, nmax, 0, 0, 0, 1) < 1:
+                raise NotImplementedError("the delta trick needs the full gamma column in LDS, "
+                                          "which n=%d at D=%d exceeds" % (nmax, D))
+            cfg = DhLaunch(w, groups, max_steps, 0, 0)
+            cfg.delj = 1
+            check(self.lib.dh_integrate_nd(tab_ptr, ninst, D, order, nmax, counter_ptr, cfg,
+                                           stream), "dh_integrate_nd")
+            return Shape(w, 0, 0, 0, 1)
         sh = self.shape(D, nmax, waves, chunk, recompute, gamma_global, slots, packable, ninst)
         cfg = DhLaunch(sh.waves, groups, max_steps, sh.chunk, sh.recompute)
         if sh.slots >= 2:
@@ -302,11 +313,12 @@ def _inst_record(phi_ptr, xx_ptr, seg_ptr, nseg, D, n, order, frozen, inject, ld
 
 
 def run_single(phi, xx, seg, order, frozen, inject, ctx=None, mode="auto", waves=None,
-               steps_per_graph=32):
+               steps_per_graph=32, delj=False):
     """Integrate one phi in place on the device.
 
-    phi: torch float64 tensor on the device (C-contiguous, shape (n,)*D), or a
-    numpy array (copied in and out; the numpy result is returned).
+    phi: torch float64 tensor on the device (C order, shape (n,)*D, last axis
+    possibly padded), or a numpy array (copied in and out; the numpy result
+    is returned).  delj: Chang and Cooper's delta (dadi's use_delj_trick).
     """
     t = torch()
     ctx = ctx or context()
@@ -319,7 +331,7 @@ def run_single(phi, xx, seg, order, frozen, inject, ctx=None, mode="auto", waves
         phid = phi
     co = getattr(_local, "coalescer", None)
     if co is not None and len(seg) and frozen != (1 << phid.dim()) - 1:
-        job = Job(phid, xx, seg, order, frozen, inject)
+        job = Job(phid, xx, seg, order, frozen, inject, delj=delj)
         co.submit(job)
         return phid.cpu().numpy() if host else phid
     if not (phid.is_cuda and phid.dtype == t.float64):
@@ -334,19 +346,27 @@ def run_single(phi, xx, seg, order, frozen, inject, ctx=None, mode="auto", waves
     xxd = ctx.grid(np.asarray(xx, dtype=np.float64))
     segd = ctx.to_dev(seg.view(np.uint8))
     nsys = _lines(D, n)
+    if delj:
+        # the delta trick runs on the plain Thomas path of the workgroup kernel
+        if ctx.max_waves(D, n, 0, 0, 0, 1) < 1:
+            raise NotImplementedError("the delta trick needs the full gamma column in LDS, "
+                                      "which n=%d at D=%d exceeds" % (n, D))
+        mode = "wg"
     if mode == "auto":
         wmax = max(ctx.max_waves(D, n, 0), ctx.max_waves(D, n, 8))
         mode = "wg" if (D == 1 or nsys <= 64 * max(wmax, 1)) and wmax >= 1 else "graph"
     stream = ctx.stream()
     if mode == "wg":
         # one instance: the full gamma column (less VALU) when it fits
-        ck = 0 if ctx.max_waves(D, n, 0) >= min(8, math.ceil(nsys / 64)) else 8
-        w = waves or max(1, min(ctx.max_waves(D, n, ck), math.ceil(nsys / 64)))
+        dj = 1 if delj else 0
+        ck = 0 if (dj or ctx.max_waves(D, n, 0) >= min(8, math.ceil(nsys / 64))) else 8
+        w = waves or max(1, min(ctx.max_waves(D, n, ck, 0, 0, dj), math.ceil(nsys / 64)))
         rec = _inst_record(phid.data_ptr(), xxd.data_ptr(), segd.data_ptr(), len(seg), D, n,
                            order, frozen, inject, ld)
         tab = ctx.to_dev(rec.view(np.uint8))
         counter = t.zeros(1, dtype=t.int32, device=ctx.device)
         cfg = DhLaunch(w, 1, 0, ck, 0)   # one instance: full gamma column when it fits
+        cfg.delj = dj
         check(ctx.lib.dh_integrate_nd(tab.data_ptr(), 1, D, order, n, counter.data_ptr(),
                                       cfg, stream), "dh_integrate_nd")
         keep = (tab, counter, segd)
@@ -374,10 +394,11 @@ def run_single(phi, xx, seg, order, frozen, inject, ctx=None, mode="auto", waves
 
 class Job:
     """One integration of a batch: device phi (updated in place) plus schedule."""
-    __slots__ = ("phi", "xx", "seg", "order", "frozen", "inject", "tag", "done")
+    __slots__ = ("phi", "xx", "seg", "order", "frozen", "inject", "tag", "done", "delj")
 
-    def __init__(self, phi, xx, seg, order, frozen, inject, tag=None):
+    def __init__(self, phi, xx, seg, order, frozen, inject, tag=None, delj=False):
         self.done = False
+        self.delj = bool(delj)
         self.phi = phi
         self.xx = np.asarray(xx, dtype=np.float64)
         self.seg = seg
@@ -427,9 +448,13 @@ class PackedBatch:
         self.counter = t.zeros(1, dtype=t.int32, device=ctx.device)
         self.nmax = max(j.n for j in jobs)
         self.workspace = {}
+        self.delj = bool(jobs[0].delj)
+        if any(j.delj != self.delj for j in jobs):
+            raise ValueError("a batch runs with or without the delta trick, not both")
         # several instances may share a workgroup when they share n and grid
         self.packable = self.D >= 2 and len({int(r) for r in tab["xx"]}) == 1 and \
-            len({int(r) for r in tab["ld"]}) == 1 and all(j.n == self.nmax for j in jobs)
+            len({int(r) for r in tab["ld"]}) == 1 and all(j.n == self.nmax for j in jobs) and \
+            not self.delj
 
     def restart(self):
         """Reset every instance's progress to the start of its schedule
@@ -448,12 +473,13 @@ class PackedBatch:
             thr = 64 * w
             g = groups or max(1, math.ceil(len(self.jobs) / thr))
             nl = g * thr
-            need = 6 * self.nmax * nl
+            need = (8 if self.delj else 6) * self.nmax * nl
             scr = self.workspace.get("lanes")
             if scr is None or scr.numel() < need:
                 scr = torch().empty(need, dtype=torch().float64, device=ctx.device)
                 self.workspace["lanes"] = scr
             cfg = DhLaunch(w, g, max_steps, 0, 0)
+            cfg.delj = 1 if self.delj else 0
             check(ctx.lib.dh_integrate_1d(self.tab.data_ptr(), len(self.jobs), self.order,
                                           self.nmax, self.counter.data_ptr(), scr.data_ptr(),
                                           scr.numel(), cfg, stream), "dh_integrate_1d")
@@ -461,7 +487,7 @@ class PackedBatch:
             return ctx.nd_launch(self.tab.data_ptr(), len(self.jobs), self.D, self.order,
                                  self.nmax, self.counter.data_ptr(), stream, self.workspace,
                                  waves, groups, max_steps, chunk, recompute, gamma_global,
-                                 slots, self.packable)
+                                 slots, self.packable, self.delj)
 
     def progress(self):
         raw = self.tab.cpu().numpy().view(INST_DTYPE)
@@ -479,7 +505,7 @@ def run_batch(jobs, ctx=None, waves=None, groups=0, sort=True, chunk=None, recom
     ctx = ctx or context()
     groups_by = {}
     for j in jobs:
-        groups_by.setdefault((j.D, j.order), []).append(j)
+        groups_by.setdefault((j.D, j.order, j.delj), []).append(j)
     parts = []
     for key in sorted(groups_by):
         js = groups_by[key]

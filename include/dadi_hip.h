@@ -109,6 +109,12 @@ typedef struct dh_launch {
                               hold groups * waves * 64 * nmax doubles (dh_nd_groups) and
                               may not be shared by launches that overlap */
     long long gamma_doubles;
+    int delj;              /* 1: Chang and Cooper's delta per interval (dadi's
+                              use_delj_trick) instead of 1/2; needs chunk 0 (no
+                              recompute, no workspace) and is not available to
+                              dh_integrate_packed.  Device exp: results agree with
+                              the reference to rounding, not bit for bit. */
+    int reserved;
 } dh_launch;
 
 /* Library / device set-up; must be called once per process before use.
@@ -122,7 +128,8 @@ size_t dh_sizeof_segment(void);
 size_t dh_sizeof_instance(void);
 /* LDS footprint per workgroup of the nD kernel for (D, n, waves, chunk,
  * recompute, gamma in a global workspace or not), bytes. */
-size_t dh_lds_bytes(int D, int n, int waves, int chunk, int recompute, int gamma_global);
+size_t dh_lds_bytes(int D, int n, int waves, int chunk, int recompute, int gamma_global,
+                    int delj);
 /* Number of persistent workgroups dh_integrate_nd launches for this launch
  * shape (`groups` if > 0, else as many as fit), capped at ninst; sizes the
  * gamma workspace.  Negative DH_E* code on error. */
@@ -161,8 +168,9 @@ size_t dh_packed_lds_bytes(int D, int n, int waves, int slots);
 /*
  * Runs ninst 1-population instances, one lane per instance, lanes pulling
  * instances from a device-side queue independently (lane-level continuous
- * batching).  scratch_dev must hold 6 * nmax * lanes doubles where
- * lanes = 64 * waves * groups; nmax is the largest n in the table.
+ * batching).  scratch_dev must hold 6 * nmax * lanes doubles (8 with
+ * cfg->delj) where lanes = 64 * waves * groups; nmax is the largest n in the
+ * table.
  * Replaces dadi/Integration.py:986-1035 / 219-258 with
  * dadi/tridiag.c:36-40 and dadi/integration1D.c:7-66.
  */

@@ -276,9 +276,9 @@ def sweep(order, phi, axis, xx, nu, mig, gamma, h, beta, dt):
 _c_int, _c_d = ctypes.c_int, ctypes.c_double
 
 
-def ref_sweep_tv(phi, axis, xx, nu, mig, gamma, h, beta, dt):
+def ref_sweep_tv(phi, axis, xx, nu, mig, gamma, h, beta, dt, delj=0):
     """The reference's implicit_<D>D<axis> (dadi/integration{1..5}D.c) via the
-    argument order of dadi/integration_c.pyx:68-196."""
+    argument order of dadi/integration_c.pyx:68-196; delj: use_delj_trick."""
     R = ref()
     D, n = phi.ndim, phi.shape[0]
     P = _ptr(phi)
@@ -287,20 +287,21 @@ def ref_sweep_tv(phi, axis, xx, nu, mig, gamma, h, beta, dt):
     ms = [_c_d(m) for m in mig]
     N = _c_int(n)
     z = _c_int(0)
+    dj = _c_int(1 if delj else 0)
     if D == 1:
-        R.implicit_1Dx(P, X, nu, gamma, h, _c_d(beta), dt, N, z)
+        R.implicit_1Dx(P, X, nu, gamma, h, _c_d(beta), dt, N, dj)
     elif D == 2:
         f = [R.implicit_2Dx, R.implicit_2Dy][axis]
-        f(P, X, X, nu, ms[0], gamma, h, dt, N, N, z, z, N)
+        f(P, X, X, nu, ms[0], gamma, h, dt, N, N, dj, z, N)
     elif D == 3:
         f = [R.implicit_3Dx, R.implicit_3Dy, R.implicit_3Dz][axis]
-        f(P, X, X, X, nu, ms[0], ms[1], gamma, h, dt, N, N, N, z, z, N)
+        f(P, X, X, X, nu, ms[0], ms[1], gamma, h, dt, N, N, N, dj, z, N)
     elif D == 4:
         f = [R.implicit_4Dx, R.implicit_4Dy, R.implicit_4Dz, R.implicit_4Da][axis]
-        f(P, X, X, X, X, nu, ms[0], ms[1], ms[2], gamma, h, dt, N, N, N, N, z)
+        f(P, X, X, X, X, nu, ms[0], ms[1], ms[2], gamma, h, dt, N, N, N, N, dj)
     elif D == 5:
         f = [R.implicit_5Dx, R.implicit_5Dy, R.implicit_5Dz, R.implicit_5Da, R.implicit_5Db][axis]
-        f(P, X, X, X, X, X, nu, ms[0], ms[1], ms[2], ms[3], gamma, h, dt, N, N, N, N, N, z)
+        f(P, X, X, X, X, X, nu, ms[0], ms[1], ms[2], ms[3], gamma, h, dt, N, N, N, N, N, dj)
     else:
         raise ValueError(D)
 
@@ -320,9 +321,22 @@ def _Mfunc(x, others, ms, gamma, h):
     return acc + sel
 
 
-def precalc_arrays(D, xx, axis, nu, mig, gamma, h, beta=1):
+def compute_delj(dx, MInt, VInt):
+    """Chang and Cooper's delta of the numpy paths, dadi/Integration.py:960-984
+    (dx and VInt already broadcast against MInt): NaN and inf become 1/2."""
+    wj = 2 * MInt * dx
+    with np.errstate(all='ignore'):
+        epsj = np.exp(wj / VInt)
+        delj = (-epsj * wj + epsj * VInt - VInt) / (wj - epsj * wj)
+    delj = np.where(np.isnan(delj), 0.5, delj)
+    delj = np.where(np.isinf(delj), 0.5, delj)
+    return delj
+
+
+def precalc_arrays(D, xx, axis, nu, mig, gamma, h, beta=1, delj=False):
     """numpy a, b (without 1/dt) and c arrays of the constant-parameter paths,
-    dadi/Integration.py:997-1015 (1D), 1056-1095 (2D), 1133-1221 (3D)."""
+    dadi/Integration.py:997-1015 (1D), 1056-1095 (2D), 1133-1221 (3D); delj:
+    use_delj_trick."""
     n = len(xx)
     shp = [1] * D
 
@@ -341,7 +355,11 @@ def precalc_arrays(D, xx, axis, nu, mig, gamma, h, beta=1):
     dfact[1:-1] = 2 / (dx[:-1] + dx[1:])
     dfact[0] = 2 / dx[0]
     dfact[-1] = 2 / dx[-1]
-    delj = 0.5
+    if delj:
+        VInt = _Vfunc((xx[:-1] + xx[1:]) / 2, nu, beta)
+        delj = compute_delj(along(dx, axis), MInt, along(VInt, axis))
+    else:
+        delj = 0.5
     shape = (n,) * D
     a, b, c = [np.zeros(shape) for _ in range(3)]
     sl = lambda s: tuple(s if k == axis else slice(None) for k in range(D))
@@ -404,7 +422,7 @@ def _Mfunc1D_X(x, gamma, h, beta):
 
 
 def one_pop_X(phi, xx, T, nu=1, gamma=0, h=0.5, beta=1, alpha=1, theta0=1.0, initial_t=0,
-              dt=None):
+              dt=None, delj=False):
     """Constant-parameter X-chromosome integration, dadi/Integration.py:1256-1352
     (numpy coefficient arrays as the reference forms them, the C Thomas
     solver).  dt: the time step (default: the reference's _compute_dt)."""
@@ -420,7 +438,10 @@ def one_pop_X(phi, xx, T, nu=1, gamma=0, h=0.5, beta=1, alpha=1, theta0=1.0, ini
     dfactor[1:-1] = 2 / (dx[:-1] + dx[1:])
     dfactor[0] = 2 / dx[0]
     dfactor[-1] = 2 / dx[-1]
-    delj = 0.5
+    if delj:
+        delj = compute_delj(dx, MInt, _Vfunc_X((xx[:-1] + xx[1:]) / 2, nu, beta))
+    else:
+        delj = 0.5
     a = np.zeros(phi.shape)
     a[1:] += dfactor[1:] * (-MInt * delj - V[:-1] / (2 * dx))
     c = np.zeros(phi.shape)
@@ -446,12 +467,15 @@ def one_pop_X(phi, xx, T, nu=1, gamma=0, h=0.5, beta=1, alpha=1, theta0=1.0, ini
 
 
 def integrate_const(phi, xx, T, nus, migs, gammas, hs, theta0=1.0, initial_t=0.0,
-                    frozen=None, nomut=None, beta=1, backend="oracle"):
+                    frozen=None, nomut=None, beta=1, backend="oracle", delj=False):
     """Constant-parameter drivers, dadi/Integration.py:986-1242.
 
     nus/gammas/hs: per population; migs[k]: rates into population k from the
-    others in increasing order.  D = phi.ndim in {1, 2, 3}.
+    others in increasing order.  D = phi.ndim in {1, 2, 3}.  delj
+    (use_delj_trick) needs backend "ref" (numpy arrays + reference C).
     """
+    if delj and backend != "ref":
+        raise ValueError("the delta trick is checked through the reference backend")
     phi = np.array(phi, dtype=np.float64, copy=True, order="C")
     D = phi.ndim
     frozen = [False] * D if frozen is None else list(frozen)
@@ -466,7 +490,7 @@ def integrate_const(phi, xx, T, nus, migs, gammas, hs, theta0=1.0, initial_t=0.0
     order = ORC_PRE1 if D == 1 else ORC_PRE
     arrays = None
     if backend == "ref":
-        arrays = [precalc_arrays(D, xx, k, nus[k], migs[k], gammas[k], hs[k], beta)
+        arrays = [precalc_arrays(D, xx, k, nus[k], migs[k], gammas[k], hs[k], beta, delj)
                   for k in range(D)]
     dt = min(compute_dt(nus[k], migs[k] if D > 1 else [0], gammas[k], hs[k]) for k in range(D))
     on = [not frozen[k] and not nomut[k] for k in range(D)]
@@ -490,10 +514,13 @@ def _f(v):
 
 
 def integrate_tv(phi, xx, T, nus, migs, gammas, hs, theta0=1.0, initial_t=0.0,
-                 frozen=None, nomut=None, beta=1, backend="oracle"):
+                 frozen=None, nomut=None, beta=1, backend="oracle", delj=False):
     """Time-varying drivers, dadi/Integration.py:219-258, 329-387, 471-554,
     611-727, 786-937.  Parameters may be floats or 1-argument callables of t.
-    The first population's beta is used in 1D only."""
+    The first population's beta is used in 1D only.  delj (use_delj_trick)
+    needs backend "ref"."""
+    if delj and backend != "ref":
+        raise ValueError("the delta trick is checked through the reference backend")
     phi = np.array(phi, dtype=np.float64, copy=True, order="C")
     D = phi.ndim
     frozen = [False] * D if frozen is None else list(frozen)
@@ -532,7 +559,7 @@ def integrate_tv(phi, xx, T, nus, migs, gammas, hs, theta0=1.0, initial_t=0.0,
             if frozen[k]:
                 continue
             if backend == "ref":
-                ref_sweep_tv(phi, k, xx, nu[k], mm[k], ga[k], hh[k], bt, this_dt)
+                ref_sweep_tv(phi, k, xx, nu[k], mm[k], ga[k], hh[k], bt, this_dt, delj)
             else:
                 sweep(ORC_TV, phi, k, xx, nu[k], mm[k], ga[k], hh[k], bt, this_dt)
         t = next_t

@@ -39,18 +39,20 @@ def test_struct_layout_and_errors_without_gpu():
     L = _lib.load()
     assert L.dh_sizeof_segment() == _lib.SEG_DTYPE.itemsize == 336
     assert L.dh_sizeof_instance() == _lib.INST_DTYPE.itemsize
-    assert L.dh_abi_version() == _lib.ABI_VERSION == 4
+    assert L.dh_abi_version() == _lib.ABI_VERSION == 5
     # calls before dh_init fail cleanly with a message (no GPU touched)
     cfg = _lib.DhLaunch(1, 1, 0, 0, 0)
     rc = L.dh_integrate_nd(None, 1, 2, 0, 10, None, cfg, None)
     assert rc == -1
     assert "dh_init" in _lib.last_error()
-    assert L.dh_lds_bytes(2, 100, 1, 0, 0, 0) == 8 * 100 * (4 + 6 + 64)
-    assert L.dh_lds_bytes(2, 100, 2, 8, 0, 0) == 8 * (100 * 10 + 2 * 64 * (13 + 9))
-    assert L.dh_lds_bytes(2, 100, 2, 8, 1, 0) == 8 * (100 * 10 + 2 * 64 * (26 + 9))
+    assert L.dh_lds_bytes(2, 100, 1, 0, 0, 0, 0) == 8 * 100 * (4 + 6 + 64)
+    assert L.dh_lds_bytes(2, 100, 2, 8, 0, 0, 0) == 8 * (100 * 10 + 2 * 64 * (13 + 9))
+    assert L.dh_lds_bytes(2, 100, 2, 8, 1, 0, 0) == 8 * (100 * 10 + 2 * 64 * (26 + 9))
     # gamma columns in a global workspace: tables and staging buffers only
-    assert L.dh_lds_bytes(2, 100, 2, 0, 0, 1) == 8 * (100 * 10 + 2 * 64 * 9)
-    assert ctypes.sizeof(_lib.DhLaunch) == 40
+    assert L.dh_lds_bytes(2, 100, 2, 0, 0, 1, 0) == 8 * (100 * 10 + 2 * 64 * 9)
+    # the delta trick adds V at the interval midpoints per axis
+    assert L.dh_lds_bytes(2, 100, 1, 0, 0, 0, 1) == 8 * 100 * (4 + 8 + 64)
+    assert ctypes.sizeof(_lib.DhLaunch) == 48
     assert L.dh_nd_groups(2, 100, 2, 0, 0, 1, 0, 10) == -1   # before dh_init
 
 

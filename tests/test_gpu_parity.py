@@ -317,3 +317,66 @@ def test_one_pop_x_bit_exact(dadi_amd, oracle, nu, gamma, h, beta, alpha, T):
     engine.run_batch(jobs)
     for j in jobs:
         assert np.array_equal(j.phi.cpu().numpy(), want)
+
+
+def test_delj_trick_matches_reference(dadi_amd, oracle):
+    """use_delj_trick (Chang and Cooper's delta per interval) on the device
+    against the reference paths: constant 1-3D parameters through the numpy
+    coefficient arrays, time-varying and 4D through the reference C sweeps,
+    one_pop_X through its restatement.  The device's exp differs from
+    glibc's and numpy's in the last bit now and then, so the bar is 1e-10
+    relative to the largest entry; batched and single results are identical."""
+    if not oracle.ref_available():
+        pytest.skip("oracle/_ref not built")
+    from dadi_amd import engine
+    I = dadi_amd.Integration
+    tol = lambda want: dict(rtol=1e-10, atol=1e-10 * np.abs(want).max())
+    I.use_delj_trick = True
+    try:
+        xx = oracle.default_grid(36)
+        p1 = oracle.phi_1D(xx)
+        want = oracle.integrate_const(p1, xx, 0.1, [0.4], [[]], [-6.0], [0.2], beta=1.7,
+                                      backend="ref", delj=True)
+        got = I.one_pop(p1, xx, 0.1, nu=0.4, gamma=-6.0, h=0.2, beta=1.7)
+        assert np.allclose(got, want, **tol(want))
+        p2 = oracle.phi_1D_to_2D(xx, p1)
+        kw = (p2, xx, 0.05, [0.3, 2.0], [[4.0], [1.5]], [-8.0, 5.0], [0.3, 0.7])
+        want2 = oracle.integrate_const(*kw, backend="ref", delj=True)
+        got2 = I.two_pops(p2, xx, 0.05, 0.3, 2.0, m12=4.0, m21=1.5, gamma1=-8.0, gamma2=5.0,
+                          h1=0.3, h2=0.7)
+        assert np.allclose(got2, want2, **tol(want2))
+        nu1 = lambda t: 0.3 + 2 * t
+        want3 = oracle.integrate_tv(p2, xx, 0.05, [nu1, 2.0], [[4.0], [1.5]], [-8.0, 5.0],
+                                    [0.3, 0.7], backend="ref", delj=True)
+        got3 = I.two_pops(p2, xx, 0.05, nu1, 2.0, m12=4.0, m21=1.5, gamma1=-8.0, gamma2=5.0,
+                          h1=0.3, h2=0.7)
+        assert np.allclose(got3, want3, **tol(want3))
+        x3 = oracle.default_grid(14)
+        p3 = oracle.phi_2D_to_3D_split_2(x3, oracle.phi_1D_to_2D(x3, oracle.phi_1D(x3)))
+        want4 = oracle.integrate_const(p3, x3, 0.02, [1.0, 0.5, 2.0], [[1, 2], [3, 0.5], [1, 1]],
+                                       [2.0, -3.0, 0.0], [0.5, 0.5, 0.5], backend="ref", delj=True)
+        got4 = I.three_pops(p3, x3, 0.02, 1.0, 0.5, 2.0, m12=1, m13=2, m21=3, m23=0.5, m31=1,
+                            m32=1, gamma1=2.0, gamma2=-3.0)
+        assert np.allclose(got4, want4, **tol(want4))
+        x4 = oracle.default_grid(10)
+        p4 = oracle.phi_3D_to_4D(oracle.phi_2D_to_3D_split_2(
+            x4, oracle.phi_1D_to_2D(x4, oracle.phi_1D(x4))), 0, 0, x4, x4, x4, x4)
+        want5 = oracle.integrate_tv(p4, x4, 0.01, [1.0, 0.7, 1.5, 2.0], [[1, 0, 2]] * 4,
+                                    [1.0, -2.0, 0.0, 3.0], [0.5] * 4, backend="ref", delj=True)
+        got5 = p4.copy()
+        I.four_pops(got5, x4, 0.01, 1.0, 0.7, 1.5, 2.0, m12=1, m13=0, m14=2, m21=1, m23=0,
+                    m24=2, m31=1, m32=0, m34=2, m41=1, m42=0, m43=2, gamma1=1.0, gamma2=-2.0,
+                    gamma4=3.0)
+        assert np.allclose(got5, want5, **tol(want5))
+        wantx = oracle.one_pop_X(p1, xx, 0.1, 0.6, -5.0, 0.3, 2.0, 1.5, delj=True)
+        gotx = I.one_pop_X(p1, xx, 0.1, 0.6, -5.0, 0.3, 2.0, 1.5)
+        assert np.allclose(gotx, wantx, **tol(wantx))
+        # batched (lane kernel for 1D, workgroup kernel for 2D) equals single
+        res = engine.map_batched(
+            lambda k: (I.one_pop(p1, xx, 0.1, nu=0.4, gamma=-6.0, h=0.2, beta=1.7) if k == 0
+                       else I.two_pops(p2, xx, 0.05, 0.3, 2.0, m12=4.0, m21=1.5, gamma1=-8.0,
+                                       gamma2=5.0, h1=0.3, h2=0.7)), [(0,), (1,), (0,), (1,)])
+        assert np.array_equal(res[0], got) and np.array_equal(res[2], got)
+        assert np.array_equal(res[1], got2) and np.array_equal(res[3], got2)
+    finally:
+        I.use_delj_trick = False

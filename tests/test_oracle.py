@@ -232,3 +232,21 @@ def test_x_chromosome_injection_factor(oracle):
         auto = oracle.inject_value(1, 1e-3, xx, 1.0)
         assert np.isclose(S.inject_value_X(1e-3, xx, 1.0, beta, alpha), auto * want,
                           rtol=1e-14)
+
+
+def test_delj_numpy_path_matches_reference_c(oracle):
+    """With use_delj_trick the oracle's numpy coefficient arrays
+    (dadi/Integration.py:960-984, 997-1095) and the reference C sweeps
+    (dadi/integration_shared.c:52-70) agree to rounding (they differ only in
+    exp and in how b and r are associated); and the trick matters: results
+    move away from delta = 1/2 where M dx / V is large."""
+    if not oracle.ref_available():
+        pytest.skip("oracle/_ref not built")
+    xx = oracle.default_grid(40)
+    phi0 = oracle.phi_1D_to_2D(xx, oracle.phi_1D(xx))
+    args = (phi0, xx, 0.05, [0.3, 2.0], [[4.0], [1.5]], [-8.0, 5.0], [0.3, 0.7])
+    pre = oracle.integrate_const(*args, backend="ref", delj=True)
+    tv = oracle.integrate_tv(*args, backend="ref", delj=True)
+    plain = oracle.integrate_const(*args, backend="ref")
+    assert np.allclose(pre, tv, rtol=1e-9, atol=1e-12 * np.abs(pre).max())
+    assert not np.allclose(pre, plain, rtol=1e-6)
