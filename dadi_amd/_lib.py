@@ -86,6 +86,9 @@ _SIGS = {
     "dh_advance_steps": ([_vp, ctypes.c_longlong, _vp], ctypes.c_int),
     "dh_integrate_graph": ([ctypes.POINTER(DhInstance), _vp, _vp, ctypes.c_longlong,
                             ctypes.c_int, ctypes.c_int, _vp], ctypes.c_int),
+    "dh_integrate_graph_range": ([ctypes.POINTER(DhInstance), _vp, _vp, ctypes.c_longlong,
+                                  ctypes.c_longlong, ctypes.c_longlong, ctypes.c_int,
+                                  ctypes.c_int, _vp], ctypes.c_int),
     "dh_contract_axis": ([_vp, ctypes.c_longlong, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp],
                          ctypes.c_int),
     "dh_sample_axis": ([_vp, ctypes.c_longlong, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp,

@@ -2007,10 +2007,22 @@ int dh_integrate_graph(const dh_instance *inst_host, long long *step_base_dev,
                        const int *seg_of_step_dev, long long total_steps, int nsteps_per_graph,
                        int waves, void *stream)
 {
+    return dh_integrate_graph_range(inst_host, step_base_dev, seg_of_step_dev, total_steps, 0,
+                                    total_steps, nsteps_per_graph, waves, stream);
+}
+
+int dh_integrate_graph_range(const dh_instance *inst_host, long long *step_base_dev,
+                             const int *seg_of_step_dev, long long total_steps,
+                             long long first_step, long long nsteps, int nsteps_per_graph,
+                             int waves, void *stream)
+{
     if (int e = check_init()) return e;
     if (!inst_host || !step_base_dev || !seg_of_step_dev) return fail(DH_EINVAL, "null pointer");
     if (nsteps_per_graph < 1) return fail(DH_EINVAL, "nsteps_per_graph must be >= 1");
-    if (total_steps <= 0) return DH_OK;
+    if (first_step < 0 || nsteps < 0) return fail(DH_EINVAL, "negative step range");
+    if (nsteps % nsteps_per_graph != 0 && first_step + nsteps < total_steps)
+        return fail(DH_EINVAL, "nsteps must be a multiple of nsteps_per_graph or reach the end");
+    if (total_steps <= 0 || nsteps == 0 || first_step >= total_steps) return DH_OK;
     const dh_instance I = *inst_host;
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     // key: everything baked into the captured launches
@@ -2056,13 +2068,16 @@ int dh_integrate_graph(const dh_instance *inst_host, long long *step_base_dev,
             return fail(DH_EHIP, std::string("instantiate: ") + hipGetErrorString(ei));
         std::lock_guard<std::mutex> lk(g_graph_mu);
         if (g_graphs.size() > 64) {
+            // replays of these graphs may still be queued on other streams
+            if (hipDeviceSynchronize() != hipSuccess) return fail(DH_EHIP, "device sync");
             for (auto &kv : g_graphs) (void)hipGraphExecDestroy(kv.second);
             g_graphs.clear();
         }
         g_graphs[key] = exec;
     }
-    hipLaunchKernelGGL(k_set_ll, dim3(1), dim3(1), 0, st, step_base_dev, 0LL);
-    const long long reps = (total_steps + nsteps_per_graph - 1) / nsteps_per_graph;
+    hipLaunchKernelGGL(k_set_ll, dim3(1), dim3(1), 0, st, step_base_dev, first_step);
+    const long long last = first_step + nsteps < total_steps ? first_step + nsteps : total_steps;
+    const long long reps = (last - first_step + nsteps_per_graph - 1) / nsteps_per_graph;
     for (long long r = 0; r < reps; ++r) DH_HIP(hipGraphLaunch(exec, st));
     return DH_OK;
 }

@@ -15,6 +15,16 @@ next operations (device-side splits, sampling) and either starts its next
 epoch in the following quantum or completes.  New arrivals join at the next
 quantum -- they do not wait for a batch to drain.
 
+Quanta are sized in time, not steps: each class's step count per launch
+adapts so that a launch lasts about ``quantum_ms``, so that classes of cheap
+steps (1D, small 2D) are not held back behind classes of expensive ones.
+Epochs whose sweeps have many lines (3D and 4D phi) would run their steps
+serially on one workgroup in the batched launch; they take the
+multi-workgroup path instead (every sweep of a step spread over the whole
+GPU, steps replayed from a captured hipGraph), advanced one quantum at a time
+on their own streams, so their latency is that of the per-step sweeps rather
+than of one compute unit.
+
 Every request's arithmetic is independent of what it shares a launch with,
 so a result is bit-identical alone and in any mix.
 """
@@ -137,6 +147,7 @@ class Request:
     t_admit: float = None
     t_done: float = None
     footprint: int = 0
+    wide: object = None
 
 
 class Server:
@@ -147,16 +158,47 @@ class Server:
     quantum: time steps each in-flight integration advances per launch.
     """
 
-    def __init__(self, budget_bytes=8 << 30, quantum=200, ctx=None):
+    def __init__(self, budget_bytes=8 << 30, quantum=200, ctx=None, quantum_ms=8.0,
+                 wide_lines=512, wide_streams=2, steps_per_graph=16, wide_waves=2):
         import torch
         self.torch = torch
         self.ctx = ctx or engine.context()
         self.budget = int(budget_bytes)
         self.quantum = int(quantum)
+        self.quantum_ms = float(quantum_ms)
+        self.wide_lines = int(wide_lines)
+        self.wide_streams = max(1, int(wide_streams))
+        self.spg = max(1, int(steps_per_graph))
+        self.wide_waves = int(wide_waves)
+        self.steps = {}       # class -> time steps per launch (adapted)
         self.streams = {}
         self.scratch = {}
         self.used = 0
         self.launches = 0
+        self.wide_launches = 0
+
+    def _steps_for(self, key):
+        q = self.steps.get(key)
+        if q is None:
+            q = self.quantum if key[0] != "wide" else 4 * self.spg
+        return q
+
+    def _adapt(self, key, q, ms):
+        """Steer the class's steps per launch toward quantum_ms per launch."""
+        if ms <= 0:
+            return
+        want = q * self.quantum_ms / ms
+        want = min(max(want, 0.5 * q), 2.0 * q)
+        if key[0] == "wide":
+            want = max(self.spg, int(round(want / self.spg)) * self.spg)
+        else:
+            want = max(1, min(int(round(want)), 50 * self.quantum))
+        self.steps[key] = want
+
+    def is_wide(self, D, n):
+        """Epochs of D >= 3 populations whose sweeps have at least wide_lines
+        lines take the multi-workgroup path."""
+        return D >= 3 and n ** (D - 1) >= self.wide_lines
 
     @staticmethod
     def footprint(prog):
@@ -185,12 +227,21 @@ class Server:
                 r.state += 1
                 continue
             ep = arg
+            if r.wide is not None:
+                if r.wide["cur"] >= r.wide["total"]:
+                    r.wide = None
+                    r.state += 1
+                    continue
+                return "wide"
             if r.rec is None:
                 if len(ep.seg) == 0 or ep.frozen == (1 << ep.D) - 1:
                     r.state += 1
                     continue
                 if r.phi.dim() != ep.D:
                     raise ValueError("epoch dimensionality does not match phi")
+                if self.is_wide(ep.D, r.prog.pts):
+                    r.wide = self._wide_state(r, ep)
+                    return "wide"
                 r.segd = self.ctx.to_dev(ep.seg.view(np.uint8))
                 xxd = self.ctx.grid(r.prog.xx)
                 r.rec = engine._inst_record(r.phi.data_ptr(), xxd.data_ptr(), r.segd.data_ptr(),
@@ -207,6 +258,44 @@ class Server:
         r.result = sample_device(r.phi, r.prog.ns, r.prog.xx, self.ctx).cpu().numpy()
         return "done"
 
+    def _wide_state(self, r, ep):
+        """Device state of a multi-workgroup epoch: the schedule, the step of
+        each time step's segment and the step counter of the captured graph."""
+        t = self.torch
+        ctx = self.ctx
+        segd = ctx.to_dev(ep.seg.view(np.uint8))
+        total = int(np.sum(ep.seg["nsteps"]))
+        sos = np.repeat(np.arange(len(ep.seg), dtype=np.int32), ep.seg["nsteps"])
+        sosd = ctx.to_dev(sos)
+        base = t.zeros(1, dtype=t.int64, device=ctx.device)
+        xxd = ctx.grid(r.prog.xx)
+        inst = _lib.DhInstance(r.phi.data_ptr(), xxd.data_ptr(), segd.data_ptr(), len(ep.seg), 0,
+                               0, ep.D, r.prog.pts, ep.order, ep.frozen, ep.inject, 0,
+                               engine.pitch_of(r.phi))
+        return {"inst": inst, "segd": segd, "sosd": sosd, "base": base, "total": total,
+                "cur": 0, "keep": xxd}
+
+    def _launch_wide(self, r):
+        """One quantum of one multi-workgroup epoch (steps from w['cur'])."""
+        t = self.torch
+        ctx = self.ctx
+        w = r.wide
+        key = ("wide", r.prog.pts ** w["inst"].D)
+        q = self._steps_for(key)
+        st = self._stream(("wide", r.rid % self.wide_streams))
+        st.wait_stream(t.cuda.current_stream(ctx.device))
+        ev0 = t.cuda.Event(enable_timing=True)
+        ev0.record(st)
+        _lib.check(ctx.lib.dh_integrate_graph_range(w["inst"], w["base"].data_ptr(),
+                                                    w["sosd"].data_ptr(), w["total"], w["cur"],
+                                                    q, self.spg, self.wide_waves,
+                                                    st.cuda_stream),
+                   "dh_integrate_graph_range")
+        w["pending"] = min(q, w["total"] - w["cur"])
+        self.launches += 1
+        self.wide_launches += 1
+        return st, key, q, ev0
+
     def _launch(self, group):
         """One quantum for every in-flight epoch of one (D, order) class."""
         t = self.torch
@@ -214,11 +303,14 @@ class Server:
         D, order = group[0].rec["D"][0], group[0].rec["order"][0]
         tab = np.concatenate([r.rec for r in group])
         key = (int(D), int(order))
+        q = self._steps_for(key)
         st = self._stream(key)
         # splits and uploads of this quantum were issued on the current stream
         st.wait_stream(t.cuda.current_stream(ctx.device))
+        ev0 = t.cuda.Event(enable_timing=True)
         with t.cuda.stream(st):
             tabd = t.from_numpy(tab.view(np.uint8).copy()).to(ctx.device, non_blocking=False)
+            ev0.record(st)
             counter = t.zeros(1, dtype=t.int32, device=ctx.device)
             nmax = int(tab["n"].max())
             h = st.cuda_stream
@@ -231,16 +323,16 @@ class Server:
                 if scr is None or scr.numel() < need:
                     scr = t.empty(need, dtype=t.float64, device=ctx.device)
                     ws["lanes"] = scr
-                cfg = _lib.DhLaunch(1, g, self.quantum, 0, 0)
+                cfg = _lib.DhLaunch(1, g, q, 0, 0)
                 _lib.check(ctx.lib.dh_integrate_1d(tabd.data_ptr(), len(group), int(order), nmax,
                                                    counter.data_ptr(), scr.data_ptr(), scr.numel(),
                                                    cfg, h), "dh_integrate_1d")
             else:
                 ws = self.scratch.setdefault(key, {})
                 ctx.nd_launch(tabd.data_ptr(), len(group), int(D), int(order), nmax,
-                              counter.data_ptr(), h, ws, max_steps=self.quantum)
+                              counter.data_ptr(), h, ws, max_steps=q)
         self.launches += 1
-        return st, tabd, counter
+        return st, tabd, counter, key, q, ev0
 
     def run(self, requests, realtime=True, clock=time.perf_counter):
         """Serve ``requests`` (arrival times in seconds from the start).
@@ -269,6 +361,8 @@ class Server:
                 status = "done"
             if status == "done":
                 self._retire(r, done, t0, clock)
+            elif status == "wide":
+                waiting.setdefault(("wide", r.rid), []).append(r)
             else:
                 key = (int(r.rec["D"][0]), int(r.rec["order"][0]))
                 waiting.setdefault(key, []).append(r)
@@ -296,26 +390,46 @@ class Server:
                     self._retire(r, done, t0, clock)
                     continue
                 enqueue(r)
-            for key in sorted(waiting):
+            for key in sorted(waiting, key=repr):
                 if key in inflight or not waiting[key]:
                     continue
                 grp = waiting.pop(key)
-                st, tabd, counter = self._launch(grp)
-                ev = t.cuda.Event()
+                try:
+                    if key[0] == "wide":
+                        st, ckey, q, ev0 = self._launch_wide(grp[0])
+                        tabd = None
+                    else:
+                        st, tabd, counter, ckey, q, ev0 = self._launch(grp)
+                except Exception as e:  # the class's requests fail, the server goes on
+                    for r in grp:
+                        r.error = repr(e)
+                        self._retire(r, done, t0, clock)
+                    continue
+                ev = t.cuda.Event(enable_timing=True)
                 ev.record(st)
-                inflight[key] = (grp, st, tabd, counter, ev)
+                inflight[key] = (grp, tabd, ev, ev0, ckey, q)
             progressed = False
             for key in list(inflight):
-                grp, st, tabd, counter, ev = inflight[key]
+                grp, tabd, ev, ev0, ckey, q = inflight[key]
                 if not ev.query():
                     continue
                 progressed = True
                 del inflight[key]
+                ms = ev0.elapsed_time(ev)
+                if tabd is None:
+                    w = grp[0].wide
+                    if w["pending"] == q:
+                        self._adapt(ckey, q, ms)
+                    w["cur"] += w["pending"]
+                    enqueue(grp[0])
+                    continue
                 back = tabd.cpu().numpy().view(INST_DTYPE)
                 for r, rec in zip(grp, back):
                     r.rec["cur_seg"] = rec["cur_seg"]
                     r.rec["cur_step"] = rec["cur_step"]
                     r.rec["status"] = rec["status"]
+                if np.any(back["status"] != 1):   # someone ran the whole quantum
+                    self._adapt(ckey, q, ms)
                 for r in grp:
                     enqueue(r)
             waiting = {k: v for k, v in waiting.items() if v}
@@ -334,6 +448,7 @@ class Server:
             "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3) if len(lat) else None,
                            "p99": float(np.percentile(lat, 99) * 1e3) if len(lat) else None},
             "launches": self.launches,
+            "wide_launches": self.wide_launches,
         }
 
     def _retire(self, r, done, t0, clock):
@@ -341,6 +456,7 @@ class Server:
         r.phi = None
         r.rec = None
         r.segd = None
+        r.wide = None
         self.used -= r.footprint
         done.append(r)
 

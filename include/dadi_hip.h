@@ -200,6 +200,18 @@ int dh_advance_steps(long long *step_base_dev, long long by, void *stream);
 int dh_integrate_graph(const dh_instance *inst_host, long long *step_base_dev,
                        const int *seg_of_step_dev, long long total_steps,
                        int nsteps_per_graph, int waves, void *stream);
+/* As dh_integrate_graph, for steps first_step .. first_step+nsteps-1 only
+ * (capped at total_steps): a long integration advanced in quanta, each call
+ * resuming where the previous one stopped (the serving loop's multi-workgroup
+ * path for large phi).  nsteps must be a multiple of nsteps_per_graph unless
+ * the range reaches total_steps.  Same graph as dh_integrate_graph, same
+ * results: splitting an integration into ranges does not change any bit.
+ * Replaces the same per-step loop as dh_integrate_graph
+ * (dadi/Integration.py:682-726, 886-936). */
+int dh_integrate_graph_range(const dh_instance *inst_host, long long *step_base_dev,
+                             const int *seg_of_step_dev, long long total_steps,
+                             long long first_step, long long nsteps, int nsteps_per_graph,
+                             int waves, void *stream);
 
 /*
  * Sampling phi -> expected spectrum (dadi/Spectrum_mod.py:1307-1348,

@@ -98,3 +98,36 @@ def test_poisoned_request_fails_alone(dadi_amd):
     assert not np.all(np.isfinite(bad.result))
     for r, s in zip(mix[:4], solo):
         assert np.array_equal(r.result, s)
+
+
+@pytest.mark.gpu
+def test_wide_path_equals_batched_path(dadi_amd):
+    """3D and 4D epochs served on the multi-workgroup path (every sweep spread
+    over the GPU, steps replayed from a captured graph in quanta of a few
+    steps) give the bits of the one-workgroup batched path."""
+    from dadi_amd import serve
+    mix = _small_mix(21, 120)
+    progs = ([r.prog for r in mix if r.prog.kind == "three_pop_chain"][:4]
+             + [r.prog for r in mix if r.prog.kind == "four_pop_chain"][:2])
+    assert len(progs) == 6
+    wide = [serve.Request(i, p, 0.0) for i, p in enumerate(progs)]
+    srv = serve.Server(wide_lines=1, steps_per_graph=4, quantum_ms=0.5)
+    st = srv.run(wide, realtime=False)
+    assert st["failed"] == 0 and st["wide_launches"] > len(progs)
+    narrow = [serve.Request(i, p, 0.0) for i, p in enumerate(progs)]
+    st2 = serve.Server(wide_lines=1 << 60, quantum=5).run(narrow, realtime=False)
+    assert st2["failed"] == 0 and st2["wide_launches"] == 0
+    for a, b in zip(wide, narrow):
+        assert np.array_equal(a.result, b.result), a.prog.kind
+
+
+@pytest.mark.gpu
+def test_graph_range_rejects_ragged_quanta(dadi_amd):
+    from dadi_amd import _lib, engine
+    import torch
+    ctx = engine.context()
+    inst = _lib.DhInstance(0, 0, 0, 1, 0, 0, 3, 10, 0, 0, 0, 0, 0)
+    base = torch.zeros(1, dtype=torch.int64, device=ctx.device)
+    rc = ctx.lib.dh_integrate_graph_range(inst, base.data_ptr(), base.data_ptr(), 100, 0, 10,
+                                          4, 2, ctx.stream())
+    assert rc == -1   # DH_EINVAL
