@@ -1585,6 +1585,331 @@ __global__ void __launch_bounds__(256) k_integrate_1d(dh_instance *__restrict__ 
 }
 
 // ---------------------------------------------------------------------------
+// Kernel 2b: 1D, one wave per integration, everything in LDS.
+//
+// Within a segment the tridiagonal matrix of every step is the same (same
+// dt, same parameters), so the Thomas factorisation -- gamma_j and bet_j of
+// dadi/tridiag.c:22-29, which do not depend on phi -- is formed once per
+// segment with exactly the per-step operations and kept in LDS.  A step
+// then only runs the right-hand-side recurrence
+//   u_j = (r_j - a_j u_{j-1}) / bet_j,   u_j -= gamma_{j+1} u_{j+1},
+// which gives the bits of the full per-step Thomas solve.  The divisions by
+// bet_j use its precomputed reciprocal (rcp_form_exact below).  The
+// recurrences are serial, so one lane runs them while the wave's other lanes
+// form the coefficients and the right-hand sides; the latency of a step is a
+// few dependent operations per row instead of two full divisions and a
+// global-memory round trip.
+// ---------------------------------------------------------------------------
+
+constexpr int W1_ARRAYS = 7;   // P, R, A, B (bet), Y (1/bet), G (gamma), T
+constexpr int W1_PF = 8;       // rows loaded ahead in the serial recurrences
+constexpr int W1_PAD = 2 * W1_PF;   // doubles of padding before and after each array
+
+// num / bet from y = RN(1/bet): q0 = num*y is within an ulp of the quotient,
+// the residual num - bet*q0 is exact with an FMA, and q0 + residual*y rounds
+// to the correctly rounded quotient (P. Markstein, IBM J. Res. Dev. 34(1),
+// 1990) -- the bits of an IEEE division -- as long as nothing leaves the
+// normal range.  With 2^-16 <= |bet| < 2^57 (checked once per segment) that
+// holds for num = +0 and for normal num of magnitude in [2^-960, 2^1000];
+// this tells whether num is such a value.
+__device__ __forceinline__ bool rcp_form_exact(double num)
+{
+    const double an = __builtin_fabs(num);
+    return (an >= 0x1p-960 && an <= 0x1p1000) || __double_as_longlong(num) == 0;
+}
+
+// Forward recurrence of one step on one lane from row j0 on
+// (dadi/tridiag.c:24-29 with bet and gamma known): u_j = (r_j - a_j u_{j-1})
+// / bet_j into P, u_{j0-1} taken from P.  MK: through the known reciprocals
+// without range checks (the caller checks afterwards and redoes rows from
+// the first one outside the exact range with MK = false).  Rows go in blocks
+// of W1_PF, the next block's operands loading while the current one is
+// solved; the arrays carry W1_PAD doubles of padding on both sides, so the
+// look-ahead reads past the end without bounds arithmetic.
+template <bool MK>
+__device__ __forceinline__ void w1_forward(double *__restrict__ P, const double *__restrict__ R,
+                                           const double *__restrict__ A,
+                                           const double *__restrict__ B,
+                                           const double *__restrict__ Y, int n, int j0)
+{
+    auto quot = [](double num, double bet, double y) -> double {
+        if constexpr (MK) {
+            const double q0 = num * y;
+            const double e = __builtin_fma(-bet, q0, num);
+            return __builtin_fma(e, y, q0);
+        } else {
+            return num / bet;
+        }
+    };
+    double u = j0 == 0 ? quot(R[0], B[0], Y[0]) : quot(R[j0] - A[j0] * P[j0 - 1], B[j0], Y[j0]);
+    P[j0] = u;
+    const double *r = R + j0 + 1, *a = A + j0 + 1, *b = B + j0 + 1, *y = Y + j0 + 1;
+    double *p = P + j0 + 1;
+    double nr[W1_PF], na_[W1_PF], nb[W1_PF], ny[W1_PF];
+#pragma unroll
+    for (int q = 0; q < W1_PF; ++q) {
+        nr[q] = r[q];
+        na_[q] = a[q];
+        nb[q] = b[q];
+        ny[q] = y[q];
+    }
+    const int rows = n - 1 - j0, full = rows / W1_PF;
+    double cr[W1_PF], ca[W1_PF], cb[W1_PF], cy[W1_PF];
+    for (int k = 0; k < full; ++k) {
+#pragma unroll
+        for (int q = 0; q < W1_PF; ++q) {
+            cr[q] = nr[q];
+            ca[q] = na_[q];
+            cb[q] = nb[q];
+            cy[q] = ny[q];
+        }
+#pragma unroll
+        for (int q = 0; q < W1_PF; ++q) {
+            nr[q] = r[W1_PF + q];
+            na_[q] = a[W1_PF + q];
+            nb[q] = b[W1_PF + q];
+            ny[q] = y[W1_PF + q];
+        }
+#pragma unroll
+        for (int q = 0; q < W1_PF; ++q) {
+            u = quot(cr[q] - ca[q] * u, cb[q], cy[q]);
+            p[q] = u;
+        }
+        r += W1_PF;
+        a += W1_PF;
+        b += W1_PF;
+        y += W1_PF;
+        p += W1_PF;
+    }
+    const int rem = rows - full * W1_PF;
+#pragma unroll
+    for (int q = 0; q < W1_PF; ++q) {
+        if (q < rem) {
+            u = quot(nr[q] - na_[q] * u, nb[q], ny[q]);
+            p[q] = u;
+        }
+    }
+}
+
+// Back substitution of one step on one lane (dadi/tridiag.c:31-33), blocks
+// of W1_PF rows going down, padded arrays as in w1_forward.
+__device__ __forceinline__ void w1_backward(double *__restrict__ P,
+                                            const double *__restrict__ G, int n)
+{
+    double u = P[n - 1];
+    double *p = P + n - 2;
+    const double *g = G + n - 1;
+    double np_[W1_PF], ng[W1_PF], cp[W1_PF], cg[W1_PF];
+#pragma unroll
+    for (int q = 0; q < W1_PF; ++q) {
+        np_[q] = p[-q];
+        ng[q] = g[-q];
+    }
+    const int rows = n - 1, full = rows / W1_PF;
+    for (int k = 0; k < full; ++k) {
+#pragma unroll
+        for (int q = 0; q < W1_PF; ++q) {
+            cp[q] = np_[q];
+            cg[q] = ng[q];
+        }
+#pragma unroll
+        for (int q = 0; q < W1_PF; ++q) {
+            np_[q] = p[-W1_PF - q];
+            ng[q] = g[-W1_PF - q];
+        }
+#pragma unroll
+        for (int q = 0; q < W1_PF; ++q) {
+            u = cp[q] - cg[q] * u;
+            p[-q] = u;
+        }
+        p -= W1_PF;
+        g -= W1_PF;
+    }
+    const int rem = rows - full * W1_PF;
+#pragma unroll
+    for (int q = 0; q < W1_PF; ++q) {
+        if (q < rem) {
+            u = np_[q] - ng[q] * u;
+            p[-q] = u;
+        }
+    }
+}
+
+template <int ORDER, int DJ>
+__global__ void __launch_bounds__(64) k_integrate_1w(dh_instance *__restrict__ tab, int ninst,
+                                                     int *__restrict__ counter, int nmax,
+                                                     long long max_steps)
+{
+    extern __shared__ double lds[];
+    __shared__ int s_id, s_first;
+    const int lane = threadIdx.x;
+    const int st = nmax + 2 * W1_PAD;   // array stride, padding on both sides
+    double *P = lds + W1_PAD, *R = P + st, *A = R + st, *B = A + st, *Y = B + st;
+    double *G = Y + st, *T = G + st;
+    for (;;) {
+        if (lane == 0) s_id = atomicAdd(counter, 1);
+        __syncthreads();
+        const int id = s_id;
+        __syncthreads();
+        if (id >= ninst) break;
+        dh_instance *ip = tab + id;
+        const int n = ip->n;
+        const double *xx = gbl(ip->xx_dev);
+        double *phi = gbl(ip->phi_dev);
+        for (int j = lane; j < n; j += 64) P[j] = phi[j];
+        long long seg = ip->cur_seg, step = ip->cur_step;
+        const long long nseg = ip->nseg;
+        long long budget = max_steps > 0 ? max_steps : LLONG_MAX;
+        const bool injon = ip->inject & 1;
+        const bool frz = ip->frozen & 1;
+        const double dx0 = xx[1] - xx[0], dxl = xx[n - 1] - xx[n - 2];
+        __syncthreads();
+        while (seg < nseg && budget > 0) {
+            const dh_segment *sp = gbl(ip->seg_dev) + seg;
+            long long nst = sp->nsteps - step;
+            if (nst > budget) nst = budget;
+            if (!frz && nst > 0) {
+                const double nu = sp->nu[0], g = sp->gamma[0], h = sp->h[0];
+                const double bp = sp->beta_p, br = sp->beta_r, bq = sp->beta_q;
+                const int chrom = sp->chrom;
+                const double dt = sp->dt, inv_dt = 1. / dt, inj = sp->inject;
+                const double sel0 = d_sel(xx[0], g, h, chrom), sel1 = d_sel(xx[n - 1], g, h, chrom);
+                // both boundary terms of the 1D line (dadi/integration1D.c:50-53)
+                const bool bf = sel0 <= 0, bl = sel1 >= 0;
+                const double bcf = (0.5 / nu - sel0) * 2. / dx0;
+                const double bcl = -(-0.5 / nu - sel1) * 2. / dxl;
+                // interface terms atemp/ctemp (compute_abc_nobc,
+                // dadi/integration_shared.c:72-93): NA in Y, NC in T
+                for (int i = lane; i < n - 1; i += 64) {
+                    const double dxi = xx[i + 1] - xx[i];
+                    const double vi = d_vfunc(xx[i], nu) * bp * br / bq;
+                    const double vj = d_vfunc(xx[i + 1], nu) * bp * br / bq;
+                    const double w0 = vi / (2 * dxi), w1 = vj / (2 * dxi);
+                    const double xi = 0.5 * (xx[i + 1] + xx[i]);
+                    const double M = d_sel(xi, g, h, chrom);
+                    double dl = 0.5;
+                    if constexpr (DJ != 0)
+                        dl = d_delj<ORDER>(M, d_vfunc(xi, nu) * bp * br / bq, xx[i + 1] - xx[i]);
+                    Y[i] = M * dl + w0;
+                    T[i] = -M * (1 - dl) + w1;
+                }
+                __syncthreads();
+                // rows: a_j in A, b_j in B, c_{j-1} in G
+                for (int j = lane; j < n; j += 64) {
+                    const double dfj = j == 0       ? 2. / (xx[1] - xx[0])
+                                       : j == n - 1 ? 2. / (xx[n - 1] - xx[n - 2])
+                                                    : 2. / ((xx[j + 1] - xx[j]) + (xx[j] - xx[j - 1]));
+                    double a = 0., b, cm = 0.;
+                    if (j == 0) {
+                        const double pa = Y[0];
+                        if constexpr (ORDER == DH_ORDER_TV) {
+                            b = inv_dt + dfj * pa;
+                            if (bf) b += bcf;
+                        } else {
+                            b = dfj * pa;
+                            if (bf) b += bcf;
+                            b = b + inv_dt;
+                        }
+                    } else {
+                        const double dfm = j - 1 == 0 ? 2. / (xx[1] - xx[0])
+                                                      : 2. / ((xx[j] - xx[j - 1]) +
+                                                              (xx[j - 1] - xx[j - 2]));
+                        const double pa = Y[j - 1], pc = T[j - 1];
+                        a = -dfj * pa;
+                        cm = -dfm * pc;
+                        if (j < n - 1) {
+                            const double na = Y[j];
+                            if constexpr (ORDER == DH_ORDER_TV)
+                                b = (inv_dt + dfj * pc) + dfj * na;
+                            else
+                                b = (dfj * na + dfj * pc) + inv_dt;
+                        } else {
+                            if constexpr (ORDER == DH_ORDER_TV) {
+                                b = inv_dt + dfj * pc;
+                                if (bl) b += bcl;
+                            } else {
+                                b = dfj * pc;
+                                if (bl) b += bcl;
+                                b = b + inv_dt;
+                            }
+                        }
+                    }
+                    A[j] = a;
+                    B[j] = b;
+                    G[j] = cm;
+                }
+                __syncthreads();
+                // factorisation (dadi/tridiag.c:22-29): gamma_j, bet_j
+                if (lane == 0) {
+                    double bet = B[0];
+                    for (int j = 1; j < n; ++j) {
+                        const double gj = G[j] / bet;
+                        bet = B[j] - A[j] * gj;
+                        G[j] = gj;
+                        B[j] = bet;
+                    }
+                }
+                __syncthreads();
+                bool safe = true;
+                for (int j = lane; j < n; j += 64) {
+                    const double bj = B[j];
+                    Y[j] = 1. / bj;
+                    const double ab = __builtin_fabs(bj);
+                    safe = safe && ab >= 0x1p-16 && ab < 0x1p57;
+                }
+                safe = __all(safe);
+                __syncthreads();
+                const int inj_row = injon ? 1 : -1;
+                for (long long k = 0; k < nst; ++k) {
+                    // right-hand side (the injected mass first)
+                    for (int j = lane; j < n; j += 64) {
+                        double pj = P[j];
+                        if (j == inj_row) pj = pj + inj;
+                        R[j] = (ORDER == DH_ORDER_PRE) ? inv_dt * pj : pj / dt;
+                    }
+                    __syncthreads();
+                    if (safe) {
+                        if (lane == 0) w1_forward<true>(P, R, A, B, Y, n, 0);
+                        if (lane == 0) s_first = n;
+                        __syncthreads();
+                        // the first row whose numerator was outside the range
+                        // where the reciprocal form is exact (the numerators
+                        // are re-formed with the same operations)
+                        for (int j = lane; j < n; j += 64) {
+                            const double num = j == 0 ? R[0] : R[j] - A[j] * P[j - 1];
+                            if (!rcp_form_exact(num)) {
+                                atomicMin(&s_first, j);
+                                break;
+                            }
+                        }
+                        __syncthreads();
+                        const int jb = s_first;
+                        if (lane == 0 && jb < n) w1_forward<false>(P, R, A, B, Y, n, jb);
+                    } else if (lane == 0) {
+                        w1_forward<false>(P, R, A, B, Y, n, 0);
+                    }
+                    if (lane == 0) w1_backward(P, G, n);
+                    __syncthreads();
+                }
+            }
+            budget -= nst;
+            step += nst;
+            if (step >= sp->nsteps) {
+                ++seg;
+                step = 0;
+            }
+        }
+        for (int j = lane; j < n; j += 64) phi[j] = P[j];
+        if (lane == 0) {
+            ip->cur_seg = seg;
+            ip->cur_step = step;
+            ip->status = seg >= nseg ? 1 : 0;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Kernel 3: one axis of one step of one instance over many workgroups.
 // ---------------------------------------------------------------------------
 template <int D, int ORDER>
@@ -1754,6 +2079,19 @@ const void *pick_sweep(int D, int order)
         DH_CASE(4, 0) DH_CASE(4, 1) DH_CASE(4, 2)
         DH_CASE(5, 0) DH_CASE(5, 1) DH_CASE(5, 2)
 #undef DH_CASE
+    }
+    return nullptr;
+}
+
+const void *pick_1w(int order, int dj)
+{
+    switch (order * 2 + (dj ? 1 : 0)) {
+    case 0: return reinterpret_cast<const void *>(&k_integrate_1w<0, 0>);
+    case 1: return reinterpret_cast<const void *>(&k_integrate_1w<0, 1>);
+    case 2: return reinterpret_cast<const void *>(&k_integrate_1w<1, 0>);
+    case 3: return reinterpret_cast<const void *>(&k_integrate_1w<1, 1>);
+    case 4: return reinterpret_cast<const void *>(&k_integrate_1w<2, 0>);
+    case 5: return reinterpret_cast<const void *>(&k_integrate_1w<2, 1>);
     }
     return nullptr;
 }
@@ -1964,6 +2302,32 @@ int dh_integrate_1d(dh_instance *table_dev, int ninst, int order, int nmax, int 
     void *args[] = {&table_dev, &ninst, &counter_dev, &scratch_dev, (void *)&nlanes, &nmax, &ms};
     DH_HIP(hipLaunchKernel(pick_1d(order, cfg->delj), dim3((unsigned)groups), dim3(thr), args, 0,
                            st));
+    return DH_OK;
+}
+
+size_t dh_1d_wave_lds_bytes(int nmax)
+{
+    return (size_t)W1_ARRAYS * (size_t)(nmax + 2 * W1_PAD) * 8;
+}
+
+int dh_integrate_1d_wave(dh_instance *table_dev, int ninst, int order, int nmax,
+                         int *counter_dev, const dh_launch *cfg, void *stream)
+{
+    if (int e = check_init()) return e;
+    if (order < 0 || order > 2) return fail(DH_EINVAL, "bad order");
+    if (ninst <= 0) return DH_OK;
+    if (!table_dev || !counter_dev || !cfg) return fail(DH_EINVAL, "null pointer");
+    if (nmax < 3) return fail(DH_EINVAL, "n must be >= 3");
+    const void *fn = pick_1w(order, cfg->delj);
+    const size_t lds = dh_1d_wave_lds_bytes(nmax);
+    if (int e = set_lds(fn, lds)) return e;
+    long long groups = cfg->groups;
+    if (groups <= 0) groups = std::min<long long>(ninst, 32LL * g_num_cu);
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, counter_dev, 0);
+    long long ms = cfg->max_steps;
+    void *args[] = {&table_dev, &ninst, &counter_dev, &nmax, &ms};
+    DH_HIP(hipLaunchKernel(fn, dim3((unsigned)groups), dim3(64), args, lds, st));
     return DH_OK;
 }
 

@@ -45,6 +45,20 @@ class Context:
         from .autotune import Tuning
         self.tuning = Tuning.load()
 
+    #: 1D batches: "auto" (wave per integration while the batch is small
+    #: enough for the GPU to hold one wave per integration), "wave" or "lane"
+    one_d_kernel = "auto"
+    one_d_wave_max = 16384
+
+    def one_d_wave(self, ninst, nmax):
+        """Whether a 1D batch runs one wave per integration (LDS-resident,
+        latency-bound steps) rather than one lane per integration."""
+        if self.lib.dh_1d_wave_lds_bytes(int(nmax)) > self.lib.dh_max_lds():
+            return False
+        if self.one_d_kernel != "auto":
+            return self.one_d_kernel == "wave"
+        return ninst <= self.one_d_wave_max
+
     # -- helpers ---------------------------------------------------------
     def stream(self):
         return torch().cuda.current_stream(self.device).cuda_stream
@@ -469,6 +483,14 @@ class PackedBatch:
         ctx = self.ctx
         stream = ctx.stream() if stream is None else stream
         if self.D == 1:
+            if ctx.one_d_wave(len(self.jobs), self.nmax):
+                cfg = DhLaunch(1, groups, max_steps, 0, 0)
+                cfg.delj = 1 if self.delj else 0
+                check(ctx.lib.dh_integrate_1d_wave(self.tab.data_ptr(), len(self.jobs),
+                                                   self.order, self.nmax,
+                                                   self.counter.data_ptr(), cfg, stream),
+                      "dh_integrate_1d_wave")
+                return
             w = waves or 1
             thr = 64 * w
             g = groups or max(1, math.ceil(len(self.jobs) / thr))

@@ -314,7 +314,12 @@ class Server:
             counter = t.zeros(1, dtype=t.int32, device=ctx.device)
             nmax = int(tab["n"].max())
             h = st.cuda_stream
-            if D == 1:
+            if D == 1 and ctx.one_d_wave(len(group), nmax):
+                cfg = _lib.DhLaunch(1, 0, q, 0, 0)
+                _lib.check(ctx.lib.dh_integrate_1d_wave(tabd.data_ptr(), len(group), int(order),
+                                                        nmax, counter.data_ptr(), cfg, h),
+                           "dh_integrate_1d_wave")
+            elif D == 1:
                 thr = 64
                 g = max(1, math.ceil(len(group) / thr))
                 need = 6 * nmax * g * thr

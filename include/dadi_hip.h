@@ -177,6 +177,25 @@ size_t dh_packed_lds_bytes(int D, int n, int waves, int slots);
 int dh_integrate_1d(dh_instance *table_dev, int ninst, int order, int nmax,
                     int *counter_dev, double *scratch_dev, long long scratch_doubles,
                     const dh_launch *cfg, void *stream);
+/*
+ * Runs ninst 1-population instances, one wave per instance with phi and the
+ * tridiagonal system in LDS (latency path: a step costs a few dependent
+ * operations per grid point).  Within a segment every step has the same
+ * matrix, so its Thomas factorisation (gamma_j, bet_j of dadi/tridiag.c) is
+ * formed once per segment with the per-step operations and only the
+ * right-hand side is solved per step; results are the bits of
+ * dh_integrate_1d.  Needs dh_1d_wave_lds_bytes(nmax) of LDS per workgroup
+ * (at most dh_max_lds()); no scratch.  cfg->groups = 0 picks the grid;
+ * cfg->max_steps and cfg->delj as for dh_integrate_1d.
+ * Replaces the same reference code as dh_integrate_1d
+ * (dadi/Integration.py:986-1035 / 219-258, dadi/tridiag.c:36-40,
+ * dadi/integration1D.c:7-66).
+ */
+int dh_integrate_1d_wave(dh_instance *table_dev, int ninst, int order, int nmax,
+                         int *counter_dev, const dh_launch *cfg, void *stream);
+/* LDS per workgroup of dh_integrate_1d_wave, bytes. */
+size_t dh_1d_wave_lds_bytes(int nmax);
+
 
 /*
  * One implicit half-step along `axis` for a single instance, spread over as

@@ -110,23 +110,51 @@ def test_batch_equals_solo_and_isolates_bad_instances(dadi_amd, oracle):
     assert not np.all(np.isfinite(jobs[13].phi.cpu().numpy()))
 
 
-def test_1d_lane_kernel_matches_oracle(dadi_amd, oracle):
-    """The lane-per-instance 1D kernel (DFE-style gamma sweep) is exact."""
+@pytest.mark.parametrize("kernel", ["lane", "wave"])
+def test_1d_kernels_match_oracle(dadi_amd, oracle, kernel):
+    """Both batched 1D kernels are exact: one lane per integration, and one
+    wave per integration (Thomas factorisation formed once per segment,
+    divisions through the known reciprocals).  DFE-style gamma sweep down to
+    gamma = -2000, whose densities underflow at high frequency (the slow
+    division path); time-varying sizes (one step per segment); a frozen
+    population; several quanta per integration."""
     from dadi_amd import engine, schedule as S, _lib
     ctx = engine.context()
-    gammas = -np.logspace(np.log10(50), -3, 70)
+    gammas = np.concatenate([-np.logspace(np.log10(2000), -3, 40), [0.0, 3.0, 25.0]])
     jobs, refs = [], []
     for i, g in enumerate(gammas):
-        n = [40, 50, 60][i % 3]
+        n = [40, 50, 60, 130][i % 4]
         xx = oracle.default_grid(n)
         phi0 = oracle.phi_1D(xx, gamma=g)
+        T = 0.01 if abs(g) < 500 else 0.002
         dt = S.compute_dt(2.0, [0], g, 0.5)
-        seg = S.const_segments(1, xx, 0.0, 0.01, dt, [2.0], [[]], [g], [0.5], 1.0)
+        seg = S.const_segments(1, xx, 0.0, T, dt, [2.0], [[]], [g], [0.5], 1.0)
         jobs.append(engine.Job(ctx.to_dev(phi0), xx, seg, _lib.ORDER_PRE1, 0, 1))
-        refs.append(oracle.integrate_const(phi0, xx, 0.01, [2.0], [[]], [g], [0.5]))
-    engine.run_batch(jobs)
-    for j, r in zip(jobs, refs):
-        assert np.array_equal(j.phi.cpu().numpy(), r)
+        refs.append(oracle.integrate_const(phi0, xx, T, [2.0], [[]], [g], [0.5]))
+    xx = oracle.default_grid(70)
+    phi0 = oracle.phi_1D(xx)
+    nu = lambda t: 0.5 + 20 * t
+    c = lambda v: (lambda t: v)
+    seg = S.tv_segments(1, xx, 0.0, 0.03, [nu], [[]], [c(-3.0)], [c(0.3)], c(1.0), c(1.0))
+    jobs.append(engine.Job(ctx.to_dev(phi0), xx, seg, _lib.ORDER_TV, 0, 1))
+    refs.append(oracle.integrate_tv(phi0, xx, 0.03, [nu], [[]], [-3.0], [0.3]))
+    seg = S.const_segments(1, xx, 0.0, 0.05, 1e-3, [1.0], [[]], [0.0], [0.5], 1.0)
+    jobs.append(engine.Job(ctx.to_dev(phi0), xx, seg, _lib.ORDER_PRE1, 1, 1))
+    refs.append(phi0)
+    old = ctx.one_d_kernel
+    ctx.one_d_kernel = kernel
+    try:
+        for order in (_lib.ORDER_PRE1, _lib.ORDER_TV):   # a batch shares one order
+            b = engine.PackedBatch(ctx, [j for j in jobs if j.order == order])
+            for _ in range(200):
+                b.launch(max_steps=97)
+                if np.all(b.progress()["status"] == 1):
+                    break
+            assert np.all(b.progress()["status"] == 1)
+    finally:
+        ctx.one_d_kernel = old
+    for k, (j, r) in enumerate(zip(jobs, refs)):
+        assert np.array_equal(j.phi.cpu().numpy(), r), k
 
 
 @pytest.mark.parametrize("D,n,ns", [(1, 60, [20]), (2, 40, [10, 14]), (3, 16, [6, 5, 7]),
