@@ -1626,112 +1626,133 @@ __device__ __forceinline__ bool rcp_form_exact(double num)
 // of W1_PF, the next block's operands loading while the current one is
 // solved; the arrays carry W1_PAD doubles of padding on both sides, so the
 // look-ahead reads past the end without bounds arithmetic.
+struct W1Rows {
+    double r[W1_PF], a[W1_PF], b[W1_PF], y[W1_PF];
+};
+
+__device__ __forceinline__ void w1_load(W1Rows &B, const double *r, const double *a,
+                                        const double *b, const double *y)
+{
+#pragma unroll
+    for (int q = 0; q < W1_PF; ++q) {
+        B.r[q] = r[q];
+        B.a[q] = a[q];
+        B.b[q] = b[q];
+        B.y[q] = y[q];
+    }
+}
+
+template <bool MK>
+__device__ __forceinline__ double w1_quot(double num, double bet, double y)
+{
+    if constexpr (MK) {
+        const double q0 = num * y;
+        const double e = __builtin_fma(-bet, q0, num);
+        return __builtin_fma(e, y, q0);
+    } else {
+        return num / bet;
+    }
+}
+
+template <bool MK>
+__device__ __forceinline__ double w1_rows(const W1Rows &B, double u, double *p, int count)
+{
+#pragma unroll
+    for (int q = 0; q < W1_PF; ++q) {
+        if (q < count) {
+            u = w1_quot<MK>(B.r[q] - B.a[q] * u, B.b[q], B.y[q]);
+            p[q] = u;
+        }
+    }
+    return u;
+}
+
 template <bool MK>
 __device__ __forceinline__ void w1_forward(double *__restrict__ P, const double *__restrict__ R,
                                            const double *__restrict__ A,
                                            const double *__restrict__ B,
                                            const double *__restrict__ Y, int n, int j0)
 {
-    auto quot = [](double num, double bet, double y) -> double {
-        if constexpr (MK) {
-            const double q0 = num * y;
-            const double e = __builtin_fma(-bet, q0, num);
-            return __builtin_fma(e, y, q0);
-        } else {
-            return num / bet;
-        }
-    };
-    double u = j0 == 0 ? quot(R[0], B[0], Y[0]) : quot(R[j0] - A[j0] * P[j0 - 1], B[j0], Y[j0]);
+    double u = j0 == 0 ? w1_quot<MK>(R[0], B[0], Y[0])
+                       : w1_quot<MK>(R[j0] - A[j0] * P[j0 - 1], B[j0], Y[j0]);
     P[j0] = u;
     const double *r = R + j0 + 1, *a = A + j0 + 1, *b = B + j0 + 1, *y = Y + j0 + 1;
     double *p = P + j0 + 1;
-    double nr[W1_PF], na_[W1_PF], nb[W1_PF], ny[W1_PF];
-#pragma unroll
-    for (int q = 0; q < W1_PF; ++q) {
-        nr[q] = r[q];
-        na_[q] = a[q];
-        nb[q] = b[q];
-        ny[q] = y[q];
+    const int rows = n - 1 - j0, full = rows / W1_PF, rem = rows - full * W1_PF;
+    // two register blocks used in turn: one is solved while the other loads
+    W1Rows X, Z;
+    w1_load(X, r, a, b, y);
+    int k = 0;
+    for (; k + 2 <= full; k += 2) {
+        w1_load(Z, r + W1_PF, a + W1_PF, b + W1_PF, y + W1_PF);
+        u = w1_rows<MK>(X, u, p, W1_PF);
+        w1_load(X, r + 2 * W1_PF, a + 2 * W1_PF, b + 2 * W1_PF, y + 2 * W1_PF);
+        u = w1_rows<MK>(Z, u, p + W1_PF, W1_PF);
+        r += 2 * W1_PF;
+        a += 2 * W1_PF;
+        b += 2 * W1_PF;
+        y += 2 * W1_PF;
+        p += 2 * W1_PF;
     }
-    const int rows = n - 1 - j0, full = rows / W1_PF;
-    double cr[W1_PF], ca[W1_PF], cb[W1_PF], cy[W1_PF];
-    for (int k = 0; k < full; ++k) {
-#pragma unroll
-        for (int q = 0; q < W1_PF; ++q) {
-            cr[q] = nr[q];
-            ca[q] = na_[q];
-            cb[q] = nb[q];
-            cy[q] = ny[q];
-        }
-#pragma unroll
-        for (int q = 0; q < W1_PF; ++q) {
-            nr[q] = r[W1_PF + q];
-            na_[q] = a[W1_PF + q];
-            nb[q] = b[W1_PF + q];
-            ny[q] = y[W1_PF + q];
-        }
-#pragma unroll
-        for (int q = 0; q < W1_PF; ++q) {
-            u = quot(cr[q] - ca[q] * u, cb[q], cy[q]);
-            p[q] = u;
-        }
-        r += W1_PF;
-        a += W1_PF;
-        b += W1_PF;
-        y += W1_PF;
-        p += W1_PF;
-    }
-    const int rem = rows - full * W1_PF;
-#pragma unroll
-    for (int q = 0; q < W1_PF; ++q) {
-        if (q < rem) {
-            u = quot(nr[q] - na_[q] * u, nb[q], ny[q]);
-            p[q] = u;
-        }
+    if (k < full) {
+        w1_load(Z, r + W1_PF, a + W1_PF, b + W1_PF, y + W1_PF);
+        u = w1_rows<MK>(X, u, p, W1_PF);
+        w1_rows<MK>(Z, u, p + W1_PF, rem);
+    } else {
+        w1_rows<MK>(X, u, p, rem);
     }
 }
 
 // Back substitution of one step on one lane (dadi/tridiag.c:31-33), blocks
 // of W1_PF rows going down, padded arrays as in w1_forward.
+__device__ __forceinline__ double w1_back_rows(const double (&cp)[W1_PF],
+                                               const double (&cg)[W1_PF], double u, double *p,
+                                               int count)
+{
+#pragma unroll
+    for (int q = 0; q < W1_PF; ++q) {
+        if (q < count) {
+            u = cp[q] - cg[q] * u;
+            p[-q] = u;
+        }
+    }
+    return u;
+}
+
+__device__ __forceinline__ void w1_back_load(double (&cp)[W1_PF], double (&cg)[W1_PF],
+                                             const double *p, const double *g)
+{
+#pragma unroll
+    for (int q = 0; q < W1_PF; ++q) {
+        cp[q] = p[-q];
+        cg[q] = g[-q];
+    }
+}
+
 __device__ __forceinline__ void w1_backward(double *__restrict__ P,
                                             const double *__restrict__ G, int n)
 {
     double u = P[n - 1];
     double *p = P + n - 2;
     const double *g = G + n - 1;
-    double np_[W1_PF], ng[W1_PF], cp[W1_PF], cg[W1_PF];
-#pragma unroll
-    for (int q = 0; q < W1_PF; ++q) {
-        np_[q] = p[-q];
-        ng[q] = g[-q];
+    const int rows = n - 1, full = rows / W1_PF, rem = rows - full * W1_PF;
+    double xp[W1_PF], xg[W1_PF], zp[W1_PF], zg[W1_PF];
+    w1_back_load(xp, xg, p, g);
+    int k = 0;
+    for (; k + 2 <= full; k += 2) {
+        w1_back_load(zp, zg, p - W1_PF, g - W1_PF);
+        u = w1_back_rows(xp, xg, u, p, W1_PF);
+        w1_back_load(xp, xg, p - 2 * W1_PF, g - 2 * W1_PF);
+        u = w1_back_rows(zp, zg, u, p - W1_PF, W1_PF);
+        p -= 2 * W1_PF;
+        g -= 2 * W1_PF;
     }
-    const int rows = n - 1, full = rows / W1_PF;
-    for (int k = 0; k < full; ++k) {
-#pragma unroll
-        for (int q = 0; q < W1_PF; ++q) {
-            cp[q] = np_[q];
-            cg[q] = ng[q];
-        }
-#pragma unroll
-        for (int q = 0; q < W1_PF; ++q) {
-            np_[q] = p[-W1_PF - q];
-            ng[q] = g[-W1_PF - q];
-        }
-#pragma unroll
-        for (int q = 0; q < W1_PF; ++q) {
-            u = cp[q] - cg[q] * u;
-            p[-q] = u;
-        }
-        p -= W1_PF;
-        g -= W1_PF;
-    }
-    const int rem = rows - full * W1_PF;
-#pragma unroll
-    for (int q = 0; q < W1_PF; ++q) {
-        if (q < rem) {
-            u = np_[q] - ng[q] * u;
-            p[-q] = u;
-        }
+    if (k < full) {
+        w1_back_load(zp, zg, p - W1_PF, g - W1_PF);
+        u = w1_back_rows(xp, xg, u, p, W1_PF);
+        w1_back_rows(zp, zg, u, p - W1_PF, rem);
+    } else {
+        w1_back_rows(xp, xg, u, p, rem);
     }
 }
 
