@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("DADI_AMD_LIB") or os.path.join(HERE, "libdadi_hip.so")
 
 MAXD = 5
-ABI_VERSION = 5
+ABI_VERSION = 6
 ORDER_TV, ORDER_PRE, ORDER_PRE1 = 0, 1, 2
 
 #: numpy mirror of ``dh_segment``
@@ -60,7 +60,7 @@ class DhLaunch(ctypes.Structure):
                 ("max_steps", ctypes.c_longlong), ("chunk", ctypes.c_int),
                 ("recompute", ctypes.c_int), ("gamma_dev", ctypes.c_void_p),
                 ("gamma_doubles", ctypes.c_longlong), ("delj", ctypes.c_int),
-                ("reserved", ctypes.c_int)]
+                ("max_us", ctypes.c_int)]
 
 
 _vp = ctypes.c_void_p

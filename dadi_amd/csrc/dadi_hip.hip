@@ -48,6 +48,7 @@ int fail(int code, const std::string &msg)
     } while (0)
 
 int g_num_cu = 0;
+long long g_wall_khz = 0;   // rate of wall_clock64()
 size_t g_max_lds = 0;
 bool g_inited = false;
 
@@ -1194,10 +1195,10 @@ __device__ __forceinline__ void sweep_tiles(double *__restrict__ phi, int n, lon
 template <int D, int ORDER, int CK, int RU, int GG, int DJ = 0>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(DH_WPE, 8)))
 k_integrate_nd(dh_instance *__restrict__ tab, int ninst, int *__restrict__ counter,
-               long long max_steps, double *__restrict__ gws, int nmax)
+               long long max_steps, double *__restrict__ gws, int nmax, long long max_ticks)
 {
     extern __shared__ double lds[];
-    __shared__ int s_next;
+    __shared__ int s_next, s_stop;
     const int nthr = blockDim.x, tid = threadIdx.x;
     const int nw = nthr >> 6, wave = tid >> 6, lane = tid & 63;
     for (;;) {
@@ -1230,7 +1231,11 @@ k_integrate_nd(dh_instance *__restrict__ tab, int ninst, int *__restrict__ count
         const int frozen = ip->frozen, injm = ip->inject;
         double *phi = gbl(ip->phi_dev);
         const long long ld = ip->ld > 0 ? ip->ld : n;
-        while (seg < nseg && budget > 0) {
+        // time budget: thread 0 reads the clock, the whole group follows its
+        // decision (read between two barriers)
+        const long long t0 = max_ticks > 0 ? (long long)wall_clock64() : 0;
+        bool out_of_time = false;
+        while (seg < nseg && budget > 0 && !out_of_time) {
             const dh_segment *sp = gbl(ip->seg_dev) + seg;
             __syncthreads();
             for (int a = 0; a < D; ++a) {
@@ -1256,6 +1261,15 @@ k_integrate_nd(dh_instance *__restrict__ tab, int ninst, int *__restrict__ count
                                                           dj);
                     __syncthreads();
                     first = false;
+                }
+                if (max_ticks > 0) {
+                    if (tid == 0) s_stop = (long long)wall_clock64() - t0 >= max_ticks;
+                    __syncthreads();
+                    if (s_stop) {
+                        nst = k + 1;
+                        out_of_time = true;
+                        break;
+                    }
                 }
             }
             budget -= nst;
@@ -1759,7 +1773,7 @@ __device__ __forceinline__ void w1_backward(double *__restrict__ P,
 template <int ORDER, int DJ>
 __global__ void __launch_bounds__(64) k_integrate_1w(dh_instance *__restrict__ tab, int ninst,
                                                      int *__restrict__ counter, int nmax,
-                                                     long long max_steps)
+                                                     long long max_steps, long long max_ticks)
 {
     extern __shared__ double lds[];
     __shared__ int s_id, s_first;
@@ -1785,7 +1799,9 @@ __global__ void __launch_bounds__(64) k_integrate_1w(dh_instance *__restrict__ t
         const bool frz = ip->frozen & 1;
         const double dx0 = xx[1] - xx[0], dxl = xx[n - 1] - xx[n - 2];
         __syncthreads();
-        while (seg < nseg && budget > 0) {
+        const long long t0 = max_ticks > 0 ? (long long)wall_clock64() : 0;
+        bool out_of_time = false;
+        while (seg < nseg && budget > 0 && !out_of_time) {
             const dh_segment *sp = gbl(ip->seg_dev) + seg;
             long long nst = sp->nsteps - step;
             if (nst > budget) nst = budget;
@@ -1911,6 +1927,12 @@ __global__ void __launch_bounds__(64) k_integrate_1w(dh_instance *__restrict__ t
                     }
                     if (lane == 0) w1_backward(P, G, n);
                     __syncthreads();
+                    // the clock is a scalar read: the wave agrees on it
+                    if (max_ticks > 0 && (long long)wall_clock64() - t0 >= max_ticks) {
+                        nst = k + 1;
+                        out_of_time = true;
+                        break;
+                    }
                 }
             }
             budget -= nst;
@@ -2165,6 +2187,12 @@ int check_init()
     return DH_OK;
 }
 
+// cfg->max_us in ticks of wall_clock64(); 0 = no time budget
+long long budget_ticks(const dh_launch *cfg)
+{
+    return cfg->max_us > 0 ? (long long)cfg->max_us * g_wall_khz / 1000 : 0;
+}
+
 int set_lds(const void *fn, size_t bytes)
 {
     if (bytes > g_max_lds)
@@ -2181,7 +2209,7 @@ std::map<std::string, hipGraphExec_t> g_graphs;
 
 extern "C" {
 
-int dh_abi_version(void) { return 5; }
+int dh_abi_version(void) { return 6; }
 size_t dh_sizeof_segment(void) { return sizeof(dh_segment); }
 size_t dh_sizeof_instance(void) { return sizeof(dh_instance); }
 size_t dh_lds_bytes(int D, int n, int waves, int chunk, int recompute, int gamma_global,
@@ -2226,6 +2254,11 @@ int dh_init(int device)
         maxsh <= 0)
         maxsh = (int)prop.sharedMemPerBlock;
     g_max_lds = (size_t)maxsh;
+    int wk = 0;
+    if (hipDeviceGetAttribute(&wk, hipDeviceAttributeWallClockRate, device) != hipSuccess ||
+        wk <= 0)
+        wk = 100000;   // 100 MHz, the CDNA constant-rate counter
+    g_wall_khz = wk;
     g_inited = true;
     return DH_OK;
 }
@@ -2261,7 +2294,8 @@ int dh_integrate_nd(dh_instance *table_dev, int ninst, int D, int order, int nma
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, counter_dev, 0);
     long long ms = cfg->max_steps;
     double *gws = cfg->gamma_dev;
-    void *args[] = {&table_dev, &ninst, &counter_dev, &ms, &gws, &nmax};
+    long long ticks = budget_ticks(cfg);
+    void *args[] = {&table_dev, &ninst, &counter_dev, &ms, &gws, &nmax, &ticks};
     DH_HIP(hipLaunchKernel(fn, dim3(groups), dim3(64 * waves), args, lds, st));
     return DH_OK;
 }
@@ -2347,7 +2381,8 @@ int dh_integrate_1d_wave(dh_instance *table_dev, int ninst, int order, int nmax,
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, counter_dev, 0);
     long long ms = cfg->max_steps;
-    void *args[] = {&table_dev, &ninst, &counter_dev, &nmax, &ms};
+    long long ticks = budget_ticks(cfg);
+    void *args[] = {&table_dev, &ninst, &counter_dev, &nmax, &ms, &ticks};
     DH_HIP(hipLaunchKernel(fn, dim3((unsigned)groups), dim3(64), args, lds, st));
     return DH_OK;
 }

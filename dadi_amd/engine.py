@@ -215,12 +215,15 @@ class Context:
 
     def nd_launch(self, tab_ptr, ninst, D, order, nmax, counter_ptr, stream, workspace,
                   waves=None, groups=0, max_steps=0, chunk=None, recompute=None,
-                  gamma_global=None, slots=None, packable=False, delj=False):
+                  gamma_global=None, slots=None, packable=False, delj=False, max_us=0):
         """One dh_integrate_nd (or, with slots >= 2, dh_integrate_packed)
         call.  ``workspace`` is a dict owned by the caller (one per stream:
         launches that may overlap must not share it) that keeps the gamma
         workspace between calls.  ``packable``: every instance has n = nmax
-        and the same grid."""
+        and the same grid.  ``max_us`` > 0: time budget per instance (the
+        unpacked kernel only, so it forbids packing)."""
+        if max_us > 0:
+            packable = False
         if delj:   # the delta trick runs on the plain Thomas path only
             tiles = -(-(nmax ** (D - 1)) // 64)
             w = waves or max(1, min(8, tiles, self.max_waves(D, nmax, 0, 0, 0, 1)))
@@ -229,11 +232,13 @@ class Context:
                                           "which n=%d at D=%d exceeds" % (nmax, D))
             cfg = DhLaunch(w, groups, max_steps, 0, 0)
             cfg.delj = 1
+            cfg.max_us = int(max_us)
             check(self.lib.dh_integrate_nd(tab_ptr, ninst, D, order, nmax, counter_ptr, cfg,
                                            stream), "dh_integrate_nd")
             return Shape(w, 0, 0, 0, 1)
         sh = self.shape(D, nmax, waves, chunk, recompute, gamma_global, slots, packable, ninst)
         cfg = DhLaunch(sh.waves, groups, max_steps, sh.chunk, sh.recompute)
+        cfg.max_us = int(max_us)
         if sh.slots >= 2:
             check(self.lib.dh_integrate_packed(tab_ptr, ninst, D, order, nmax, sh.slots,
                                                counter_ptr, cfg, stream), "dh_integrate_packed")

@@ -15,9 +15,14 @@ next operations (device-side splits, sampling) and either starts its next
 epoch in the following quantum or completes.  New arrivals join at the next
 quantum -- they do not wait for a batch to drain.
 
-Quanta are sized in time, not steps: each class's step count per launch
-adapts so that a launch lasts about ``quantum_ms``, so that classes of cheap
+Quanta are sized in time, not steps: in a launch every instance advances as
+many steps as fit in about ``quantum_ms`` (a wall-clock budget checked by
+the kernel after each step), so a small integration is not held to the step
+count of the largest one it shares the launch with, and classes of cheap
 steps (1D, small 2D) are not held back behind classes of expensive ones.
+Where a kernel has no time budget (the lane-per-instance 1D kernel, the
+multi-workgroup path) the class's step count per launch adapts toward the
+same duration.
 Epochs whose sweeps have many lines (3D and 4D phi) would run their steps
 serially on one workgroup in the batched launch; they take the
 multi-workgroup path instead (every sweep of a step spread over the whole
@@ -314,8 +319,10 @@ class Server:
             counter = t.zeros(1, dtype=t.int32, device=ctx.device)
             nmax = int(tab["n"].max())
             h = st.cuda_stream
+            budget_us = int(self.quantum_ms * 1000)
             if D == 1 and ctx.one_d_wave(len(group), nmax):
-                cfg = _lib.DhLaunch(1, 0, q, 0, 0)
+                cfg = _lib.DhLaunch(1, 0, 0, 0, 0)   # a time budget, not a step count
+                cfg.max_us = budget_us
                 _lib.check(ctx.lib.dh_integrate_1d_wave(tabd.data_ptr(), len(group), int(order),
                                                         nmax, counter.data_ptr(), cfg, h),
                            "dh_integrate_1d_wave")
@@ -335,7 +342,7 @@ class Server:
             else:
                 ws = self.scratch.setdefault(key, {})
                 ctx.nd_launch(tabd.data_ptr(), len(group), int(D), int(order), nmax,
-                              counter.data_ptr(), h, ws, max_steps=q)
+                              counter.data_ptr(), h, ws, max_us=budget_us)
         self.launches += 1
         return st, tabd, counter, key, q, ev0
 

@@ -114,7 +114,12 @@ typedef struct dh_launch {
                               recompute, no workspace) and is not available to
                               dh_integrate_packed.  Device exp: results agree with
                               the reference to rounding, not bit for bit. */
-    int reserved;
+    int max_us;            /* > 0: time budget of this call (dh_integrate_nd,
+                              dh_integrate_1d_wave): a workgroup takes no further time
+                              step of an instance once this many microseconds have
+                              passed since it took the instance up (at least one step
+                              is taken); progress is recorded as with max_steps.
+                              0 = none. */
 } dh_launch;
 
 /* Library / device set-up; must be called once per process before use.

@@ -408,3 +408,53 @@ def test_delj_trick_matches_reference(dadi_amd, oracle):
         assert np.array_equal(res[1], got2) and np.array_equal(res[3], got2)
     finally:
         I.use_delj_trick = False
+
+
+@pytest.mark.parametrize("D,n", [(1, 300), (2, 40), (3, 14)])
+def test_time_budget_quanta_bit_exact(dadi_amd, oracle, D, n):
+    """A launch with a wall-clock budget (dh_launch.max_us) stops taking
+    steps once the budget is spent and records its progress; resuming until
+    done gives the oracle's bits, whatever the number of launches."""
+    from dadi_amd import engine, schedule as S, _lib
+    ctx = engine.context()
+    xx = oracle.default_grid(n)
+    phi0 = oracle.phi_1D(xx)
+    if D >= 2:
+        phi0 = oracle.phi_1D_to_2D(xx, phi0)
+    if D == 3:
+        phi0 = oracle.phi_2D_to_3D_split_2(xx, phi0)
+    nus = [0.7, 1.6, 0.9][:D]
+    migs = [[1.2] * (D - 1) for _ in range(D)]
+    gam = [-2.0, 1.0, 0.0][:D]
+    T = 0.05
+    dt = min(S.compute_dt(nus[k], migs[k] if D > 1 else [0], gam[k], 0.5) for k in range(D))
+    seg = S.const_segments(D, xx, 0.0, T, dt, nus, migs if D > 1 else [[]], gam, [0.5] * D, 1.0)
+    want = oracle.integrate_const(phi0, xx, T, nus, migs if D > 1 else [[]], gam, [0.5] * D)
+    order = _lib.ORDER_PRE1 if D == 1 else _lib.ORDER_PRE
+    import torch
+    phis = [ctx.to_dev(phi0) for _ in range(3)]
+    segd = ctx.to_dev(seg.view(np.uint8))
+    xxd = ctx.grid(xx)
+    fz, inj = S.masks(D, [False] * D, [False] * D)
+    tab = ctx.to_dev(np.concatenate([engine._inst_record(
+        p.data_ptr(), xxd.data_ptr(), segd.data_ptr(), len(seg), D, n, order, fz, inj,
+        engine.pitch_of(p)) for p in phis]).view(np.uint8))
+    counter = torch.zeros(1, dtype=torch.int32, device=ctx.device)
+    launches = 0
+    while True:
+        if D == 1:
+            cfg = _lib.DhLaunch(1, 0, 0, 0, 0)
+            cfg.max_us = 40
+            _lib.check(ctx.lib.dh_integrate_1d_wave(tab.data_ptr(), 3, order, n,
+                                                     counter.data_ptr(), cfg, ctx.stream()),
+                       "dh_integrate_1d_wave")
+        else:
+            ctx.nd_launch(tab.data_ptr(), 3, D, order, n, counter.data_ptr(), ctx.stream(), {},
+                          max_us=40)
+        launches += 1
+        st = tab.cpu().numpy().view(_lib.INST_DTYPE)
+        if np.all(st["status"] == 1) or launches > 100000:
+            break
+    assert launches > 2, launches
+    for p in phis:
+        assert np.array_equal(p.cpu().numpy(), want)
