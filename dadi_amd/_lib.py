@@ -83,6 +83,9 @@ _SIGS = {
     "dh_integrate_1d_wave": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp,
                               ctypes.POINTER(DhLaunch), _vp], ctypes.c_int),
     "dh_1d_wave_lds_bytes": ([ctypes.c_int], ctypes.c_size_t),
+    "dh_integrate_factored": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp,
+                               ctypes.POINTER(DhLaunch), _vp], ctypes.c_int),
+    "dh_factored_slice_doubles": ([ctypes.c_int, ctypes.c_int], ctypes.c_longlong),
     "dh_sweep_axis": ([ctypes.POINTER(DhInstance), ctypes.c_int, ctypes.c_int, _vp,
                        ctypes.c_longlong, _vp, ctypes.c_longlong, ctypes.c_int, _vp],
                       ctypes.c_int),

@@ -1953,6 +1953,323 @@ __global__ void __launch_bounds__(64) k_integrate_1w(dh_instance *__restrict__ t
 }
 
 // ---------------------------------------------------------------------------
+// Kernel 1c: latency path for 2-5 populations -- one workgroup per instance,
+// the Thomas factorisation of every line formed once per segment.
+//
+// As in the wave-1D kernel, every step of a segment has the same tridiagonal
+// matrices, so a_j, bet_j, 1/bet_j and gamma_j of every line of every axis
+// are formed once per segment with the per-step operations (row0, row_coeffs
+// and dadi/tridiag.c:22-29) into a global workspace laid out [row][line], so
+// that the lanes of a wave (consecutive lines) read consecutive addresses.  A
+// step then runs only the right-hand-side recurrences, a few dependent
+// operations per row instead of the coefficient build and three divisions.
+// The forward pass writes u' to the workspace (phi stays intact until the
+// back substitution overwrites it), so that a line whose numerators leave the
+// range where the reciprocal form is exact can be redone with divisions from
+// its first such row.  More memory traffic per point than k_integrate_nd
+// (the factorisation is read every step): for few instances, where latency
+// rather than bandwidth bounds a step.
+// ---------------------------------------------------------------------------
+
+constexpr int FZ_PF = 8;   // rows loaded ahead
+
+// Workspace doubles per workgroup: a, bet, 1/bet, gamma per axis, plus u'.
+long long fz_slice(int D, int n)
+{
+    long long N = 1;
+    for (int q = 0; q < D; ++q) N *= n;
+    return (4LL * D + 1) * N;
+}
+
+__device__ __forceinline__ bool fz_bet_ok(double bet)
+{
+    const double ab = __builtin_fabs(bet);
+    return ab >= 0x1p-16 && ab < 0x1p57;
+}
+
+// Factorisation of every line of axis `a`: lines over the lanes of the
+// workgroup, tile t of 64 lines on wave t % nw (the same mapping as the
+// per-step sweeps, so each lane reads back only what it wrote).
+template <int D, int ORDER>
+__device__ void fz_factor_axis(int a, int n, long long L, const dh_segment *sp,
+                               const double *sx, const double *sxi, const double *sdf,
+                               const double *sdx, const double *W0, const double *W1,
+                               const double *S, double dt, double inv_dt, double *Fa,
+                               long long tile0, long long tstep, int lane, bool &safe)
+{
+    constexpr int NO = D - 1;
+    const long long N = L * n;
+    double *A = Fa, *Bt = Fa + N, *Yr = Fa + 2 * N, *G = Fa + 3 * N;
+    double mig[NO];
+#pragma unroll
+    for (int q = 0; q < NO; ++q) mig[q] = sp->mig[a][q];
+    const double nu = sp->nu[a], ga = sp->gamma[a], ha = sp->h[a];
+    for (long long t = tile0; t * 64 < L; t += tstep) {
+        const long long line = t * 64 + lane;
+        if (line >= L) continue;
+        LineSetup<D, ORDER> Ls;
+        line_setup<D, ORDER>(Ls, line, n, a, sx, mig, nu, ga, ha, sdx[0], sdx[n - 2], false, 0,
+                             sp->chrom);
+        double bet, u0, pa, pc;
+        row0<NO, ORDER>(0.0, sxi, sdf, W0, W1, S, Ls.c, mig, dt, inv_dt, Ls.bcf_on, Ls.bcf, bet,
+                        u0, pa, pc);
+        A[line] = 0.;
+        Bt[line] = bet;
+        Yr[line] = 1. / bet;
+        G[line] = 0.;
+        safe = safe && fz_bet_ok(bet);
+        for (int j = 1; j < n; ++j) {
+            double aj, bj, cm;
+            row_coeffs<NO, ORDER>(j, n, sxi, sdf, W0, W1, S, 1, Ls.c, mig, inv_dt, Ls.bcl_on,
+                                  Ls.bcl, pa, pc, aj, bj, cm);
+            const double g = cm / bet;
+            bet = bj - aj * g;
+            const long long o = (long long)j * L + line;
+            A[o] = aj;
+            Bt[o] = bet;
+            Yr[o] = 1. / bet;
+            G[o] = g;
+            safe = safe && fz_bet_ok(bet);
+        }
+    }
+}
+
+template <int ORDER>
+__device__ __forceinline__ double fz_rhs(double p, double dt, double inv_dt)
+{
+    return (ORDER == DH_ORDER_PRE) ? inv_dt * p : p / dt;
+}
+
+// One step's sweep of axis `a` from the factorisation.  MK: divisions through
+// the known reciprocals (each line redone with divisions from its first row
+// outside the exact range).
+template <int D, int ORDER, bool MK>
+__device__ void fz_sweep_axis(double *__restrict__ phi, int n, long long ld, long long L, int a,
+                              bool first_axis, int injm, const dh_segment *sp, const double *sx,
+                              const double *sdx, double dt, double inv_dt,
+                              const double *__restrict__ Fa, double *__restrict__ U,
+                              long long tile0, long long tstep, int lane)
+{
+    const long long N = L * n;
+    const double *A = Fa, *Bt = Fa + N, *Yr = Fa + 2 * N, *G = Fa + 3 * N;
+    double mig[D - 1];
+#pragma unroll
+    for (int q = 0; q < D - 1; ++q) mig[q] = sp->mig[a][q];
+    const double inj = sp->inject;
+    for (long long t = tile0; t * 64 < L; t += tstep) {
+        const long long line = t * 64 + lane;
+        if (line >= L) continue;
+        LineSetup<D, ORDER> Ls;
+        line_setup<D, ORDER>(Ls, line, n, a, sx, mig, sp->nu[a], sp->gamma[a], sp->h[a], sdx[0],
+                             sdx[n - 2], first_axis, injm, sp->chrom);
+        long long off, s;
+        line_addr<D>(n, ld, a, line, off, s);
+        double *p = phi + off;
+        if (Ls.inj_row >= 0) p[Ls.inj_row * s] = p[Ls.inj_row * s] + inj;
+
+        // forward, rows 0..n-1, u' into U
+        int bad = n;
+        double u;
+        {
+            const double r0 = fz_rhs<ORDER>(p[0], dt, inv_dt);
+            if constexpr (MK) {
+                const double q0 = r0 * Yr[line];
+                u = __builtin_fma(__builtin_fma(-Bt[line], q0, r0), Yr[line], q0);
+                if (!rcp_form_exact(r0)) bad = 0;
+            } else {
+                u = r0 / Bt[line];
+            }
+            U[line] = u;
+        }
+        double np_[FZ_PF], na_[FZ_PF], nb[FZ_PF], ny[FZ_PF];
+#pragma unroll
+        for (int q = 0; q < FZ_PF; ++q) {
+            const int jj = 1 + q < n ? 1 + q : n - 1;
+            const long long o = (long long)jj * L + line;
+            np_[q] = p[jj * s];
+            na_[q] = A[o];
+            nb[q] = Bt[o];
+            ny[q] = Yr[o];
+        }
+        for (int j0 = 1; j0 < n; j0 += FZ_PF) {
+            double cp[FZ_PF], ca[FZ_PF], cb[FZ_PF], cy[FZ_PF];
+#pragma unroll
+            for (int q = 0; q < FZ_PF; ++q) {
+                cp[q] = np_[q];
+                ca[q] = na_[q];
+                cb[q] = nb[q];
+                cy[q] = ny[q];
+            }
+#pragma unroll
+            for (int q = 0; q < FZ_PF; ++q) {
+                const int jj = j0 + FZ_PF + q < n ? j0 + FZ_PF + q : n - 1;
+                const long long o = (long long)jj * L + line;
+                np_[q] = p[jj * s];
+                na_[q] = A[o];
+                nb[q] = Bt[o];
+                ny[q] = Yr[o];
+            }
+#pragma unroll
+            for (int q = 0; q < FZ_PF; ++q) {
+                const int j = j0 + q;
+                if (j < n) {
+                    const double num = fz_rhs<ORDER>(cp[q], dt, inv_dt) - ca[q] * u;
+                    if constexpr (MK) {
+                        const double q0 = num * cy[q];
+                        u = __builtin_fma(__builtin_fma(-cb[q], q0, num), cy[q], q0);
+                        if (bad == n && !rcp_form_exact(num)) bad = j;
+                    } else {
+                        u = num / cb[q];
+                    }
+                    U[(long long)j * L + line] = u;
+                }
+            }
+        }
+        if (MK && bad < n) {
+            // rows from the first one outside the exact range, with divisions
+            u = bad == 0 ? 0. : U[(long long)(bad - 1) * L + line];
+            for (int j = bad; j < n; ++j) {
+                const long long o = (long long)j * L + line;
+                const double r = fz_rhs<ORDER>(p[j * s], dt, inv_dt);
+                u = (j == 0 ? r : r - A[o] * u) / Bt[o];
+                U[o] = u;
+            }
+        }
+
+        // back substitution (dadi/tridiag.c:31-33), u' from U, result into phi
+        u = U[(long long)(n - 1) * L + line];
+        p[(long long)(n - 1) * s] = u;
+        double nu_[FZ_PF], ng[FZ_PF];
+#pragma unroll
+        for (int q = 0; q < FZ_PF; ++q) {
+            const int jj = n - 2 - q >= 0 ? n - 2 - q : 0;
+            nu_[q] = U[(long long)jj * L + line];
+            ng[q] = G[(long long)(jj + 1) * L + line];
+        }
+        for (int j0 = n - 2; j0 >= 0; j0 -= FZ_PF) {
+            double cu[FZ_PF], cg[FZ_PF];
+#pragma unroll
+            for (int q = 0; q < FZ_PF; ++q) {
+                cu[q] = nu_[q];
+                cg[q] = ng[q];
+            }
+#pragma unroll
+            for (int q = 0; q < FZ_PF; ++q) {
+                const int jj = j0 - FZ_PF - q >= 0 ? j0 - FZ_PF - q : 0;
+                nu_[q] = U[(long long)jj * L + line];
+                ng[q] = G[(long long)(jj + 1) * L + line];
+            }
+#pragma unroll
+            for (int q = 0; q < FZ_PF; ++q) {
+                const int j = j0 - q;
+                if (j >= 0) {
+                    u = cu[q] - cg[q] * u;
+                    p[j * s] = u;
+                }
+            }
+        }
+    }
+}
+
+template <int D, int ORDER>
+__global__ void __launch_bounds__(512) k_integrate_fz(dh_instance *__restrict__ tab, int ninst,
+                                                       int *__restrict__ counter,
+                                                       long long max_steps,
+                                                       double *__restrict__ ws, long long slice,
+                                                       long long max_ticks)
+{
+    extern __shared__ double lds[];
+    __shared__ int s_next, s_stop, s_unsafe;
+    const int nthr = blockDim.x, tid = threadIdx.x;
+    const int nw = nthr >> 6, wave = tid >> 6, lane = tid & 63;
+    double *F = ws + (long long)blockIdx.x * slice;
+    for (;;) {
+        __syncthreads();
+        if (tid == 0) s_next = atomicAdd(counter, 1);
+        __syncthreads();
+        const int id = __builtin_amdgcn_readfirstlane(s_next);
+        if (id >= ninst) break;
+        dh_instance *ip = tab + id;
+        const int n = ip->n;
+        long long L = 1;
+        for (int q = 0; q < D - 1; ++q) L *= n;
+        const long long N = L * n;
+        double *U = F + 4LL * D * N;
+        double *sx = lds, *sxi = sx + n, *sdf = sxi + n, *sdx = sdf + n;
+        double *tabs = sdx + n;
+        build_grid_tables(gbl(ip->xx_dev), n, sx, sxi, sdf, sdx, tid, nthr);
+        long long seg = ip->cur_seg, step = ip->cur_step;
+        const long long nseg = ip->nseg;
+        long long budget = max_steps > 0 ? max_steps : LLONG_MAX;
+        const int frozen = ip->frozen, injm = ip->inject;
+        double *phi = gbl(ip->phi_dev);
+        const long long ld = ip->ld > 0 ? ip->ld : n;
+        const long long t0 = max_ticks > 0 ? (long long)wall_clock64() : 0;
+        bool out_of_time = false;
+        while (seg < nseg && budget > 0 && !out_of_time) {
+            const dh_segment *sp = gbl(ip->seg_dev) + seg;
+            long long nst = sp->nsteps - step;
+            if (nst > budget) nst = budget;
+            __syncthreads();
+            for (int a = 0; a < D; ++a) {
+                if ((frozen >> a) & 1) continue;
+                double *W0 = tabs + LDS_TABLES_PER_AXIS * a * n;
+                build_axis_tables<D>(sp, a, n, sx, sxi, sdx, W0, W0 + n, W0 + 2 * n, tid, nthr);
+            }
+            if (tid == 0) s_unsafe = 0;
+            __syncthreads();
+            const double dt = sp->dt, inv_dt = 1. / dt;
+            bool safe = true;
+            for (int a = 0; a < D; ++a) {
+                if ((frozen >> a) & 1) continue;
+                const double *W0 = tabs + LDS_TABLES_PER_AXIS * a * n;
+                fz_factor_axis<D, ORDER>(a, n, L, sp, sx, sxi, sdf, sdx, W0, W0 + n, W0 + 2 * n,
+                                         dt, inv_dt, F + 4LL * a * N, wave, nw, lane, safe);
+            }
+            if (!safe) s_unsafe = 1;
+            __syncthreads();
+            const bool mk = s_unsafe == 0;
+            for (long long k = 0; k < nst; ++k) {
+                bool first = true;
+                for (int a = 0; a < D; ++a) {
+                    if ((frozen >> a) & 1) continue;
+                    if (mk)
+                        fz_sweep_axis<D, ORDER, true>(phi, n, ld, L, a, first, injm, sp, sx, sdx,
+                                                      dt, inv_dt, F + 4LL * a * N, U, wave, nw,
+                                                      lane);
+                    else
+                        fz_sweep_axis<D, ORDER, false>(phi, n, ld, L, a, first, injm, sp, sx,
+                                                       sdx, dt, inv_dt, F + 4LL * a * N, U, wave,
+                                                       nw, lane);
+                    __syncthreads();
+                    first = false;
+                }
+                if (max_ticks > 0) {
+                    if (tid == 0) s_stop = (long long)wall_clock64() - t0 >= max_ticks;
+                    __syncthreads();
+                    if (s_stop) {
+                        nst = k + 1;
+                        out_of_time = true;
+                        break;
+                    }
+                }
+            }
+            budget -= nst;
+            step += nst;
+            if (step >= sp->nsteps) {
+                ++seg;
+                step = 0;
+            }
+        }
+        if (tid == 0) {
+            ip->cur_seg = seg;
+            ip->cur_step = step;
+            ip->status = seg >= nseg ? 1 : 0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Kernel 3: one axis of one step of one instance over many workgroups.
 // ---------------------------------------------------------------------------
 template <int D, int ORDER>
@@ -2117,6 +2434,22 @@ const void *pick_sweep(int D, int order)
     switch (D * 4 + order) {
 #define DH_CASE(d, o) case d * 4 + o: return sw_kernel<d, o>();
         DH_CASE(1, 0) DH_CASE(1, 1) DH_CASE(1, 2)
+        DH_CASE(2, 0) DH_CASE(2, 1) DH_CASE(2, 2)
+        DH_CASE(3, 0) DH_CASE(3, 1) DH_CASE(3, 2)
+        DH_CASE(4, 0) DH_CASE(4, 1) DH_CASE(4, 2)
+        DH_CASE(5, 0) DH_CASE(5, 1) DH_CASE(5, 2)
+#undef DH_CASE
+    }
+    return nullptr;
+}
+
+template <int D, int ORDER>
+const void *fz_kernel() { return reinterpret_cast<const void *>(&k_integrate_fz<D, ORDER>); }
+
+const void *pick_fz(int D, int order)
+{
+    switch (D * 4 + order) {
+#define DH_CASE(d, o) case d * 4 + o: return fz_kernel<d, o>();
         DH_CASE(2, 0) DH_CASE(2, 1) DH_CASE(2, 2)
         DH_CASE(3, 0) DH_CASE(3, 1) DH_CASE(3, 2)
         DH_CASE(4, 0) DH_CASE(4, 1) DH_CASE(4, 2)
@@ -2357,6 +2690,43 @@ int dh_integrate_1d(dh_instance *table_dev, int ninst, int order, int nmax, int 
     void *args[] = {&table_dev, &ninst, &counter_dev, &scratch_dev, (void *)&nlanes, &nmax, &ms};
     DH_HIP(hipLaunchKernel(pick_1d(order, cfg->delj), dim3((unsigned)groups), dim3(thr), args, 0,
                            st));
+    return DH_OK;
+}
+
+long long dh_factored_slice_doubles(int D, int nmax)
+{
+    if (D < 2 || D > DH_MAXD || nmax < 3) return 0;
+    return fz_slice(D, nmax);
+}
+
+int dh_integrate_factored(dh_instance *table_dev, int ninst, int D, int order, int nmax,
+                          int *counter_dev, const dh_launch *cfg, void *stream)
+{
+    if (int e = check_init()) return e;
+    if (D < 2 || D > DH_MAXD) return fail(DH_EINVAL, "D must be 2..5");
+    if (order < 0 || order > 2) return fail(DH_EINVAL, "bad order");
+    if (ninst <= 0) return DH_OK;
+    if (!table_dev || !counter_dev || !cfg) return fail(DH_EINVAL, "null pointer");
+    if (nmax < 3) return fail(DH_EINVAL, "n must be >= 3");
+    if (cfg->delj) return fail(DH_EINVAL, "the delta trick is not available on this path");
+    const int waves = cfg->waves;
+    if (waves < 1 || waves > 8) return fail(DH_EINVAL, "waves must be 1..8");
+    long long groups = cfg->groups > 0 ? cfg->groups : std::min<long long>(ninst, g_num_cu);
+    const long long slice = fz_slice(D, nmax);
+    if (!cfg->gamma_dev || cfg->gamma_doubles < groups * slice)
+        return fail(DH_ERANGE, "factorisation workspace too small: need " +
+                                   std::to_string(groups * slice) + " doubles");
+    const void *fn = pick_fz(D, order);
+    const size_t lds =
+        sizeof(double) * (size_t)nmax * (LDS_GRID_TABLES + LDS_TABLES_PER_AXIS * (size_t)D);
+    if (int e = set_lds(fn, lds)) return e;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, counter_dev, 0);
+    long long ms = cfg->max_steps, ticks = budget_ticks(cfg);
+    double *ws = cfg->gamma_dev;
+    long long sl = slice;
+    void *args[] = {&table_dev, &ninst, &counter_dev, &ms, &ws, &sl, &ticks};
+    DH_HIP(hipLaunchKernel(fn, dim3((unsigned)groups), dim3(64 * waves), args, lds, st));
     return DH_OK;
 }
 

@@ -59,6 +59,36 @@ class Context:
             return self.one_d_kernel == "wave"
         return ninst <= self.one_d_wave_max
 
+    def factored_launch(self, tab_ptr, ninst, D, order, nmax, counter_ptr, stream, workspace,
+                        waves=None, groups=0, max_steps=0, max_us=0):
+        """One dh_integrate_factored call (latency path for D >= 2: the Thomas
+        factorisation of every line kept per segment).  ``workspace`` (a
+        dict owned by the caller, one per stream) keeps the factorisation
+        workspace between calls."""
+        tiles = -(-(nmax ** (D - 1)) // 64)
+        w = waves or max(1, min(8, tiles))
+        g = groups if groups > 0 else min(ninst, self.num_cu)
+        need = g * int(self.lib.dh_factored_slice_doubles(D, nmax))
+        ws = workspace.get("factored")
+        if ws is None or ws.numel() < need:
+            ws = torch().empty(need, dtype=torch().float64, device=self.device)
+            workspace["factored"] = ws
+        cfg = DhLaunch(w, g, max_steps, 0, 0)
+        cfg.gamma_dev = ws.data_ptr()
+        cfg.gamma_doubles = ws.numel()
+        cfg.max_us = int(max_us)
+        check(self.lib.dh_integrate_factored(tab_ptr, ninst, D, order, nmax, counter_ptr, cfg,
+                                             stream), "dh_integrate_factored")
+
+    @staticmethod
+    def factorable(segs, min_steps=4):
+        """Whether schedules have segments long enough for the factorisation
+        (formed once per segment) to pay: at least ``min_steps`` steps per
+        segment on average."""
+        nst = sum(int(np.sum(sg["nsteps"])) for sg in segs)
+        nseg = sum(len(sg) for sg in segs)
+        return nseg > 0 and nst >= min_steps * nseg
+
     # -- helpers ---------------------------------------------------------
     def stream(self):
         return torch().cuda.current_stream(self.device).cuda_stream
@@ -389,6 +419,15 @@ def run_single(phi, xx, seg, order, frozen, inject, ctx=None, mode="auto", waves
         check(ctx.lib.dh_integrate_nd(tab.data_ptr(), 1, D, order, n, counter.data_ptr(),
                                       cfg, stream), "dh_integrate_nd")
         keep = (tab, counter, segd)
+    elif mode == "factored":
+        rec = _inst_record(phid.data_ptr(), xxd.data_ptr(), segd.data_ptr(), len(seg), D, n,
+                           order, frozen, inject, ld)
+        tab = ctx.to_dev(rec.view(np.uint8))
+        counter = t.zeros(1, dtype=t.int32, device=ctx.device)
+        ws = {}
+        ctx.factored_launch(tab.data_ptr(), 1, D, order, n, counter.data_ptr(), stream, ws,
+                            waves=waves)
+        keep = (tab, counter, segd, ws)
     elif mode == "graph":
         nst = int(np.sum(seg["nsteps"]))
         seg_of_step = np.repeat(np.arange(len(seg), dtype=np.int32), seg["nsteps"])

@@ -153,6 +153,7 @@ class Request:
     t_done: float = None
     footprint: int = 0
     wide: object = None
+    seg: object = None
 
 
 class Server:
@@ -164,7 +165,8 @@ class Server:
     """
 
     def __init__(self, budget_bytes=8 << 30, quantum=200, ctx=None, quantum_ms=8.0,
-                 wide_lines=512, wide_streams=2, steps_per_graph=16, wide_waves=2):
+                 wide_lines=512, wide_streams=2, steps_per_graph=16, wide_waves=2,
+                 factored=True):
         import torch
         self.torch = torch
         self.ctx = ctx or engine.context()
@@ -175,6 +177,7 @@ class Server:
         self.wide_streams = max(1, int(wide_streams))
         self.spg = max(1, int(steps_per_graph))
         self.wide_waves = int(wide_waves)
+        self.factored = bool(factored)
         self.steps = {}       # class -> time steps per launch (adapted)
         self.streams = {}
         self.scratch = {}
@@ -252,6 +255,7 @@ class Server:
                 r.rec = engine._inst_record(r.phi.data_ptr(), xxd.data_ptr(), r.segd.data_ptr(),
                                             len(ep.seg), ep.D, r.prog.pts, ep.order, ep.frozen,
                                             ep.inject, engine.pitch_of(r.phi))
+                r.seg = ep.seg
                 return "epoch"
             if r.rec["status"][0] == 1:
                 r.rec = None
@@ -339,6 +343,10 @@ class Server:
                 _lib.check(ctx.lib.dh_integrate_1d(tabd.data_ptr(), len(group), int(order), nmax,
                                                    counter.data_ptr(), scr.data_ptr(), scr.numel(),
                                                    cfg, h), "dh_integrate_1d")
+            elif self.factored and ctx.factorable([r.seg for r in group]):
+                ws = self.scratch.setdefault(key, {})
+                ctx.factored_launch(tabd.data_ptr(), len(group), int(D), int(order), nmax,
+                                    counter.data_ptr(), h, ws, max_us=budget_us)
             else:
                 ws = self.scratch.setdefault(key, {})
                 ctx.nd_launch(tabd.data_ptr(), len(group), int(D), int(order), nmax,

@@ -198,6 +198,26 @@ int dh_integrate_1d(dh_instance *table_dev, int ninst, int order, int nmax,
  */
 int dh_integrate_1d_wave(dh_instance *table_dev, int ninst, int order, int nmax,
                          int *counter_dev, const dh_launch *cfg, void *stream);
+/*
+ * Runs ninst instances of 2-5 populations, one workgroup per instance at a
+ * time (latency path).  Within a segment every step has the same tridiagonal
+ * matrices, so the Thomas factorisation of every line of every axis (a_j,
+ * bet_j, 1/bet_j, gamma_j; dadi/tridiag.c:22-29 with the coefficients of
+ * dadi/integration_shared.c:72-93) is formed once per segment with the
+ * per-step operations into a workspace, and a step only solves the right-hand
+ * sides: a few dependent operations per row instead of the coefficient build
+ * and three divisions.  Results are the bits of dh_integrate_nd.  More memory
+ * traffic per point than dh_integrate_nd, for few instances at a time (a
+ * single integration, a serving loop).  cfg->gamma_dev / gamma_doubles is the
+ * workspace: groups * dh_factored_slice_doubles(D, nmax) doubles, groups =
+ * cfg->groups if > 0, else min(ninst, compute units).  cfg->waves 1..8;
+ * cfg->max_steps and cfg->max_us as for dh_integrate_nd; no delta trick.
+ * Replaces the same reference code as dh_integrate_nd.
+ */
+int dh_integrate_factored(dh_instance *table_dev, int ninst, int D, int order, int nmax,
+                          int *counter_dev, const dh_launch *cfg, void *stream);
+/* Workspace doubles one workgroup of dh_integrate_factored needs (0 if D or nmax is out of range). */
+long long dh_factored_slice_doubles(int D, int nmax);
 /* LDS per workgroup of dh_integrate_1d_wave, bytes. */
 size_t dh_1d_wave_lds_bytes(int nmax);
 

@@ -458,3 +458,64 @@ def test_time_budget_quanta_bit_exact(dadi_amd, oracle, D, n):
     assert launches > 2, launches
     for p in phis:
         assert np.array_equal(p.cpu().numpy(), want)
+
+
+@pytest.mark.parametrize("D,n,order,frozen0", [(2, 30, "pre", False), (2, 23, "tv", False),
+                                               (2, 40, "pre", True), (3, 13, "pre", False),
+                                               (3, 11, "tv", False), (4, 8, "tv", False),
+                                               (5, 6, "tv", False)])
+def test_factored_kernel_matches_oracle(dadi_amd, oracle, D, n, order, frozen0):
+    """The latency kernel for 2-5 populations (Thomas factorisation formed
+    once per segment, divisions through known reciprocals with a per-line
+    fallback) gives the oracle's bits: constant and time-varying schedules,
+    strong selection (densities that underflow: the fallback rows), a frozen
+    population, several instances per workgroup in turn, several launches
+    (max_steps quanta)."""
+    from dadi_amd import engine, schedule as S, _lib
+    import torch
+    ctx = engine.context()
+    xx = oracle.default_grid(n)
+    p = oracle.phi_1D(xx, gamma=-400.0)   # subnormal and zero tails
+    for d in range(2, D + 1):
+        p = oracle.phi_1D_to_2D(xx, p) if d == 2 else \
+            oracle.phi_2D_to_3D_split_2(xx, p) if d == 3 else \
+            oracle.phi_3D_to_4D(p, 0, 0, xx, xx, xx, xx) if d == 4 else \
+            oracle.phi_4D_to_5D(p, 0, 0, 0, xx, xx, xx, xx, xx)
+    T = 0.03
+    nus = [0.8, 1.7, 0.6, 1.2, 2.0][:D]
+    migs = [[0.9 + 0.1 * k] * (D - 1) for k in range(D)]
+    gams = [-400.0, 2.0, -5.0, 0.0, 1.0][:D]
+    hs = [0.3, 0.5, 0.7, 0.5, 0.2][:D]
+    frozen = [frozen0] + [False] * (D - 1)
+    if frozen0:
+        migs[0] = [0.0] * (D - 1)
+    fz, inj = S.masks(D, frozen, [False] * D)
+    if order == "pre":
+        dt = min(S.compute_dt(nus[k], migs[k], gams[k], hs[k]) for k in range(D))
+        seg = S.const_segments(D, xx, 0.0, T, dt, nus, migs, gams, hs, 1.0)
+        ordc = _lib.ORDER_PRE if D <= 3 else _lib.ORDER_TV
+        want = oracle.integrate_const(p, xx, T, nus, migs, gams, hs, frozen=frozen)
+    else:
+        c = lambda v: (lambda t: v)
+        nu0 = lambda t: 0.8 + 10 * t
+        seg = S.tv_segments(D, xx, 0.0, T, [nu0] + [c(v) for v in nus[1:]],
+                            [[c(m) for m in ms] for ms in migs], [c(g) for g in gams],
+                            [c(h) for h in hs], c(1.0), None)
+        ordc = _lib.ORDER_TV
+        want = oracle.integrate_tv(p, xx, T, [nu0] + nus[1:], migs, gams, hs, frozen=frozen)
+    phis = [ctx.to_dev(p) for _ in range(3)]
+    segd = ctx.to_dev(seg.view(np.uint8))
+    xxd = ctx.grid(xx)
+    tab = ctx.to_dev(np.concatenate([engine._inst_record(
+        q.data_ptr(), xxd.data_ptr(), segd.data_ptr(), len(seg), D, n, ordc, fz, inj,
+        engine.pitch_of(q)) for q in phis]).view(np.uint8))
+    counter = torch.zeros(1, dtype=torch.int32, device=ctx.device)
+    ws = {}
+    for _ in range(10000):
+        ctx.factored_launch(tab.data_ptr(), 3, D, ordc, n, counter.data_ptr(), ctx.stream(), ws,
+                            groups=2, max_steps=7)
+        if np.all(tab.cpu().numpy().view(_lib.INST_DTYPE)["status"] == 1):
+            break
+    for q in phis:
+        got = q.cpu().numpy()
+        assert np.array_equal(got, want), np.max(np.abs(got - want))
