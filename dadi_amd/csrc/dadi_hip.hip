@@ -2040,18 +2040,88 @@ __device__ __forceinline__ double fz_rhs(double p, double dt, double inv_dt)
     return (ORDER == DH_ORDER_PRE) ? inv_dt * p : p / dt;
 }
 
+// Operands of FZ_PF rows of one line: phi, a, bet, 1/bet.  Rows past the
+// end of the line repeat its last row (only the look-ahead reads them).
+struct FzRows {
+    double p[FZ_PF], a[FZ_PF], b[FZ_PF], y[FZ_PF];
+};
+
+__device__ __forceinline__ void fz_load(FzRows &R, const double *p, long long s,
+                                        const double *A, const double *Bt, const double *Yr,
+                                        long long L, int j0, int n)
+{
+#pragma unroll
+    for (int q = 0; q < FZ_PF; ++q) {
+        const int jj = j0 + q < n ? j0 + q : n - 1;
+        const long long o = (long long)jj * L;
+        R.p[q] = p[jj * s];
+        R.a[q] = A[o];
+        R.b[q] = Bt[o];
+        R.y[q] = Yr[o];
+    }
+}
+
+template <int ORDER, bool MK>
+__device__ __forceinline__ double fz_rows(const FzRows &R, double u, double *U, long long L,
+                                          int j0, int n, double dt, double inv_dt, int &bad)
+{
+#pragma unroll
+    for (int q = 0; q < FZ_PF; ++q) {
+        const int j = j0 + q;
+        if (j < n) {
+            const double num = fz_rhs<ORDER>(R.p[q], dt, inv_dt) - R.a[q] * u;
+            if constexpr (MK) {
+                const double q0 = num * R.y[q];
+                u = __builtin_fma(__builtin_fma(-R.b[q], q0, num), R.y[q], q0);
+                if (bad == n && !rcp_form_exact(num)) bad = j;
+            } else {
+                u = num / R.b[q];
+            }
+            U[(long long)j * L] = u;
+        }
+    }
+    return u;
+}
+
+__device__ __forceinline__ void fz_back_load(double (&cu)[FZ_PF], double (&cg)[FZ_PF],
+                                             const double *U, const double *G, long long L,
+                                             int j0)
+{
+#pragma unroll
+    for (int q = 0; q < FZ_PF; ++q) {
+        const int jj = j0 - q >= 0 ? j0 - q : 0;
+        cu[q] = U[(long long)jj * L];
+        cg[q] = G[(long long)(jj + 1) * L];
+    }
+}
+
+__device__ __forceinline__ double fz_back_rows(const double (&cu)[FZ_PF],
+                                               const double (&cg)[FZ_PF], double u, double *p,
+                                               long long s, int j0)
+{
+#pragma unroll
+    for (int q = 0; q < FZ_PF; ++q) {
+        const int j = j0 - q;
+        if (j >= 0) {
+            u = cu[q] - cg[q] * u;
+            p[j * s] = u;
+        }
+    }
+    return u;
+}
+
 // One step's sweep of axis `a` from the factorisation.  MK: divisions through
 // the known reciprocals (each line redone with divisions from its first row
-// outside the exact range).
+// outside the exact range).  Rows go in blocks of FZ_PF from two register
+// blocks used in turn, one loading while the other is solved.
 template <int D, int ORDER, bool MK>
 __device__ void fz_sweep_axis(double *__restrict__ phi, int n, long long ld, long long L, int a,
                               bool first_axis, int injm, const dh_segment *sp, const double *sx,
                               const double *sdx, double dt, double inv_dt,
-                              const double *__restrict__ Fa, double *__restrict__ U,
+                              const double *__restrict__ Fa, double *__restrict__ Uw,
                               long long tile0, long long tstep, int lane)
 {
     const long long N = L * n;
-    const double *A = Fa, *Bt = Fa + N, *Yr = Fa + 2 * N, *G = Fa + 3 * N;
     double mig[D - 1];
 #pragma unroll
     for (int q = 0; q < D - 1; ++q) mig[q] = sp->mig[a][q];
@@ -2059,6 +2129,9 @@ __device__ void fz_sweep_axis(double *__restrict__ phi, int n, long long ld, lon
     for (long long t = tile0; t * 64 < L; t += tstep) {
         const long long line = t * 64 + lane;
         if (line >= L) continue;
+        const double *A = Fa + line, *Bt = Fa + N + line, *Yr = Fa + 2 * N + line;
+        const double *G = Fa + 3 * N + line;
+        double *U = Uw + line;
         LineSetup<D, ORDER> Ls;
         line_setup<D, ORDER>(Ls, line, n, a, sx, mig, sp->nu[a], sp->gamma[a], sp->h[a], sdx[0],
                              sdx[n - 2], first_axis, injm, sp->chrom);
@@ -2073,63 +2146,29 @@ __device__ void fz_sweep_axis(double *__restrict__ phi, int n, long long ld, lon
         {
             const double r0 = fz_rhs<ORDER>(p[0], dt, inv_dt);
             if constexpr (MK) {
-                const double q0 = r0 * Yr[line];
-                u = __builtin_fma(__builtin_fma(-Bt[line], q0, r0), Yr[line], q0);
+                const double q0 = r0 * Yr[0];
+                u = __builtin_fma(__builtin_fma(-Bt[0], q0, r0), Yr[0], q0);
                 if (!rcp_form_exact(r0)) bad = 0;
             } else {
-                u = r0 / Bt[line];
+                u = r0 / Bt[0];
             }
-            U[line] = u;
+            U[0] = u;
         }
-        double np_[FZ_PF], na_[FZ_PF], nb[FZ_PF], ny[FZ_PF];
-#pragma unroll
-        for (int q = 0; q < FZ_PF; ++q) {
-            const int jj = 1 + q < n ? 1 + q : n - 1;
-            const long long o = (long long)jj * L + line;
-            np_[q] = p[jj * s];
-            na_[q] = A[o];
-            nb[q] = Bt[o];
-            ny[q] = Yr[o];
+        FzRows X, Z;
+        fz_load(X, p, s, A, Bt, Yr, L, 1, n);
+        int j0 = 1;
+        for (; j0 + FZ_PF < n; j0 += 2 * FZ_PF) {
+            fz_load(Z, p, s, A, Bt, Yr, L, j0 + FZ_PF, n);
+            u = fz_rows<ORDER, MK>(X, u, U, L, j0, n, dt, inv_dt, bad);
+            if (j0 + 2 * FZ_PF < n) fz_load(X, p, s, A, Bt, Yr, L, j0 + 2 * FZ_PF, n);
+            u = fz_rows<ORDER, MK>(Z, u, U, L, j0 + FZ_PF, n, dt, inv_dt, bad);
         }
-        for (int j0 = 1; j0 < n; j0 += FZ_PF) {
-            double cp[FZ_PF], ca[FZ_PF], cb[FZ_PF], cy[FZ_PF];
-#pragma unroll
-            for (int q = 0; q < FZ_PF; ++q) {
-                cp[q] = np_[q];
-                ca[q] = na_[q];
-                cb[q] = nb[q];
-                cy[q] = ny[q];
-            }
-#pragma unroll
-            for (int q = 0; q < FZ_PF; ++q) {
-                const int jj = j0 + FZ_PF + q < n ? j0 + FZ_PF + q : n - 1;
-                const long long o = (long long)jj * L + line;
-                np_[q] = p[jj * s];
-                na_[q] = A[o];
-                nb[q] = Bt[o];
-                ny[q] = Yr[o];
-            }
-#pragma unroll
-            for (int q = 0; q < FZ_PF; ++q) {
-                const int j = j0 + q;
-                if (j < n) {
-                    const double num = fz_rhs<ORDER>(cp[q], dt, inv_dt) - ca[q] * u;
-                    if constexpr (MK) {
-                        const double q0 = num * cy[q];
-                        u = __builtin_fma(__builtin_fma(-cb[q], q0, num), cy[q], q0);
-                        if (bad == n && !rcp_form_exact(num)) bad = j;
-                    } else {
-                        u = num / cb[q];
-                    }
-                    U[(long long)j * L + line] = u;
-                }
-            }
-        }
+        if (j0 < n) u = fz_rows<ORDER, MK>(X, u, U, L, j0, n, dt, inv_dt, bad);
         if (MK && bad < n) {
             // rows from the first one outside the exact range, with divisions
-            u = bad == 0 ? 0. : U[(long long)(bad - 1) * L + line];
+            u = bad == 0 ? 0. : U[(long long)(bad - 1) * L];
             for (int j = bad; j < n; ++j) {
-                const long long o = (long long)j * L + line;
+                const long long o = (long long)j * L;
                 const double r = fz_rhs<ORDER>(p[j * s], dt, inv_dt);
                 u = (j == 0 ? r : r - A[o] * u) / Bt[o];
                 U[o] = u;
@@ -2137,37 +2176,18 @@ __device__ void fz_sweep_axis(double *__restrict__ phi, int n, long long ld, lon
         }
 
         // back substitution (dadi/tridiag.c:31-33), u' from U, result into phi
-        u = U[(long long)(n - 1) * L + line];
+        u = U[(long long)(n - 1) * L];
         p[(long long)(n - 1) * s] = u;
-        double nu_[FZ_PF], ng[FZ_PF];
-#pragma unroll
-        for (int q = 0; q < FZ_PF; ++q) {
-            const int jj = n - 2 - q >= 0 ? n - 2 - q : 0;
-            nu_[q] = U[(long long)jj * L + line];
-            ng[q] = G[(long long)(jj + 1) * L + line];
+        double xu[FZ_PF], xg[FZ_PF], zu[FZ_PF], zg[FZ_PF];
+        fz_back_load(xu, xg, U, G, L, n - 2);
+        int k0 = n - 2;
+        for (; k0 - FZ_PF >= 0; k0 -= 2 * FZ_PF) {
+            fz_back_load(zu, zg, U, G, L, k0 - FZ_PF);
+            u = fz_back_rows(xu, xg, u, p, s, k0);
+            if (k0 - 2 * FZ_PF >= 0) fz_back_load(xu, xg, U, G, L, k0 - 2 * FZ_PF);
+            u = fz_back_rows(zu, zg, u, p, s, k0 - FZ_PF);
         }
-        for (int j0 = n - 2; j0 >= 0; j0 -= FZ_PF) {
-            double cu[FZ_PF], cg[FZ_PF];
-#pragma unroll
-            for (int q = 0; q < FZ_PF; ++q) {
-                cu[q] = nu_[q];
-                cg[q] = ng[q];
-            }
-#pragma unroll
-            for (int q = 0; q < FZ_PF; ++q) {
-                const int jj = j0 - FZ_PF - q >= 0 ? j0 - FZ_PF - q : 0;
-                nu_[q] = U[(long long)jj * L + line];
-                ng[q] = G[(long long)(jj + 1) * L + line];
-            }
-#pragma unroll
-            for (int q = 0; q < FZ_PF; ++q) {
-                const int j = j0 - q;
-                if (j >= 0) {
-                    u = cu[q] - cg[q] * u;
-                    p[j * s] = u;
-                }
-            }
-        }
+        if (k0 >= 0) u = fz_back_rows(xu, xg, u, p, s, k0);
     }
 }
 

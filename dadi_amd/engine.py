@@ -49,6 +49,9 @@ class Context:
     #: enough for the GPU to hold one wave per integration), "wave" or "lane"
     one_d_kernel = "auto"
     one_d_wave_max = 16384
+    #: 2D batches of at most this many integrations run on the factored
+    #: (latency) kernel when their segments are long; 0 disables it
+    factored_max_inst = 512
 
     def one_d_wave(self, ninst, nmax):
         """Whether a 1D batch runs one wave per integration (LDS-resident,
@@ -404,6 +407,9 @@ def run_single(phi, xx, seg, order, frozen, inject, ctx=None, mode="auto", waves
     if mode == "auto":
         wmax = max(ctx.max_waves(D, n, 0), ctx.max_waves(D, n, 8))
         mode = "wg" if (D == 1 or nsys <= 64 * max(wmax, 1)) and wmax >= 1 else "graph"
+        if mode == "wg" and D == 2 and not delj and ctx.factored_max_inst > 0 and \
+                ctx.factorable([seg]):
+            mode = "factored"
     stream = ctx.stream()
     if mode == "wg":
         # one instance: the full gamma column (less VALU) when it fits
@@ -549,11 +555,29 @@ class PackedBatch:
             check(ctx.lib.dh_integrate_1d(self.tab.data_ptr(), len(self.jobs), self.order,
                                           self.nmax, self.counter.data_ptr(), scr.data_ptr(),
                                           scr.numel(), cfg, stream), "dh_integrate_1d")
+        elif self.latency_bound(waves, chunk, recompute, gamma_global, slots):
+            ctx.factored_launch(self.tab.data_ptr(), len(self.jobs), self.D, self.order,
+                                self.nmax, self.counter.data_ptr(), stream, self.workspace,
+                                waves, groups, max_steps)
         else:
             return ctx.nd_launch(self.tab.data_ptr(), len(self.jobs), self.D, self.order,
                                  self.nmax, self.counter.data_ptr(), stream, self.workspace,
                                  waves, groups, max_steps, chunk, recompute, gamma_global,
                                  slots, self.packable, self.delj)
+
+    def latency_bound(self, waves=None, chunk=None, recompute=None, gamma_global=None,
+                      slots=None):
+        """Whether this batch goes to the factored kernel: two populations,
+        too few integrations to fill the GPU (each one's step latency, not
+        the batch's bandwidth, sets the time), long segments, and no launch
+        shape asked for explicitly."""
+        ctx = self.ctx
+        if self.D != 2 or self.delj or ctx.factored_max_inst <= 0:
+            return False
+        if any(v is not None for v in (waves, chunk, recompute, gamma_global, slots)):
+            return False
+        return (len(self.jobs) <= ctx.factored_max_inst and
+                ctx.factorable([j.seg for j in self.jobs]))
 
     def progress(self):
         raw = self.tab.cpu().numpy().view(INST_DTYPE)
