@@ -37,6 +37,12 @@ INST_DTYPE = np.dtype([
     ("inject", "<i4"), ("status", "<i4"), ("ld", "<i4"), ("reserved", "<i4"),
 ])
 
+#: numpy mirror of ``dh_sweep_item``
+ITEM_DTYPE = np.dtype([
+    ("inst", INST_DTYPE), ("seg_of_step", "<u8"), ("total_steps", "<i8"),
+    ("first_axis", "<i4"), ("wg_begin", "<i4"), ("wgs", "<i4"), ("reserved", "<i4"),
+])
+
 
 class BackendUnavailable(RuntimeError):
     """The HIP library or a GPU is not available."""
@@ -83,6 +89,11 @@ _SIGS = {
     "dh_integrate_1d_wave": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp,
                               ctypes.POINTER(DhLaunch), _vp], ctypes.c_int),
     "dh_1d_wave_lds_bytes": ([ctypes.c_int], ctypes.c_size_t),
+    "dh_integrate_batch": ([_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                            ctypes.c_int, ctypes.c_int, _vp, ctypes.c_longlong,
+                            ctypes.c_longlong, ctypes.c_longlong, ctypes.c_int, _vp],
+                           ctypes.c_int),
+    "dh_sizeof_sweep_item": ([], ctypes.c_size_t),
     "dh_integrate_factored": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp,
                                ctypes.POINTER(DhLaunch), _vp], ctypes.c_int),
     "dh_factored_slice_doubles": ([ctypes.c_int, ctypes.c_int], ctypes.c_longlong),
@@ -133,7 +144,8 @@ def load(path=LIB_PATH):
         if L.dh_abi_version() != ABI_VERSION:
             raise BackendUnavailable("libdadi_hip.so ABI version mismatch")
         if L.dh_sizeof_segment() != SEG_DTYPE.itemsize or \
-                L.dh_sizeof_instance() != INST_DTYPE.itemsize:
+                L.dh_sizeof_instance() != INST_DTYPE.itemsize or \
+                L.dh_sizeof_sweep_item() != ITEM_DTYPE.itemsize:
             raise BackendUnavailable("libdadi_hip.so struct layout mismatch")
         _lib = L
         return L

@@ -2320,6 +2320,44 @@ __global__ void __launch_bounds__(256) k_sweep_axis(dh_instance inst, int a, int
                           (long long)gridDim.x * nw, lane);
 }
 
+// ---------------------------------------------------------------------------
+// Kernel 3b: one axis of one step of a batch of integrations, each one's
+// lines spread over several workgroups (checkpointed Thomas).
+// ---------------------------------------------------------------------------
+template <int D, int ORDER>
+__global__ void __launch_bounds__(256) k_sweep_batch(const dh_sweep_item *__restrict__ items,
+                                                     const int *__restrict__ wg_item, int a,
+                                                     const long long *__restrict__ base,
+                                                     long long off)
+{
+    const dh_sweep_item &it = items[wg_item[blockIdx.x]];
+    const long long st = *base + off;
+    if (st >= it.total_steps) return;
+    const dh_instance &I = it.inst;
+    if ((I.frozen >> a) & 1) return;
+    extern __shared__ double lds[];
+    const int n = I.n;
+    const int nthr = blockDim.x, tid = threadIdx.x;
+    const int nw = nthr >> 6, wave = tid >> 6, lane = tid & 63;
+    constexpr int CK = 8;
+    const int col = (n - 1) / CK + 1;
+    const dh_segment *sp = gbl(I.seg_dev) + gbl(it.seg_of_step_dev)[st];
+    double *sx = lds, *sxi = sx + n, *sdf = sxi + n, *sdx = sdf + n;
+    double *W0 = sdx + n, *W1 = W0 + n, *S = W1 + n;
+    double *gamcol = S + n + (long long)wave * 64 * col + lane;
+    double *stage = S + n + (long long)nw * 64 * col + (long long)wave * 64 * (STAGE_C + 1);
+    build_grid_tables(gbl(I.xx_dev), n, sx, sxi, sdf, sdx, tid, nthr);
+    __syncthreads();
+    build_axis_tables<D>(sp, a, n, sx, sxi, sdx, W0, W1, S, tid, nthr);
+    __syncthreads();
+    const double dt = sp->dt, inv_dt = 1. / dt;
+    sweep_tiles<D, ORDER, CK, 0, 0>(gbl(I.phi_dev), n, I.ld > 0 ? I.ld : n, a,
+                                    a == it.first_axis, I.inject, sp, sx, sxi, sdf, sdx, W0, W1,
+                                    S, dt, inv_dt, gamcol, stage,
+                                    (long long)(blockIdx.x - it.wg_begin) * nw + wave,
+                                    (long long)it.wgs * nw, lane);
+}
+
 __global__ void k_advance(long long *base, long long by)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) *base += by;
@@ -2470,6 +2508,22 @@ const void *pick_fz(int D, int order)
 {
     switch (D * 4 + order) {
 #define DH_CASE(d, o) case d * 4 + o: return fz_kernel<d, o>();
+        DH_CASE(2, 0) DH_CASE(2, 1) DH_CASE(2, 2)
+        DH_CASE(3, 0) DH_CASE(3, 1) DH_CASE(3, 2)
+        DH_CASE(4, 0) DH_CASE(4, 1) DH_CASE(4, 2)
+        DH_CASE(5, 0) DH_CASE(5, 1) DH_CASE(5, 2)
+#undef DH_CASE
+    }
+    return nullptr;
+}
+
+template <int D, int ORDER>
+const void *sb_kernel() { return reinterpret_cast<const void *>(&k_sweep_batch<D, ORDER>); }
+
+const void *pick_sweep_batch(int D, int order)
+{
+    switch (D * 4 + order) {
+#define DH_CASE(d, o) case d * 4 + o: return sb_kernel<d, o>();
         DH_CASE(2, 0) DH_CASE(2, 1) DH_CASE(2, 2)
         DH_CASE(3, 0) DH_CASE(3, 1) DH_CASE(3, 2)
         DH_CASE(4, 0) DH_CASE(4, 1) DH_CASE(4, 2)
@@ -2887,6 +2941,90 @@ int dh_integrate_graph_range(const dh_instance *inst_host, long long *step_base_
     }
     hipLaunchKernelGGL(k_set_ll, dim3(1), dim3(1), 0, st, step_base_dev, first_step);
     const long long last = first_step + nsteps < total_steps ? first_step + nsteps : total_steps;
+    const long long reps = (last - first_step + nsteps_per_graph - 1) / nsteps_per_graph;
+    for (long long r = 0; r < reps; ++r) DH_HIP(hipGraphLaunch(exec, st));
+    return DH_OK;
+}
+
+size_t dh_sizeof_sweep_item(void) { return sizeof(dh_sweep_item); }
+
+int dh_integrate_batch(const dh_sweep_item *items_dev, const int *wg_item_dev, int ninst,
+                       int total_wgs, int D, int order, int nmax, int waves,
+                       long long *step_base_dev, long long max_total_steps,
+                       long long first_step, long long nsteps, int nsteps_per_graph,
+                       void *stream)
+{
+    if (int e = check_init()) return e;
+    if (!items_dev || !wg_item_dev || !step_base_dev) return fail(DH_EINVAL, "null pointer");
+    if (D < 2 || D > DH_MAXD || order < 0 || order > 2) return fail(DH_EINVAL, "bad D/order");
+    if (ninst <= 0 || total_wgs <= 0 || max_total_steps <= 0 || nsteps <= 0 ||
+        first_step >= max_total_steps)
+        return DH_OK;
+    if (nmax < 3) return fail(DH_EINVAL, "n must be >= 3");
+    if (waves < 1 || waves > 4) return fail(DH_EINVAL, "waves must be 1..4");
+    if (nsteps_per_graph < 1) return fail(DH_EINVAL, "nsteps_per_graph must be >= 1");
+    if (first_step < 0) return fail(DH_EINVAL, "negative first step");
+    if (nsteps % nsteps_per_graph != 0 && first_step + nsteps < max_total_steps)
+        return fail(DH_EINVAL, "nsteps must be a multiple of nsteps_per_graph or reach the end");
+    const void *fn = pick_sweep_batch(D, order);
+    const size_t col = (size_t)((nmax - 1) / 8 + 1);
+    const size_t lds = sizeof(double) * ((size_t)nmax * (LDS_GRID_TABLES + LDS_TABLES_PER_AXIS) +
+                                         (size_t)waves * 64 * (col + STAGE_C + 1));
+    if (int e = set_lds(fn, lds)) return e;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    std::string key("batch");
+    key.append((const char *)&items_dev, sizeof(items_dev));
+    key.append((const char *)&wg_item_dev, sizeof(wg_item_dev));
+    key.append((const char *)&step_base_dev, sizeof(step_base_dev));
+    const int ints[] = {ninst, total_wgs, D, order, nmax, waves, nsteps_per_graph};
+    key.append((const char *)ints, sizeof(ints));
+    hipGraphExec_t exec = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_graph_mu);
+        auto it = g_graphs.find(key);
+        if (it != g_graphs.end()) exec = it->second;
+    }
+    if (!exec) {
+        hipStream_t cap;
+        DH_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+        DH_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+        hipError_t el = hipSuccess;
+        for (int k = 0; k < nsteps_per_graph && el == hipSuccess; ++k) {
+            for (int a = 0; a < D && el == hipSuccess; ++a) {
+                long long off = k;
+                int ax = a;
+                void *args[] = {(void *)&items_dev, (void *)&wg_item_dev, &ax,
+                                (void *)&step_base_dev, &off};
+                el = hipLaunchKernel(fn, dim3((unsigned)total_wgs), dim3(64 * waves), args, lds,
+                                     cap);
+            }
+        }
+        if (el == hipSuccess)
+            hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, cap, step_base_dev,
+                               (long long)nsteps_per_graph);
+        hipGraph_t graph = nullptr;
+        hipError_t ec = hipStreamEndCapture(cap, &graph);
+        (void)hipStreamDestroy(cap);
+        if (el != hipSuccess) ec = el;
+        if (ec != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return fail(DH_EHIP, std::string("capture: ") + hipGetErrorString(ec));
+        }
+        hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess)
+            return fail(DH_EHIP, std::string("instantiate: ") + hipGetErrorString(ei));
+        std::lock_guard<std::mutex> lk(g_graph_mu);
+        if (g_graphs.size() > 64) {
+            if (hipDeviceSynchronize() != hipSuccess) return fail(DH_EHIP, "device sync");
+            for (auto &kv : g_graphs) (void)hipGraphExecDestroy(kv.second);
+            g_graphs.clear();
+        }
+        g_graphs[key] = exec;
+    }
+    hipLaunchKernelGGL(k_set_ll, dim3(1), dim3(1), 0, st, step_base_dev, first_step);
+    const long long last =
+        first_step + nsteps < max_total_steps ? first_step + nsteps : max_total_steps;
     const long long reps = (last - first_step + nsteps_per_graph - 1) / nsteps_per_graph;
     for (long long r = 0; r < reps; ++r) DH_HIP(hipGraphLaunch(exec, st));
     return DH_OK;

@@ -19,7 +19,7 @@ import threading
 import numpy as np
 
 from . import _lib
-from ._lib import INST_DTYPE, SEG_DTYPE, DhInstance, DhLaunch, check
+from ._lib import ITEM_DTYPE, INST_DTYPE, SEG_DTYPE, DhInstance, DhLaunch, check
 
 _ctx_lock = threading.Lock()
 _ctx = {}
@@ -52,6 +52,9 @@ class Context:
     #: 2D batches of at most this many integrations run on the factored
     #: (latency) kernel when their segments are long; 0 disables it
     factored_max_inst = 512
+    #: batches of 3-5 populations of at most this many integrations spread
+    #: each integration's sweeps over several workgroups; 0 disables it
+    spread_max_inst = 512
 
     def one_d_wave(self, ninst, nmax):
         """Whether a 1D batch runs one wave per integration (LDS-resident,
@@ -555,6 +558,8 @@ class PackedBatch:
             check(ctx.lib.dh_integrate_1d(self.tab.data_ptr(), len(self.jobs), self.order,
                                           self.nmax, self.counter.data_ptr(), scr.data_ptr(),
                                           scr.numel(), cfg, stream), "dh_integrate_1d")
+        elif max_steps <= 0 and self.spread_bound(waves, chunk, recompute, gamma_global, slots):
+            self.launch_spread(stream)
         elif self.latency_bound(waves, chunk, recompute, gamma_global, slots):
             ctx.factored_launch(self.tab.data_ptr(), len(self.jobs), self.D, self.order,
                                 self.nmax, self.counter.data_ptr(), stream, self.workspace,
@@ -564,6 +569,64 @@ class PackedBatch:
                                  self.nmax, self.counter.data_ptr(), stream, self.workspace,
                                  waves, groups, max_steps, chunk, recompute, gamma_global,
                                  slots, self.packable, self.delj)
+
+    def spread_bound(self, waves=None, chunk=None, recompute=None, gamma_global=None,
+                     slots=None):
+        """Whether this batch runs with each integration's sweeps spread over
+        several workgroups (dh_integrate_batch): three or more populations,
+        fewer integrations than twice the compute units (one workgroup each
+        would leave the GPU idle while each integration's thousands of lines
+        go through one compute unit), and no launch shape asked for."""
+        ctx = self.ctx
+        if self.D < 3 or self.delj or ctx.spread_max_inst <= 0:
+            return False
+        if any(v is not None for v in (waves, chunk, recompute, gamma_global, slots)):
+            return False
+        return len(self.jobs) <= ctx.spread_max_inst
+
+    def launch_spread(self, stream, waves=4, steps_per_graph=8, max_wgs=4096):
+        """Run every job to completion with dh_integrate_batch."""
+        ctx = self.ctx
+        t = torch()
+        items = np.zeros(len(self.jobs), dtype=ITEM_DTYPE)
+        raw = self.tab.cpu().numpy().view(INST_DTYPE)
+        keep = []
+        tiles = [-(-(j.n ** (j.D - 1)) // 64) for j in self.jobs]
+        total_tiles = sum(tiles)
+        # workgroups per integration: one wave per 64-line tile while the
+        # launch stays within max_wgs workgroups
+        scale = min(1.0, max_wgs * waves / max(total_tiles, 1))
+        wg = 0
+        wg_item = []
+        longest = 0
+        for i, (j, nt) in enumerate(zip(self.jobs, tiles)):
+            items[i]["inst"] = raw[i]
+            act = [k for k in range(j.D) if not (j.frozen >> k) & 1]
+            total = int(np.sum(j.seg["nsteps"])) if act else 0
+            sos = ctx.to_dev(np.repeat(np.arange(len(j.seg), dtype=np.int32), j.seg["nsteps"])
+                             if total else np.zeros(1, dtype=np.int32))
+            keep.append(sos)
+            items[i]["seg_of_step"] = sos.data_ptr()
+            items[i]["total_steps"] = total
+            items[i]["first_axis"] = act[0] if act else 0
+            w = max(1, int(-(-nt * scale // waves)))
+            items[i]["wg_begin"] = wg
+            items[i]["wgs"] = w
+            wg_item.extend([i] * w)
+            wg += w
+            longest = max(longest, total)
+        if longest == 0:
+            return
+        itd = ctx.to_dev(items.view(np.uint8))
+        wgd = ctx.to_dev(np.asarray(wg_item, dtype=np.int32))
+        base = t.zeros(1, dtype=t.int64, device=ctx.device)
+        nmax = max(j.n for j in self.jobs)
+        check(ctx.lib.dh_integrate_batch(itd.data_ptr(), wgd.data_ptr(), len(self.jobs), wg,
+                                         self.D, self.order, nmax, waves, base.data_ptr(),
+                                         longest, 0, longest, steps_per_graph, stream),
+              "dh_integrate_batch")
+        # the buffers must outlive the queued work
+        self.workspace["spread_keep"] = (itd, wgd, base, keep)
 
     def latency_bound(self, waves=None, chunk=None, recompute=None, gamma_global=None,
                       slots=None):

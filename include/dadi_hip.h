@@ -257,6 +257,39 @@ int dh_integrate_graph_range(const dh_instance *inst_host, long long *step_base_
                              long long first_step, long long nsteps, int nsteps_per_graph,
                              int waves, void *stream);
 
+/* One integration of a multi-workgroup batch (dh_integrate_batch). */
+typedef struct dh_sweep_item {
+    dh_instance inst;              /* phi, grid, schedule, n, order, masks, pitch */
+    const int *seg_of_step_dev;    /* segment index of each time step */
+    long long total_steps;
+    int first_axis;                /* first non-frozen axis: receives the step's injection */
+    int wg_begin;                  /* first workgroup of this integration in a sweep launch */
+    int wgs;                       /* workgroups sweeping its lines */
+    int reserved;
+} dh_sweep_item;
+
+/*
+ * A batch of integrations of D >= 2 populations advanced together, each sweep
+ * of each step spread over many workgroups: one launch per axis per step
+ * covers every integration (workgroup w sweeps lines of integration
+ * wg_item_dev[w]; items[i].wgs workgroups share integration i's lines), the
+ * steps of all integrations run in lockstep from a device step counter, and
+ * an integration past its last step is skipped.  nsteps_per_graph steps
+ * (all axes plus the counter advance) are captured into a hipGraph and
+ * replayed for steps first_step .. first_step+nsteps-1 (nsteps a multiple of
+ * nsteps_per_graph unless the range reaches the longest schedule).  For
+ * batches too small to fill the GPU with one workgroup per integration (a
+ * 3-population phi on 60 points sweeps 3,600 lines).  Same results as
+ * dh_integrate_nd.  Replaces, for each integration, the per-step loop of
+ * dadi/Integration.py:514-553 / 682-726 / 886-936 like dh_integrate_graph.
+ */
+int dh_integrate_batch(const dh_sweep_item *items_dev, const int *wg_item_dev, int ninst,
+                       int total_wgs, int D, int order, int nmax, int waves,
+                       long long *step_base_dev, long long max_total_steps,
+                       long long first_step, long long nsteps, int nsteps_per_graph,
+                       void *stream);
+size_t dh_sizeof_sweep_item(void);
+
 /*
  * Sampling phi -> expected spectrum (dadi/Spectrum_mod.py:1307-1348,
  * 1509-1548, 1774-1871): contracts the last axis of in_dev, shaped

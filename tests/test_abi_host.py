@@ -39,6 +39,7 @@ def test_struct_layout_and_errors_without_gpu():
     L = _lib.load()
     assert L.dh_sizeof_segment() == _lib.SEG_DTYPE.itemsize == 336
     assert L.dh_sizeof_instance() == _lib.INST_DTYPE.itemsize
+    assert L.dh_sizeof_sweep_item() == _lib.ITEM_DTYPE.itemsize == 112
     assert L.dh_abi_version() == _lib.ABI_VERSION == 6
     # calls before dh_init fail cleanly with a message (no GPU touched)
     cfg = _lib.DhLaunch(1, 1, 0, 0, 0)

@@ -519,3 +519,57 @@ def test_factored_kernel_matches_oracle(dadi_amd, oracle, D, n, order, frozen0):
     for q in phis:
         got = q.cpu().numpy()
         assert np.array_equal(got, want), np.max(np.abs(got - want))
+
+
+def test_spread_batch_matches_oracle(dadi_amd, oracle):
+    """Small batches of 3-5 populations run with each integration's sweeps
+    spread over several workgroups (dh_integrate_batch, lockstep steps from a
+    captured graph): every integration gets the oracle's bits, whatever its
+    step count, frozen populations included, and the same bits as the
+    one-workgroup kernel."""
+    from dadi_amd import engine, schedule as S, _lib
+    ctx = engine.context()
+    c = lambda v: (lambda t: v)
+    specs = []
+    for D, n, T, tv, frozen0 in [(3, 13, 0.02, False, False), (3, 11, 0.05, False, False),
+                                 (3, 12, 0.03, True, False), (4, 8, 0.02, True, False),
+                                 (4, 8, 0.01, True, True), (5, 6, 0.01, True, False)]:
+        xx = oracle.default_grid(n)
+        p = oracle.phi_1D(xx)
+        for d in range(2, D + 1):
+            p = oracle.phi_1D_to_2D(xx, p) if d == 2 else \
+                oracle.phi_2D_to_3D_split_2(xx, p) if d == 3 else \
+                oracle.phi_3D_to_4D(p, 0, 0, xx, xx, xx, xx) if d == 4 else \
+                oracle.phi_4D_to_5D(p, 0, 0, 0, xx, xx, xx, xx, xx)
+        nus = [0.8, 1.7, 0.6, 1.2, 2.0][:D]
+        migs = [[0.9 + 0.2 * k] * (D - 1) for k in range(D)]
+        gams = [-3.0, 2.0, -5.0, 0.0, 1.0][:D]
+        hs = [0.3, 0.5, 0.7, 0.5, 0.2][:D]
+        frozen = [frozen0] + [False] * (D - 1)
+        if frozen0:
+            migs[0] = [0.0] * (D - 1)
+        fz, inj = S.masks(D, frozen, [False] * D)
+        if tv:
+            nu0 = lambda t: 0.8 + 10 * t
+            seg = S.tv_segments(D, xx, 0.0, T, [nu0] + [c(v) for v in nus[1:]],
+                                [[c(m) for m in ms] for ms in migs], [c(g) for g in gams],
+                                [c(h) for h in hs], c(1.0), None)
+            want = oracle.integrate_tv(p, xx, T, [nu0] + nus[1:], migs, gams, hs, frozen=frozen)
+            order = _lib.ORDER_TV
+        else:
+            dt = min(S.compute_dt(nus[k], migs[k], gams[k], hs[k]) for k in range(D))
+            seg = S.const_segments(D, xx, 0.0, T, dt, nus, migs, gams, hs, 1.0)
+            want = oracle.integrate_const(p, xx, T, nus, migs, gams, hs, frozen=frozen)
+            order = _lib.ORDER_PRE
+        specs.append((p, xx, seg, order, fz, inj, want))
+    old = ctx.spread_max_inst
+    try:
+        for spread in (512, 0):
+            ctx.spread_max_inst = spread
+            jobs = [engine.Job(ctx.to_dev(p), xx, seg, order, fz, inj)
+                    for p, xx, seg, order, fz, inj, _ in specs]
+            engine.run_batch(jobs)
+            for k, (j, sp) in enumerate(zip(jobs, specs)):
+                assert np.array_equal(j.phi.cpu().numpy(), sp[-1]), (spread, k)
+    finally:
+        ctx.spread_max_inst = old
