@@ -40,7 +40,7 @@ INST_DTYPE = np.dtype([
 #: numpy mirror of ``dh_sweep_item``
 ITEM_DTYPE = np.dtype([
     ("inst", INST_DTYPE), ("seg_of_step", "<u8"), ("total_steps", "<i8"),
-    ("first_axis", "<i4"), ("wg_begin", "<i4"), ("wgs", "<i4"), ("reserved", "<i4"),
+    ("first_axis", "<i4"), ("wg_begin", "<i4"), ("wgs", "<i4"), ("step_shift", "<i4"),
 ])
 
 

@@ -2331,7 +2331,7 @@ __global__ void __launch_bounds__(256) k_sweep_batch(const dh_sweep_item *__rest
                                                      long long off)
 {
     const dh_sweep_item &it = items[wg_item[blockIdx.x]];
-    const long long st = *base + off;
+    const long long st = *base + off + it.step_shift;
     if (st >= it.total_steps) return;
     const dh_instance &I = it.inst;
     if ((I.frozen >> a) & 1) return;

@@ -166,7 +166,7 @@ class Server:
 
     def __init__(self, budget_bytes=8 << 30, quantum=200, ctx=None, quantum_ms=8.0,
                  wide_lines=512, wide_streams=2, steps_per_graph=16, wide_waves=2,
-                 factored=True):
+                 factored=True, wide_mode="graph"):
         import torch
         self.torch = torch
         self.ctx = ctx or engine.context()
@@ -178,6 +178,9 @@ class Server:
         self.spg = max(1, int(steps_per_graph))
         self.wide_waves = int(wide_waves)
         self.factored = bool(factored)
+        if wide_mode not in ("spread", "graph"):
+            raise ValueError("wide_mode is 'spread' or 'graph'")
+        self.wide_mode = wide_mode
         self.steps = {}       # class -> time steps per launch (adapted)
         self.streams = {}
         self.scratch = {}
@@ -188,7 +191,7 @@ class Server:
     def _steps_for(self, key):
         q = self.steps.get(key)
         if q is None:
-            q = self.quantum if key[0] != "wide" else 4 * self.spg
+            q = self.quantum if key[0] not in ("wide", "spread") else 4 * self.spg
         return q
 
     def _adapt(self, key, q, ms):
@@ -197,7 +200,7 @@ class Server:
             return
         want = q * self.quantum_ms / ms
         want = min(max(want, 0.5 * q), 2.0 * q)
-        if key[0] == "wide":
+        if key[0] in ("wide", "spread"):
             want = max(self.spg, int(round(want / self.spg)) * self.spg)
         else:
             want = max(1, min(int(round(want)), 50 * self.quantum))
@@ -305,6 +308,67 @@ class Server:
         self.wide_launches += 1
         return st, key, q, ev0
 
+    SPREAD_CAP = 128      # multi-workgroup epochs per spread launch
+    SPREAD_WGS = 2048     # workgroups per spread sweep launch
+    SPREAD_WAVES = 4
+
+    def _launch_spread(self, key, group):
+        """One quantum of every in-flight multi-workgroup epoch of one
+        (D, order) class in shared launches (dh_integrate_batch): each sweep
+        launch covers all of them, each epoch at its own step (step_shift).
+        The item and workgroup tables live in fixed buffers and the launch
+        shape is fixed, so one captured graph serves every quantum."""
+        t = self.torch
+        ctx = self.ctx
+        _, D, order = key
+        for r in group:                # beyond the capacity: no steps this quantum
+            r.wide["pending"] = 0
+        group = group[:self.SPREAD_CAP]
+        cap, nwg, waves = self.SPREAD_CAP, self.SPREAD_WGS, self.SPREAD_WAVES
+        bufs = self.scratch.setdefault(key, {})
+        if "items" not in bufs:
+            bufs["items"] = t.zeros((cap + 1) * _lib.ITEM_DTYPE.itemsize, dtype=t.uint8,
+                                    device=ctx.device)
+            bufs["wg"] = t.zeros(nwg, dtype=t.int32, device=ctx.device)
+            bufs["base"] = t.zeros(1, dtype=t.int64, device=ctx.device)
+        q = self._steps_for(("spread", D, order))
+        items = np.zeros(cap + 1, dtype=_lib.ITEM_DTYPE)   # the last one is empty
+        tiles = [-(-(r.prog.pts ** (D - 1)) // 64) for r in group]
+        want = [max(1, -(-nt // waves)) for nt in tiles]
+        scale = min(1.0, nwg / max(sum(want), 1))
+        wg_item = np.full(nwg, cap, dtype=np.int32)
+        wg = 0
+        longest = 0
+        for i, (r, w0) in enumerate(zip(group, want)):
+            w = r.wide
+            items[i]["inst"] = np.frombuffer(bytes(w["inst"]), dtype=_lib.INST_DTYPE)[0]
+            items[i]["seg_of_step"] = w["sosd"].data_ptr()
+            items[i]["total_steps"] = w["total"]
+            items[i]["first_axis"] = next(k for k in range(D) if not (w["inst"].frozen >> k) & 1)
+            items[i]["step_shift"] = w["cur"]
+            nw_i = max(1, int(w0 * scale))
+            items[i]["wg_begin"] = wg
+            items[i]["wgs"] = nw_i
+            wg_item[wg:wg + nw_i] = i
+            wg += nw_i
+            w["pending"] = min(q, w["total"] - w["cur"])
+            longest = max(longest, w["total"] - w["cur"])
+        st = self._stream(key)
+        st.wait_stream(t.cuda.current_stream(ctx.device))
+        nmax = -(-max(r.prog.pts for r in group) // 16) * 16
+        ev0 = t.cuda.Event(enable_timing=True)
+        with t.cuda.stream(st):
+            bufs["items"].copy_(t.from_numpy(items.view(np.uint8)))
+            bufs["wg"].copy_(t.from_numpy(wg_item))
+            ev0.record(st)
+            _lib.check(ctx.lib.dh_integrate_batch(
+                bufs["items"].data_ptr(), bufs["wg"].data_ptr(), cap + 1, nwg, int(D), int(order),
+                nmax, waves, bufs["base"].data_ptr(), longest, 0, q, self.spg, st.cuda_stream),
+                "dh_integrate_batch")
+        self.launches += 1
+        self.wide_launches += 1
+        return st, ("spread", D, order), q, ev0
+
     def _launch(self, group):
         """One quantum for every in-flight epoch of one (D, order) class."""
         t = self.torch
@@ -381,6 +445,9 @@ class Server:
                 status = "done"
             if status == "done":
                 self._retire(r, done, t0, clock)
+            elif status == "wide" and self.wide_mode == "spread":
+                inst = r.wide["inst"]
+                waiting.setdefault(("spread", inst.D, inst.order), []).append(r)
             elif status == "wide":
                 waiting.setdefault(("wide", r.rid), []).append(r)
             else:
@@ -418,6 +485,9 @@ class Server:
                     if key[0] == "wide":
                         st, ckey, q, ev0 = self._launch_wide(grp[0])
                         tabd = None
+                    elif key[0] == "spread":
+                        st, ckey, q, ev0 = self._launch_spread(key, grp)
+                        tabd = None
                     else:
                         st, tabd, counter, ckey, q, ev0 = self._launch(grp)
                 except Exception as e:  # the class's requests fail, the server goes on
@@ -437,11 +507,11 @@ class Server:
                 del inflight[key]
                 ms = ev0.elapsed_time(ev)
                 if tabd is None:
-                    w = grp[0].wide
-                    if w["pending"] == q:
+                    if any(r.wide["pending"] == q for r in grp):
                         self._adapt(ckey, q, ms)
-                    w["cur"] += w["pending"]
-                    enqueue(grp[0])
+                    for r in grp:
+                        r.wide["cur"] += r.wide["pending"]
+                        enqueue(r)
                     continue
                 back = tabd.cpu().numpy().view(INST_DTYPE)
                 for r, rec in zip(grp, back):

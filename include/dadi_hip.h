@@ -265,7 +265,7 @@ typedef struct dh_sweep_item {
     int first_axis;                /* first non-frozen axis: receives the step's injection */
     int wg_begin;                  /* first workgroup of this integration in a sweep launch */
     int wgs;                       /* workgroups sweeping its lines */
-    int reserved;
+    int step_shift;                /* this integration's step = batch step + step_shift */
 } dh_sweep_item;
 
 /*
@@ -274,7 +274,9 @@ typedef struct dh_sweep_item {
  * covers every integration (workgroup w sweeps lines of integration
  * wg_item_dev[w]; items[i].wgs workgroups share integration i's lines), the
  * steps of all integrations run in lockstep from a device step counter, and
- * an integration past its last step is skipped.  nsteps_per_graph steps
+ * an integration past its last step is skipped (integration i takes step
+ * s + items[i].step_shift at batch step s, so integrations at different points
+ * of their schedules can share the launches).  nsteps_per_graph steps
  * (all axes plus the counter advance) are captured into a hipGraph and
  * replayed for steps first_step .. first_step+nsteps-1 (nsteps a multiple of
  * nsteps_per_graph unless the range reaches the longest schedule).  For

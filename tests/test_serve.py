@@ -102,23 +102,26 @@ def test_poisoned_request_fails_alone(dadi_amd):
 
 @pytest.mark.gpu
 def test_wide_path_equals_batched_path(dadi_amd):
-    """3D and 4D epochs served on the multi-workgroup path (every sweep spread
+    """3D and 4D epochs served on the multi-workgroup paths (every sweep spread
     over the GPU, steps replayed from a captured graph in quanta of a few
-    steps) give the bits of the one-workgroup batched path."""
+    steps: one graph per request, or one shared launch per sweep for all
+    in-flight epochs of a class at their own steps) give the bits of the
+    one-workgroup batched path."""
     from dadi_amd import serve
     mix = _small_mix(21, 120)
     progs = ([r.prog for r in mix if r.prog.kind == "three_pop_chain"][:4]
              + [r.prog for r in mix if r.prog.kind == "four_pop_chain"][:2])
     assert len(progs) == 6
-    wide = [serve.Request(i, p, 0.0) for i, p in enumerate(progs)]
-    srv = serve.Server(wide_lines=1, steps_per_graph=4, quantum_ms=0.5)
-    st = srv.run(wide, realtime=False)
-    assert st["failed"] == 0 and st["wide_launches"] > len(progs)
     narrow = [serve.Request(i, p, 0.0) for i, p in enumerate(progs)]
     st2 = serve.Server(wide_lines=1 << 60, quantum=5).run(narrow, realtime=False)
     assert st2["failed"] == 0 and st2["wide_launches"] == 0
-    for a, b in zip(wide, narrow):
-        assert np.array_equal(a.result, b.result), a.prog.kind
+    for mode in ("graph", "spread"):
+        wide = [serve.Request(i, p, 0.0) for i, p in enumerate(progs)]
+        srv = serve.Server(wide_lines=1, steps_per_graph=4, quantum_ms=0.5, wide_mode=mode)
+        st = srv.run(wide, realtime=False)
+        assert st["failed"] == 0 and st["wide_launches"] > 2, (mode, st)
+        for a, b in zip(wide, narrow):
+            assert np.array_equal(a.result, b.result), (mode, a.prog.kind)
 
 
 @pytest.mark.gpu
