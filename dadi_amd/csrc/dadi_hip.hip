@@ -2322,9 +2322,11 @@ __global__ void __launch_bounds__(256) k_sweep_axis(dh_instance inst, int a, int
 
 // ---------------------------------------------------------------------------
 // Kernel 3b: one axis of one step of a batch of integrations, each one's
-// lines spread over several workgroups (checkpointed Thomas).
+// lines spread over several workgroups.  CK = 0: each lane's gamma column in
+// LDS (as k_sweep_axis); CK = 8: checkpointed Thomas, for grids whose
+// columns do not fit.
 // ---------------------------------------------------------------------------
-template <int D, int ORDER>
+template <int D, int ORDER, int CK>
 __global__ void __launch_bounds__(256) k_sweep_batch(const dh_sweep_item *__restrict__ items,
                                                      const int *__restrict__ wg_item, int a,
                                                      const long long *__restrict__ base,
@@ -2339,8 +2341,7 @@ __global__ void __launch_bounds__(256) k_sweep_batch(const dh_sweep_item *__rest
     const int n = I.n;
     const int nthr = blockDim.x, tid = threadIdx.x;
     const int nw = nthr >> 6, wave = tid >> 6, lane = tid & 63;
-    constexpr int CK = 8;
-    const int col = (n - 1) / CK + 1;
+    const int col = CK ? (n - 1) / (CK ? CK : 1) + 1 : n;
     const dh_segment *sp = gbl(I.seg_dev) + gbl(it.seg_of_step_dev)[st];
     double *sx = lds, *sxi = sx + n, *sdf = sxi + n, *sdx = sdf + n;
     double *W0 = sdx + n, *W1 = W0 + n, *S = W1 + n;
@@ -2518,12 +2519,16 @@ const void *pick_fz(int D, int order)
 }
 
 template <int D, int ORDER>
-const void *sb_kernel() { return reinterpret_cast<const void *>(&k_sweep_batch<D, ORDER>); }
+const void *sb_kernel(int ck)
+{
+    return ck ? reinterpret_cast<const void *>(&k_sweep_batch<D, ORDER, 8>)
+              : reinterpret_cast<const void *>(&k_sweep_batch<D, ORDER, 0>);
+}
 
-const void *pick_sweep_batch(int D, int order)
+const void *pick_sweep_batch(int D, int order, int ck)
 {
     switch (D * 4 + order) {
-#define DH_CASE(d, o) case d * 4 + o: return sb_kernel<d, o>();
+#define DH_CASE(d, o) case d * 4 + o: return sb_kernel<d, o>(ck);
         DH_CASE(2, 0) DH_CASE(2, 1) DH_CASE(2, 2)
         DH_CASE(3, 0) DH_CASE(3, 1) DH_CASE(3, 2)
         DH_CASE(4, 0) DH_CASE(4, 1) DH_CASE(4, 2)
@@ -2966,10 +2971,14 @@ int dh_integrate_batch(const dh_sweep_item *items_dev, const int *wg_item_dev, i
     if (first_step < 0) return fail(DH_EINVAL, "negative first step");
     if (nsteps % nsteps_per_graph != 0 && first_step + nsteps < max_total_steps)
         return fail(DH_EINVAL, "nsteps must be a multiple of nsteps_per_graph or reach the end");
-    const void *fn = pick_sweep_batch(D, order);
+    // the full gamma column per lane when it fits, else checkpoints
+    const size_t tables = (size_t)nmax * (LDS_GRID_TABLES + LDS_TABLES_PER_AXIS);
+    const size_t lds0 = sizeof(double) * (tables + (size_t)waves * 64 * (size_t)nmax);
+    const int ck = lds0 <= g_max_lds ? 0 : 8;
+    const void *fn = pick_sweep_batch(D, order, ck);
     const size_t col = (size_t)((nmax - 1) / 8 + 1);
-    const size_t lds = sizeof(double) * ((size_t)nmax * (LDS_GRID_TABLES + LDS_TABLES_PER_AXIS) +
-                                         (size_t)waves * 64 * (col + STAGE_C + 1));
+    const size_t lds = ck ? sizeof(double) * (tables + (size_t)waves * 64 * (col + STAGE_C + 1))
+                          : lds0;
     if (int e = set_lds(fn, lds)) return e;
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     std::string key("batch");

@@ -55,6 +55,7 @@ class Context:
     #: batches of 3-5 populations of at most this many integrations spread
     #: each integration's sweeps over several workgroups; 0 disables it
     spread_max_inst = 512
+    spread_waves = 0          # 0: chosen per batch
 
     def one_d_wave(self, ninst, nmax):
         """Whether a 1D batch runs one wave per integration (LDS-resident,
@@ -584,7 +585,7 @@ class PackedBatch:
             return False
         return len(self.jobs) <= ctx.spread_max_inst
 
-    def launch_spread(self, stream, waves=4, steps_per_graph=8, max_wgs=4096):
+    def launch_spread(self, stream, waves=None, steps_per_graph=8, max_wgs=4096):
         """Run every job to completion with dh_integrate_batch."""
         ctx = self.ctx
         t = torch()
@@ -593,6 +594,11 @@ class PackedBatch:
         keep = []
         tiles = [-(-(j.n ** (j.D - 1)) // 64) for j in self.jobs]
         total_tiles = sum(tiles)
+        if waves is None:
+            # two waves per workgroup (each lane's whole gamma column in LDS)
+            # while that leaves at most one workgroup per compute unit, else
+            # four (more workgroups resident per compute unit)
+            waves = ctx.spread_waves or (2 if total_tiles <= 2 * ctx.num_cu else 4)
         # workgroups per integration: one wave per 64-line tile while the
         # launch stays within max_wgs workgroups
         scale = min(1.0, max_wgs * waves / max(total_tiles, 1))
