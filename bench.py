@@ -221,7 +221,7 @@ def main():
     ap.add_argument("--dfe1d", type=int, default=1, help="time a Cache1D build (500 gammas)")
     ap.add_argument("--opt", type=int, default=4,
                     help="BFGS iterations of the optimiser comparison (0 = skip)")
-    ap.add_argument("--dfe2d", type=int, default=0,
+    ap.add_argument("--dfe2d", type=int, default=1,
                     help="time a Cache2D build (10x10 gammas; long: up to ~5e5*T steps)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
