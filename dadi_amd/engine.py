@@ -666,9 +666,17 @@ def run_batch(jobs, ctx=None, waves=None, groups=0, sort=True, chunk=None, recom
     for j in jobs:
         groups_by.setdefault((j.D, j.order, j.delj), []).append(j)
     parts = []
+    explicit = any(v is not None for v in (waves, chunk, recompute, gamma_global, slots))
     for key in sorted(groups_by):
         js = groups_by[key]
-        if key[0] >= 2 and slots != 1:
+        D, _, delj = key
+        # the latency paths take mixed grids in one launch: one launch, not
+        # one per grid size on streams that may share a hardware queue
+        one_launch = not explicit and not delj and (
+            (D == 2 and len(js) <= ctx.factored_max_inst and
+             ctx.factorable([j.seg for j in js])) or
+            (D >= 3 and len(js) <= ctx.spread_max_inst))
+        if key[0] >= 2 and slots != 1 and not one_launch:
             by = {}
             for j in js:
                 by.setdefault((j.n, ctx.grid(j.xx).data_ptr(), pitch_of(j.phi)), []).append(j)
@@ -705,10 +713,19 @@ class Coalescer:
         self.live = 0
         self.launches = 0
 
-    def register(self):
-        with self.cv:
-            self.live += 1
+    def register(self, counted=False):
+        """Make the calling thread a worker.  counted: its place in ``live``
+        was already taken by expect()."""
+        if not counted:
+            with self.cv:
+                self.live += 1
         _local.coalescer = self
+
+    def expect(self, k):
+        """Count k workers as live before they start, so that the first
+        ones to submit do not flush a batch without the rest."""
+        with self.cv:
+            self.live += k
 
     def unregister(self):
         _local.coalescer = None
@@ -748,7 +765,7 @@ def map_batched(func, args_list, workers=256, ctx=None):
     lock = threading.Lock()
 
     def worker():
-        co.register()
+        co.register(counted=True)
         try:
             while True:
                 with lock:
@@ -765,6 +782,7 @@ def map_batched(func, args_list, workers=256, ctx=None):
 
     nthreads = max(1, min(workers, len(args_list)))
     threads = [threading.Thread(target=worker, daemon=True) for _ in range(nthreads)]
+    co.expect(nthreads)   # every worker counts from the start: one batch, not several
     for th in threads:
         th.start()
     for th in threads:
