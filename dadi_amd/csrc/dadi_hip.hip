@@ -1971,7 +1971,10 @@ __global__ void __launch_bounds__(64) k_integrate_1w(dh_instance *__restrict__ t
 // rather than bandwidth bounds a step.
 // ---------------------------------------------------------------------------
 
-constexpr int FZ_PF = 8;   // rows loaded ahead
+#ifndef DH_FZ_PF
+#define DH_FZ_PF 8
+#endif
+constexpr int FZ_PF = DH_FZ_PF;   // rows per block (two blocks in flight)
 
 // Workspace doubles per workgroup: a, bet, 1/bet, gamma per axis, plus u'.
 long long fz_slice(int D, int n)
