@@ -6,7 +6,6 @@ evaluation on the GPU through engine.map_batched.
 import numpy as np
 import scipy.integrate
 
-from .. import Numerics
 from ..Spectrum_mod import Spectrum
 from .Cache1D_mod import _evaluate_grid, _trapz
 

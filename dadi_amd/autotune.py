@@ -26,7 +26,6 @@ Results live in a JSON table (``tuning_gfx950.json`` beside this file by
 default, or $DADI_AMD_TUNING) that engine.Context consults.
 """
 import json
-import math
 import os
 import time
 

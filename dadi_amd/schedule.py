@@ -11,7 +11,7 @@ what dadi's Python loops produce.
 """
 import numpy as np
 
-from ._lib import SEG_DTYPE, MAXD
+from ._lib import SEG_DTYPE
 
 
 def compute_dt(nu, ms, gamma, h, timescale_factor=1e-3, use_old_timestep=False,

@@ -35,7 +35,7 @@ so a result is bit-identical alone and in any mix.
 """
 import math
 import time
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 
 import numpy as np
 
