@@ -216,8 +216,9 @@ def main():
                     help="split_mig model evaluations (3 grids each) through the public API")
     ap.add_argument("--evals3d", type=int, default=96,
                     help="3-population chain evaluations (3 grids each)")
-    ap.add_argument("--workers", type=int, default=512,
-                    help="threads evaluating models concurrently")
+    ap.add_argument("--workers", type=int, default=256,
+                    help="threads evaluating models concurrently (each runs the grids of "
+                         "its extrapolated evaluation in two more threads)")
     ap.add_argument("--dfe1d", type=int, default=1, help="time a Cache1D build (500 gammas)")
     ap.add_argument("--opt", type=int, default=4,
                     help="BFGS iterations of the optimiser comparison (0 = skip)")

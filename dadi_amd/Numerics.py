@@ -90,6 +90,11 @@ def extrapolate(result_l, x_l, extrap_log=False, fail_mag=10):
     return ex_result
 
 
+# Evaluate the grids of an extrapolated function concurrently when called
+# from a batching worker (engine.map_batched); False: one grid after another.
+parallel_grids = True
+
+
 def make_extrap_func(func, extrap_x_l=None, extrap_log=False, fail_mag=10):
     """A version of ``func`` whose last argument is a list of grid sizes and
     which returns the result extrapolated to infinitely many grid points
@@ -106,7 +111,13 @@ def make_extrap_func(func, extrap_x_l=None, extrap_log=False, fail_mag=10):
         if numpy.isscalar(pts_l):
             pts_l = [pts_l]
         partial_func = functools.partial(func, *other_args, **kwargs)
-        result_l = list(map(partial_func, pts_l))
+        if parallel_grids:
+            # inside engine.map_batched the grids are evaluated side by side,
+            # their integrations in the same launches (same results)
+            from . import engine
+            result_l = engine.map_in_worker(partial_func, list(pts_l))
+        else:
+            result_l = list(map(partial_func, pts_l))
         if no_extrap:
             return result_l
         if x_l_from_results:

@@ -50,8 +50,13 @@ class Context:
     one_d_kernel = "auto"
     one_d_wave_max = 16384
     #: 2D batches of at most this many integrations run on the factored
-    #: (latency) kernel when their segments are long; 0 disables it
-    factored_max_inst = 512
+    #: kernel when their segments are long; 0 disables it.  Measured on
+    #: MI355X (split_mig draws, n = 40..100): 1.2-3.1x the packed kernel's
+    #: integrations/s for 512-2048 integrations, 0.68x at 8192 (n = 100)
+    factored_max_inst = 2048
+    #: total factorisation workspace up to which a large factored batch runs
+    #: two workgroups per compute unit (more stays out of the last-level cache)
+    factored_ws_bytes = 160 << 20
     #: batches of 3-5 populations of at most this many integrations spread
     #: each integration's sweeps over several workgroups; 0 disables it
     spread_max_inst = 512
@@ -74,8 +79,9 @@ class Context:
         workspace between calls."""
         tiles = -(-(nmax ** (D - 1)) // 64)
         w = waves or max(1, min(8, tiles))
-        g = groups if groups > 0 else min(ninst, self.num_cu)
-        need = g * int(self.lib.dh_factored_slice_doubles(D, nmax))
+        slice_d = int(self.lib.dh_factored_slice_doubles(D, nmax))
+        g = groups if groups > 0 else self.factored_groups(ninst, slice_d)
+        need = g * slice_d
         ws = workspace.get("factored")
         if ws is None or ws.numel() < need:
             ws = torch().empty(need, dtype=torch().float64, device=self.device)
@@ -86,6 +92,17 @@ class Context:
         cfg.max_us = int(max_us)
         check(self.lib.dh_integrate_factored(tab_ptr, ninst, D, order, nmax, counter_ptr, cfg,
                                              stream), "dh_integrate_factored")
+
+    def factored_groups(self, ninst, slice_doubles):
+        """Workgroups of a factored launch: one per compute unit, two for
+        batches of at least eight integrations per compute unit whose
+        workspaces then stay within factored_ws_bytes (measured: n = 40 and
+        60, 2048 integrations, 1.6x and 1.35x; at n = 100 two per compute
+        unit are slower)."""
+        if ninst >= 8 * self.num_cu and \
+                2 * self.num_cu * slice_doubles * 8 <= self.factored_ws_bytes:
+            return 2 * self.num_cu
+        return min(ninst, self.num_cu)
 
     @staticmethod
     def factorable(segs, min_steps=4):
@@ -637,9 +654,10 @@ class PackedBatch:
     def latency_bound(self, waves=None, chunk=None, recompute=None, gamma_global=None,
                       slots=None):
         """Whether this batch goes to the factored kernel: two populations,
-        too few integrations to fill the GPU (each one's step latency, not
-        the batch's bandwidth, sets the time), long segments, and no launch
-        shape asked for explicitly."""
+        at most factored_max_inst integrations (up to there each one's step
+        latency, not the batch's bandwidth, sets the time, and the factored
+        kernel's shorter steps win), long segments, and no launch shape asked
+        for explicitly."""
         ctx = self.ctx
         if self.D != 2 or self.delj or ctx.factored_max_inst <= 0:
             return False
@@ -733,6 +751,17 @@ class Coalescer:
             self.live -= 1
             self._maybe_flush()
 
+    def suspend(self):
+        """The calling worker stops counting as live (it is about to wait for
+        something other than a launch, e.g. threads of its own)."""
+        with self.cv:
+            self.live -= 1
+            self._maybe_flush()
+
+    def resume(self):
+        with self.cv:
+            self.live += 1
+
     def submit(self, job):
         with self.cv:
             self.waiting.append(job)
@@ -752,6 +781,59 @@ class Coalescer:
                     j.done = True
                 self.launches += 1
                 self.cv.notify_all()
+
+
+#: map_in_worker runs its items one after another once a coalescer has this
+#: many workers (bounds the threads of one process)
+max_worker_threads = 768
+
+
+def current_coalescer():
+    """The Coalescer the calling thread works for, or None."""
+    return getattr(_local, "coalescer", None)
+
+
+def map_in_worker(func, items):
+    """[func(x) for x in items].  Called from a Coalescer's worker thread, the
+    items after the first run in threads of their own that join the same
+    coalescer, so that the integrations of all items go into the same batched
+    launches instead of one launch round per item (the grids of an
+    extrapolated model evaluation); elsewhere a plain loop.  An exception of
+    any item is raised after all have finished (the first in item order)."""
+    co = current_coalescer()
+    if co is None or len(items) < 2 or co.live + len(items) - 1 > max_worker_threads:
+        return [func(x) for x in items]
+    n = len(items)
+    out = [None] * n
+    err = [None] * n
+
+    def run(i):
+        co.register(counted=True)
+        try:
+            out[i] = func(items[i])
+        except BaseException as e:  # noqa: BLE001  (re-raised by the caller)
+            err[i] = e
+        finally:
+            co.unregister()
+
+    threads = [threading.Thread(target=run, args=(i,), daemon=True) for i in range(1, n)]
+    co.expect(n - 1)
+    for th in threads:
+        th.start()
+    try:
+        out[0] = func(items[0])
+    except BaseException as e:  # noqa: BLE001
+        err[0] = e
+    try:
+        co.suspend()
+    finally:
+        for th in threads:
+            th.join()
+        co.resume()
+    for e in err:
+        if e is not None:
+            raise e
+    return out
 
 
 def map_batched(func, args_list, workers=256, ctx=None):

@@ -51,3 +51,50 @@ def test_map_batched_second_round_batches_again(monkeypatch):
 
     engine.map_batched(model, [(k,) for k in range(64)], workers=16, ctx=object())
     assert sum(batches) == 128 and len(batches) <= 16
+
+
+def test_grids_of_an_extrapolated_evaluation_share_one_launch(monkeypatch):
+    """Inside map_batched the grids of make_extrap_log_func run side by side:
+    every evaluation's three integrations go into the first launch (one
+    round, not three), and results come back in grid order."""
+    from dadi_amd import Numerics
+    batches = []
+    monkeypatch.setattr(engine, "run_batch", lambda jobs, ctx=None: batches.append(len(jobs)))
+    monkeypatch.setattr(Numerics, "parallel_grids", True)
+
+    def model(k, pts):
+        time.sleep(0.001 * (pts % 7))
+        job = _Job(k)
+        engine._local.coalescer.submit(job)
+        assert job.done
+        return (k, pts)
+
+    out = engine.map_batched(lambda k: Numerics.make_extrap_func(model)(k, [40, 50, 60],
+                                                                         no_extrap=True),
+                             [(k,) for k in range(40)], workers=40, ctx=object())
+    assert out == [[(k, 40), (k, 50), (k, 60)] for k in range(40)]
+    assert batches == [120]
+
+
+def test_grid_threads_report_errors_and_release_the_batch(monkeypatch):
+    """A grid that raises is reported to its evaluation; the other
+    evaluations' integrations still run (no worker is left waiting)."""
+    batches = []
+    monkeypatch.setattr(engine, "run_batch", lambda jobs, ctx=None: batches.append(len(jobs)))
+
+    def model(k, pts):
+        if k == 3 and pts == 50:
+            raise ValueError("bad grid")
+        job = _Job(k)
+        engine._local.coalescer.submit(job)
+        return pts
+
+    out = engine.map_batched(lambda k: engine.map_in_worker(lambda p: model(k, p), [40, 50, 60]),
+                             [(k,) for k in range(8)], workers=8, ctx=object())
+    assert isinstance(out[3], ValueError)
+    assert all(out[k] == [40, 50, 60] for k in range(8) if k != 3)
+    assert sum(batches) == 7 * 3 + 2
+
+
+def test_map_in_worker_outside_a_coalescer_is_a_loop():
+    assert engine.map_in_worker(lambda x: x * x, [1, 2, 3]) == [1, 4, 9]

@@ -573,3 +573,49 @@ def test_spread_batch_matches_oracle(dadi_amd, oracle):
                 assert np.array_equal(j.phi.cpu().numpy(), sp[-1]), (spread, k)
     finally:
         ctx.spread_max_inst = old
+
+
+def test_large_factored_batch_equals_packed_kernel(dadi_amd, oracle):
+    """A 2D batch of eight integrations per compute unit on a small grid runs
+    on the factored kernel with two workgroups per compute unit (the
+    automatic choice); every result equals the packed kernel's and, spot
+    checked, the oracle's."""
+    import torch
+    from dadi_amd import engine, schedule as S, _lib
+    ctx = engine.context()
+    B, n = 8 * ctx.num_cu, 20
+    assert B <= ctx.factored_max_inst
+    slice_d = int(ctx.lib.dh_factored_slice_doubles(2, n))
+    assert ctx.factored_groups(B, slice_d) == 2 * ctx.num_cu
+    rng = np.random.default_rng(77)
+    xx = oracle.default_grid(n)
+    phi0 = oracle.phi_1D_to_2D(xx, oracle.phi_1D(xx))
+    specs = []
+    for _ in range(B):
+        nus = list(np.exp(rng.uniform(np.log(0.2), np.log(5), 2)))
+        m = float(rng.uniform(0, 3))
+        T = float(np.exp(rng.uniform(np.log(0.01), np.log(0.2))))
+        dt = min(S.compute_dt(v, [m], 0, 0.5) for v in nus)
+        specs.append((S.const_segments(2, xx, 0.0, T, dt, nus, [[m], [m]], [0, 0], [0.5, 0.5],
+                                       1.0), (T, nus, m)))
+    fz, inj = S.masks(2, [False, False], [False, False])
+
+    def jobs():
+        arena = ctx.empty_phi((n, n), batch=B)
+        arena.copy_(torch.from_numpy(phi0).to(ctx.device).expand(B, n, n))
+        return arena, [engine.Job(arena[i], xx, specs[i][0], _lib.ORDER_PRE, fz, inj)
+                       for i in range(B)]
+
+    a_fz, j_fz = jobs()
+    pb = engine.PackedBatch(ctx, j_fz)
+    assert pb.latency_bound()
+    pb.launch()
+    a_pk, j_pk = jobs()
+    engine.run_batch(j_pk, slots=2)   # an explicit shape: the packed kernel
+    torch.cuda.synchronize()
+    got, want = a_fz.cpu().numpy(), a_pk.cpu().numpy()
+    assert np.array_equal(got, want)
+    for i in (0, B // 2, B - 1):
+        T, nus, m = specs[i][1]
+        ref = oracle.integrate_const(phi0, xx, T, nus, [[m], [m]], [0, 0], [0.5, 0.5])
+        assert np.array_equal(got[i], ref), i
