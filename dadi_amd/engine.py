@@ -479,10 +479,12 @@ def run_single(phi, xx, seg, order, frozen, inject, ctx=None, mode="auto", waves
 
 class Job:
     """One integration of a batch: device phi (updated in place) plus schedule."""
-    __slots__ = ("phi", "xx", "seg", "order", "frozen", "inject", "tag", "done", "delj")
+    __slots__ = ("phi", "xx", "seg", "order", "frozen", "inject", "tag", "done", "delj",
+                 "error")
 
     def __init__(self, phi, xx, seg, order, frozen, inject, tag=None, delj=False):
         self.done = False
+        self.error = None
         self.delj = bool(delj)
         self.phi = phi
         self.xx = np.asarray(xx, dtype=np.float64)
@@ -768,19 +770,26 @@ class Coalescer:
             self._maybe_flush()
             while not job.done:
                 self.cv.wait()
+        if job.error is not None:
+            raise job.error
 
     def _maybe_flush(self):
+        """Runs the batch once every live worker waits; a launch that fails
+        raises its error in every worker whose job it held."""
         if self.waiting and len(self.waiting) >= self.live:
             jobs, self.waiting = self.waiting, []
+            err = None
             try:
                 if self._ctx is None:
                     self._ctx = context()
                 run_batch(jobs, self._ctx)
-            finally:
-                for j in jobs:
-                    j.done = True
-                self.launches += 1
-                self.cv.notify_all()
+            except BaseException as e:  # noqa: BLE001  (raised in every worker)
+                err = e
+            for j in jobs:
+                j.error = err
+                j.done = True
+            self.launches += 1
+            self.cv.notify_all()
 
 
 #: map_in_worker runs its items one after another once a coalescer has this

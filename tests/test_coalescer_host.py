@@ -98,3 +98,20 @@ def test_grid_threads_report_errors_and_release_the_batch(monkeypatch):
 
 def test_map_in_worker_outside_a_coalescer_is_a_loop():
     assert engine.map_in_worker(lambda x: x * x, [1, 2, 3]) == [1, 4, 9]
+
+
+def test_failed_launch_raises_in_every_worker_of_it(monkeypatch):
+    """A launch that fails raises its error in each evaluation whose
+    integration it held, not only in the thread that ran it."""
+    def launch(jobs, ctx=None):
+        raise RuntimeError("launch failed")
+
+    monkeypatch.setattr(engine, "run_batch", launch)
+
+    def model(k):
+        job = _Job(k)
+        engine._local.coalescer.submit(job)
+        return k
+
+    out = engine.map_batched(model, [(k,) for k in range(16)], workers=16, ctx=object())
+    assert all(isinstance(r, RuntimeError) for r in out)
