@@ -224,6 +224,9 @@ def main():
                     help="BFGS iterations of the optimiser comparison (0 = skip)")
     ap.add_argument("--dfe2d", type=int, default=1,
                     help="time a Cache2D build (10x10 gammas; long: up to ~5e5*T steps)")
+    ap.add_argument("--routing", default="512,2048",
+                    help="batch sizes (comma-separated, from the same draws) timed with the "
+                         "automatic kernel choice and with the packed kernel; '' = skip")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=200)
@@ -332,6 +335,39 @@ def main():
     plain_rate = len(lat) / sum(lat) if lat else None
     plain_steps = sum(int(np.sum(segs[i]["nsteps"])) for i in range(args.plain))
 
+    # kernel choice for smaller batches: the automatic route (the factored
+    # kernel up to engine.Context.factored_max_inst integrations) against the
+    # packed kernel on the same integrations
+    routing = []
+    for B in [int(v) for v in args.routing.split(",") if v.strip()]:
+        B = min(B, args.batch)
+        ar = ctx.empty_phi((n, n), batch=B)
+        rjobs = [engine.Job(ar[i], xx, segs[i], _lib.ORDER_PRE, 0, 3) for i in range(B)]
+        rpb = engine.PackedBatch(ctx, sorted(rjobs, key=lambda j: -j.work()))
+        rates, outs = {}, {}
+        for mode in ("auto", "packed"):
+            def go():
+                ar.copy_(phi0d.expand(B, n, n))
+                rpb.restart()
+                if mode == "auto":
+                    rpb.launch()
+                else:
+                    rpb.launch(waves=shape.waves, chunk=shape.chunk, recompute=shape.recompute,
+                               gamma_global=shape.gamma_global, slots=shape.slots)
+            go()
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            go()
+            torch.cuda.synchronize()
+            rates[mode] = B / (time.perf_counter() - t1)
+            outs[mode] = ar.cpu().numpy()
+        routing.append({"batch": B, "unit": "integrations/s",
+                        "auto_kernel": "k_integrate_fz<2, 1>" if rpb.latency_bound() else
+                        "k_integrate_packed<2, 1>",
+                        "auto": rates["auto"], "packed": rates["packed"],
+                        "same_result": bool(np.array_equal(outs["auto"], outs["packed"]))})
+        del rpb, rjobs, ar
+
     serving = None
     if args.serve_requests > 0:
         from dadi_amd import serve
@@ -377,6 +413,7 @@ def main():
                   "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3) if lat else None,
                                  "p99": float(np.percentile(lat, 99) * 1e3) if lat else None}},
         "batched_over_plain": value / plain_rate if plain_rate else None,
+        "routing": routing,
         "serving": serving,
         "model_workloads": workloads,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
