@@ -250,3 +250,51 @@ def test_delj_numpy_path_matches_reference_c(oracle):
     plain = oracle.integrate_const(*args, backend="ref")
     assert np.allclose(pre, tv, rtol=1e-9, atol=1e-12 * np.abs(pre).max())
     assert not np.allclose(pre, plain, rtol=1e-6)
+
+
+@pytest.mark.parametrize("gamma,h,beta", [(2.0, 0.2, 1.0), (-3.0, 0.2, 1.0), (-3.0, 0.8, 1.0),
+                                          (4.0, 0.7, 1.0), (1.5, 0.3, 2.0), (0.0, 0.3, 1.0)])
+def test_phi_1D_dominance_branch_matches_closed_form(gamma, h, beta):
+    """phi_1D for h != 0.5 (dadi/PhiManip.py:45-108: numerical quadrature of
+    exp(-Q), Q(x) = 4 g h x + 2 g (1-2h) x^2, g = gamma 4 beta/(beta+1)^2)
+    against the closed form of the same integrals through erf / erfi,
+    evaluated with mpmath at 40 digits:
+    phi(x) = e^Q(x) int_x^1 e^-Q / int_0^1 e^-Q / (x(1-x)) * nu theta0 4 beta/(beta+1)^2,
+    with the end values phi[0] = phi[1] and phi[-1] = 1/int_0^1 e^-Q.  For
+    gamma = 0 this is the neutral 1/x."""
+    import mpmath
+    from dadi_amd import Numerics, PhiManip
+    mpmath.mp.dps = 40
+    xx = Numerics.default_grid(31)
+    nu, theta0 = 1.7, 0.9
+    got = PhiManip.phi_1D(xx, nu=nu, theta0=theta0, gamma=gamma, h=h, beta=beta)
+    g = mpmath.mpf(gamma) * 4 * beta / (beta + 1) ** 2
+    a, b = 4 * g * h, 2 * g * (1 - 2 * mpmath.mpf(h))
+
+    def prim(v):   # an antiderivative of exp(-a v - b v^2)
+        v = mpmath.mpf(v)
+        if b > 0:
+            c = a / (2 * b)
+            return mpmath.exp(c * c * b) * mpmath.sqrt(mpmath.pi / b) / 2 * mpmath.erf(
+                mpmath.sqrt(b) * (v + c))
+        if b < 0:
+            c = a / (2 * b)
+            return mpmath.exp(c * c * b) * mpmath.sqrt(mpmath.pi / -b) / 2 * mpmath.erfi(
+                mpmath.sqrt(-b) * (v + c))
+        return v if a == 0 else -mpmath.exp(-a * v) / a
+
+    scale = nu * theta0 * 4 * beta / (beta + 1) ** 2
+    int0 = prim(1) - prim(0)
+    want = np.empty(len(xx))
+    for i, x in enumerate(xx):
+        if 0 < i < len(xx) - 1:
+            q = mpmath.mpf(float(x))
+            Q = a * q + b * q * q
+            want[i] = float(mpmath.exp(Q) * (prim(1) - prim(q)) / int0 / (q * (1 - q)) * scale)
+    want[0] = want[1]
+    want[-1] = float(1 / int0 * scale)
+    np.testing.assert_allclose(got, want, rtol=1e-9)
+    # the check separates a plausible mistake: the dominance orientation (h <-> 1-h)
+    if gamma != 0:
+        other = PhiManip.phi_1D(xx, nu=nu, theta0=theta0, gamma=gamma, h=1 - h, beta=beta)
+        assert np.max(np.abs(other / want - 1)) > 1e-3
