@@ -137,3 +137,47 @@ def test_param_projection_round_trip():
     up = Inference._project_params_up(p, fixed)
     assert list(up) == [1.5, 0.3, 0.7, 2.0]
     assert list(Inference._project_params_down(up, fixed)) == [1.5, 0.7]
+
+
+@pytest.mark.parametrize("params", [(np.log(5), 2.0, np.log(5), 2.0),
+                                    (np.log(4), 1.8, np.log(7), 2.2)])
+def test_cache2d_integrate_accounts_for_all_mass(params):
+    """Cache2D.integrate (dadi/DFE/Cache2D_mod.py:209-310) with every cached
+    spectrum equal to one returns the mass of the density it integrates: the
+    trapezoid over the cached square, the four strips outside it (scipy quad
+    of each marginal beyond the grid, assigned to the edge spectra) and three
+    of the four outer corners (dblquad) -- the corner beyond the grid in both
+    coordinates is left out, as in the reference.  The density is a product of
+    two lognormals times 1e4 (integrate is linear in it; the scale keeps the
+    reference's absolute quadrature tolerance of 1e-4 small against every
+    weight), with 8-19% of each marginal's mass on either side of a grid on
+    [0.5, 50]: every strip is >= 7% and every corner >= 0.9% of the total,
+    so a missing term (or one integrated over the wrong range) exceeds the
+    5e-3 tolerance.  Symmetric and asymmetric densities take different code
+    paths."""
+    from scipy.stats import lognorm
+    from dadi_amd.DFE.Cache2D_mod import Cache2D
+    mu1, s1, mu2, s2 = params
+    d1 = lognorm(s=s1, scale=np.exp(mu1))
+    d2 = lognorm(s=s2, scale=np.exp(mu2))
+    K = 1e4
+
+    def sel_dist(x, y, p):
+        x, y = np.asarray(x, dtype=float), np.asarray(y, dtype=float)
+        if x.ndim == 1 and y.ndim == 1:
+            return K * d1.pdf(x)[:, None] * d2.pdf(y)[None, :]
+        return K * d1.pdf(x) * d2.pdf(y)
+
+    lo, hi, N = 0.5, 50., 50
+    c = Cache2D.__new__(Cache2D)
+    c.neg_gammas = -np.logspace(np.log10(hi), np.log10(lo), N)
+    c.gammas = c.neg_gammas
+    c.spectra = np.ones((N, N, 2, 2))
+    for p in (d1.cdf(lo), d1.sf(hi), d2.cdf(lo), d2.sf(hi)):
+        assert p > 0.08
+    fs = c.integrate(params, None, sel_dist, 1.0, None)
+    np.testing.assert_allclose(np.asarray(fs), K * (1 - d1.sf(hi) * d2.sf(hi)), rtol=5e-3)
+    # without the exterior terms only the cached square's mass remains
+    inner = c.integrate(params, None, sel_dist, 1.0, None, exterior_int=False)
+    sq = (d1.cdf(hi) - d1.cdf(lo)) * (d2.cdf(hi) - d2.cdf(lo))
+    np.testing.assert_allclose(np.asarray(inner), K * sq, rtol=5e-3)
