@@ -123,17 +123,19 @@ def sample_device_direct(phi, ns, xxs, het_ascertained=None, ctx=None):
     return cur
 
 
-def sample_device(phi, ns, xx, ctx=None):
+def sample_device(phi, ns, xx, ctx=None, batched=False):
     """Expected sample spectrum data of phi on the device; returns a device
     tensor of shape (n0+1, ..., n_{D-1}+1).  phi may be numpy or a device
-    tensor; every axis uses grid xx."""
+    tensor; every axis uses grid xx.  batched: phi has a leading batch axis
+    (B, n, ..., n) and the result is (B, n0+1, ..., n_{D-1}+1); each entry is
+    computed with the same operations as for its phi alone."""
     import torch
     ctx = ctx or engine.context()
     if isinstance(phi, numpy.ndarray):
         cur = ctx.to_dev(numpy.ascontiguousarray(phi, dtype=numpy.float64))
     else:
         cur = phi.to(dtype=torch.float64).contiguous()
-    D = cur.dim()
+    D = cur.dim() - (1 if batched else 0)
     if len(ns) != D:
         raise ValueError('Dimensionality of phi and lengths of ns do not agree.')
     xx = numpy.asarray(xx, dtype=numpy.float64)
@@ -157,9 +159,31 @@ def sample_device(phi, ns, xx, ctx=None):
                                           out.data_ptr(), stream), "dh_sample_axis")
         keep.append((cur, work))
         cur = out
+    if batched:   # the batch axis ends last: (n0+1, ..., n_{D-1}+1, B)
+        cur = cur.permute((D,) + tuple(range(D))).contiguous()
     torch.cuda.current_stream(ctx.device).synchronize()
     del keep
     return cur
+
+
+def sample_jobs(jobs, ctx=None):
+    """Runs waiting engine.SampleJob samplings: the phis of one shape,
+    sample sizes and grid go to the device in one copy and are sampled in
+    one pass per axis (at most ~2^27 grid points per pass, which bounds the
+    slope workspace to ~2 GB); results back to the host in one copy."""
+    ctx = ctx or engine.context()
+    groups = {}
+    for j in jobs:
+        groups.setdefault((j.phi.shape, j.ns, j.xx.tobytes()), []).append(j)
+    for (shape, ns, _), grp in groups.items():
+        per = max(1, (1 << 27) // int(numpy.prod(shape)))
+        for c0 in range(0, len(grp), per):
+            part = grp[c0:c0 + per]
+            stack = ctx.to_dev(numpy.stack([j.phi for j in part]))
+            data = sample_device(stack, ns, part[0].xx, ctx, batched=True).cpu().numpy()
+            for i, j in enumerate(part):
+                j.result = numpy.array(data[i])
+            del stack
 
 
 class Spectrum(numpy.ma.masked_array):
@@ -234,7 +258,14 @@ class Spectrum(numpy.ma.masked_array):
             if not (len(other) == len(xx) and numpy.allclose(other, xx)):
                 raise ValueError('Must have xx==yy to use linear algebra calculation of FS '
                                  'from phi.')
-        data = sample_device(phi, ns, xx).cpu().numpy()
+        co = engine.current_coalescer()
+        if co is not None and isinstance(phi, numpy.ndarray):
+            # a batching worker: sampled with every other waiting phi
+            job = engine.SampleJob(numpy.ascontiguousarray(phi, dtype=numpy.float64), ns, xx)
+            co.submit(job)
+            data = job.result
+        else:
+            data = sample_device(phi, ns, xx).cpu().numpy()
         fs = Spectrum(data, mask_corners=mask_corners, pop_ids=pop_ids)
         fs.extrap_x = xxs[0][1]
         for x in xxs[1:]:

@@ -396,13 +396,21 @@ def run_single(phi, xx, seg, order, frozen, inject, ctx=None, mode="auto", waves
     t = torch()
     ctx = ctx or context()
     host = isinstance(phi, np.ndarray)
+    co = getattr(_local, "coalescer", None)
     if host:
         src = np.ascontiguousarray(phi, dtype=np.float64)
+        if src.shape[0] < 3:
+            raise ValueError("grids need at least 3 points")
+        if co is not None and len(seg) and frozen != (1 << src.ndim) - 1:
+            # the launch uploads every waiting host phi at once
+            job = Job(None, xx, seg, order, frozen, inject, delj=delj)
+            job.host = src
+            co.submit(job)
+            return job.result
         phid = ctx.empty_phi(src.shape)
         phid.copy_(t.from_numpy(src))
     else:
         phid = phi
-    co = getattr(_local, "coalescer", None)
     if co is not None and len(seg) and frozen != (1 << phid.dim()) - 1:
         job = Job(phid, xx, seg, order, frozen, inject, delj=delj)
         co.submit(job)
@@ -478,13 +486,18 @@ def run_single(phi, xx, seg, order, frozen, inject, ctx=None, mode="auto", waves
 
 
 class Job:
-    """One integration of a batch: device phi (updated in place) plus schedule."""
+    """One integration of a batch: device phi (updated in place) plus schedule.
+    A job submitted to a Coalescer may instead carry a host array (``host``,
+    phi None): the launch uploads all such arrays at once and hands back
+    ``result``, the integrated host array."""
     __slots__ = ("phi", "xx", "seg", "order", "frozen", "inject", "tag", "done", "delj",
-                 "error")
+                 "error", "host", "result")
 
     def __init__(self, phi, xx, seg, order, frozen, inject, tag=None, delj=False):
         self.done = False
         self.error = None
+        self.host = None
+        self.result = None
         self.delj = bool(delj)
         self.phi = phi
         self.xx = np.asarray(xx, dtype=np.float64)
@@ -496,11 +509,11 @@ class Job:
 
     @property
     def D(self):
-        return self.phi.dim()
+        return (self.phi if self.phi is not None else self.host).ndim
 
     @property
     def n(self):
-        return int(self.phi.shape[0])
+        return int((self.phi if self.phi is not None else self.host).shape[0])
 
     def work(self):
         act = sum(1 for k in range(self.D) if not (self.frozen >> k) & 1)
@@ -782,7 +795,15 @@ class Coalescer:
             try:
                 if self._ctx is None:
                     self._ctx = context()
-                run_batch(jobs, self._ctx)
+                samp = [j for j in jobs if isinstance(j, SampleJob)]
+                ints = [j for j in jobs if not isinstance(j, SampleJob)]
+                if ints:
+                    staged = _upload_host_jobs(self._ctx, ints)
+                    run_batch(ints, self._ctx)
+                    _download_host_jobs(staged)
+                if samp:
+                    from .Spectrum_mod import sample_jobs
+                    sample_jobs(samp, self._ctx)
             except BaseException as e:  # noqa: BLE001  (raised in every worker)
                 err = e
             for j in jobs:
@@ -790,6 +811,46 @@ class Coalescer:
                 j.done = True
             self.launches += 1
             self.cv.notify_all()
+
+
+class SampleJob:
+    """A phi -> spectrum sampling submitted to a Coalescer: host phi in, host
+    spectrum data (``result``) out; the launch samples all waiting phis of
+    one shape, sample sizes and grid together."""
+    __slots__ = ("phi", "ns", "xx", "done", "error", "result")
+
+    def __init__(self, phi, ns, xx):
+        self.phi, self.ns, self.xx = phi, tuple(int(v) for v in ns), xx
+        self.done = False
+        self.error = None
+        self.result = None
+
+
+def _upload_host_jobs(ctx, jobs):
+    """Device phis for the jobs that carry host arrays: one arena and one
+    host-to-device copy per shape."""
+    t = torch()
+    groups = {}
+    for j in jobs:
+        if getattr(j, "host", None) is not None and j.phi is None:
+            groups.setdefault(j.host.shape, []).append(j)
+    staged = []
+    for shape, grp in groups.items():
+        arena = ctx.empty_phi(shape, batch=len(grp))
+        arena.copy_(t.from_numpy(np.stack([j.host for j in grp])))
+        for i, j in enumerate(grp):
+            j.phi = arena[i]
+        staged.append((arena, grp))
+    return staged
+
+
+def _download_host_jobs(staged):
+    """The integrated arrays back to the host, one copy per arena."""
+    for arena, grp in staged:
+        res = arena.cpu().numpy()
+        for i, j in enumerate(grp):
+            j.result = np.array(res[i])
+            j.phi = None
 
 
 #: map_in_worker runs its items one after another once a coalescer has this

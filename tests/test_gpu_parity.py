@@ -619,3 +619,22 @@ def test_large_factored_batch_equals_packed_kernel(dadi_amd, oracle):
         T, nus, m = specs[i][1]
         ref = oracle.integrate_const(phi0, xx, T, nus, [[m], [m]], [0, 0], [0.5, 0.5])
         assert np.array_equal(got[i], ref), i
+
+
+@pytest.mark.parametrize("D,n,ns", [(1, 50, (12,)), (2, 30, (8, 11)), (3, 12, (5, 4, 6))])
+def test_batched_sampling_equals_single(dadi_amd, D, n, ns):
+    """Phis sampled together (a leading batch axis, as the coalescer does for
+    concurrent model evaluations) give each phi the bits it gets alone, and
+    the coalescer's host-array path gives the bits of the device path."""
+    from dadi_amd import Spectrum_mod, engine
+    rng = np.random.default_rng(D * 100 + n)
+    xx = dadi_amd.Numerics.default_grid(n)
+    phis = rng.random((5,) + (n,) * D)
+    alone = [Spectrum_mod.sample_device(p, ns, xx).cpu().numpy() for p in phis]
+    together = Spectrum_mod.sample_device(phis, ns, xx, batched=True).cpu().numpy()
+    for a, b in zip(alone, together):
+        assert np.array_equal(a, b)
+    got = engine.map_batched(lambda p: dadi_amd.Spectrum.from_phi(p, ns, [xx] * D).data,
+                             [(p,) for p in phis], workers=5)
+    for a, b in zip(alone, got):
+        assert np.array_equal(a, b)
