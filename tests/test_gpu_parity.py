@@ -638,3 +638,45 @@ def test_batched_sampling_equals_single(dadi_amd, D, n, ns):
                              [(p,) for p in phis], workers=5)
     for a, b in zip(alone, got):
         assert np.array_equal(a, b)
+
+
+def test_large_spread_batch_uses_checkpoints_and_matches(dadi_amd, oracle):
+    """A 3-population batch whose spread launch needs more workgroups than
+    there are compute units runs the checkpointed sweep; every integration
+    equals the one-workgroup kernel's, and spot checks equal the oracle."""
+    import torch
+    from dadi_amd import engine, schedule as S, _lib
+    ctx = engine.context()
+    n, B = 24, 128
+    xx = oracle.default_grid(n)
+    p = oracle.phi_2D_to_3D_split_2(xx, oracle.phi_1D_to_2D(xx, oracle.phi_1D(xx)))
+    rng = np.random.default_rng(31)
+    specs = []
+    for _ in range(B):
+        nus = list(np.exp(rng.uniform(np.log(0.3), np.log(3), 3)))
+        migs = [list(rng.uniform(0, 2, 2)) for _ in range(3)]
+        T = float(rng.uniform(0.005, 0.03))
+        dt = min(S.compute_dt(nus[k], migs[k], 0, 0.5) for k in range(3))
+        specs.append((S.const_segments(3, xx, 0.0, T, dt, nus, migs, [0] * 3, [0.5] * 3, 1.0),
+                      (T, nus, migs)))
+    fz, inj = S.masks(3, [False] * 3, [False] * 3)
+    tiles = B * -(-n * n // 64)
+    assert tiles > 2 * ctx.num_cu          # four waves per workgroup
+    assert -(-tiles // 4) > ctx.num_cu      # more workgroups than compute units
+    old = ctx.spread_max_inst
+    try:
+        out = {}
+        for spread in (512, 0):
+            ctx.spread_max_inst = spread
+            jobs = [engine.Job(ctx.to_dev(p), xx, sp[0], _lib.ORDER_PRE, fz, inj) for sp in specs]
+            engine.run_batch(jobs)
+            torch.cuda.synchronize()
+            out[spread] = [j.phi.cpu().numpy() for j in jobs]
+        for a, b in zip(out[512], out[0]):
+            assert np.array_equal(a, b)
+        for i in (0, B - 1):
+            T, nus, migs = specs[i][1]
+            ref = oracle.integrate_const(p, xx, T, nus, migs, [0] * 3, [0.5] * 3)
+            assert np.array_equal(out[512][i], ref), i
+    finally:
+        ctx.spread_max_inst = old
