@@ -680,3 +680,57 @@ def test_large_spread_batch_uses_checkpoints_and_matches(dadi_amd, oracle):
             assert np.array_equal(out[512][i], ref), i
     finally:
         ctx.spread_max_inst = old
+
+
+@pytest.mark.parametrize("order", ["pre", "tv"])
+def test_mixed_factored_batch_equals_workgroup_kernel(dadi_amd, oracle, order):
+    """A 2D batch of 2048 integrations on grids of 20-31 points (mixed in one
+    launch), with frozen populations and populations without new mutations
+    mixed in, takes the factored kernel automatically; every result equals
+    the one-workgroup kernel's."""
+    import torch
+    from dadi_amd import engine, schedule as S, _lib
+    from dadi_amd.Misc import ensure_1arg_func as f1
+    ctx = engine.context()
+    B = 2048
+    assert B <= ctx.factored_max_inst
+    rng = np.random.default_rng(2048 + (order == "tv"))
+    grids = {n: oracle.default_grid(n) for n in range(20, 32)}
+    phis = {n: oracle.phi_1D_to_2D(xx, oracle.phi_1D(xx)) for n, xx in grids.items()}
+    specs = []
+    for i in range(B):
+        n = int(rng.integers(20, 32))
+        xx = grids[n]
+        nus = list(np.exp(rng.uniform(np.log(0.3), np.log(3), 2)))
+        migs = [[float(rng.uniform(0, 2))], [float(rng.uniform(0, 2))]]
+        gams = list(rng.uniform(-4, 4, 2))
+        frozen = [bool(i % 7 == 3), False]
+        nomut = [False, bool(i % 5 == 1)]
+        if frozen[0]:
+            migs = [[0.0], [0.0]]
+        T = float(rng.uniform(0.01, 0.1))
+        fz, inj = S.masks(2, frozen, nomut)
+        if order == "tv":
+            seg = S.tv_segments(2, xx, 0.0, T, [f1(v) for v in nus],
+                                [[f1(m) for m in ms] for ms in migs], [f1(g) for g in gams],
+                                [f1(0.5)] * 2, f1(1.0), None)
+            o = _lib.ORDER_TV
+        else:
+            dt = min(S.compute_dt(nus[k], migs[k], gams[k], 0.5) for k in range(2))
+            seg = S.const_segments(2, xx, 0.0, T, dt, nus, migs, gams, [0.5, 0.5], 1.0)
+            o = _lib.ORDER_PRE
+        specs.append((n, xx, seg, o, fz, inj))
+
+    def run(**kw):
+        jobs = [engine.Job(ctx.to_dev(phis[n]), xx, seg, o, fz, inj)
+                for n, xx, seg, o, fz, inj in specs]
+        engine.run_batch(jobs, **kw)
+        torch.cuda.synchronize()
+        return jobs
+
+    auto = run()
+    pb = engine.PackedBatch(ctx, [j for j in auto])
+    assert pb.latency_bound()
+    ref = run(slots=1)
+    for i, (a, b) in enumerate(zip(auto, ref)):
+        assert np.array_equal(a.phi.cpu().numpy(), b.phi.cpu().numpy()), i
