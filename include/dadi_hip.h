@@ -207,8 +207,13 @@ int dh_integrate_1d_wave(dh_instance *table_dev, int ninst, int order, int nmax,
  * per-step operations into a workspace, and a step only solves the right-hand
  * sides: a few dependent operations per row instead of the coefficient build
  * and three divisions.  Results are the bits of dh_integrate_nd.  More memory
- * traffic per point than dh_integrate_nd, for few instances at a time (a
- * single integration, a serving loop).  cfg->gamma_dev / gamma_doubles is the
+ * traffic per point than dh_integrate_nd, so it is for batches whose time is
+ * set by their longest integrations rather than by bandwidth: a single
+ * integration, a serving loop, 2D batches of up to a few thousand
+ * integrations (on MI355X faster than dh_integrate_packed up to 2048, slower
+ * at 8192).  Launching two workgroups per compute unit pays for large
+ * batches on small grids (workspaces staying in the last-level cache).
+ * cfg->gamma_dev / gamma_doubles is the
  * workspace: groups * dh_factored_slice_doubles(D, nmax) doubles, groups =
  * cfg->groups if > 0, else min(ninst, compute units).  cfg->waves 1..8;
  * cfg->max_steps and cfg->max_us as for dh_integrate_nd; no delta trick.
