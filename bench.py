@@ -70,6 +70,44 @@ def _cpu_share(task):
     return done, steps, time.monotonic() - t0
 
 
+#: host cores a one-GPU job may keep busy on the GPU boxes (their CPU share per GPU)
+CPU_SHARE_PER_GPU = 16
+
+
+def host_cores():
+    """What the host offers this process: os.cpu_count(), the scheduler
+    affinity mask, the cgroup CPU quota (cores, None without one), and the
+    processes the CPU baseline runs: one per core it may use, i.e. the
+    smallest of the affinity, the quota and the per-GPU CPU share."""
+    count = os.cpu_count() or 1
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        aff = count
+    quota = None
+    for path, parse in (("/sys/fs/cgroup/cpu.max", lambda t: t.split()),
+                        ("/sys/fs/cgroup/cpu/cpu.cfs_quota_us", None)):
+        try:
+            txt = open(path).read().strip()
+        except OSError:
+            continue
+        if parse:
+            q, per = parse(txt)
+            if q != "max":
+                quota = float(q) / float(per)
+        else:
+            q = float(txt)
+            per = float(open("/sys/fs/cgroup/cpu/cpu.cfs_period_us").read())
+            if q > 0:
+                quota = q / per
+        break
+    use = min(aff, CPU_SHARE_PER_GPU)
+    if quota:
+        use = min(use, max(1, int(quota)))
+    return {"os_cpu_count": count, "affinity": aff, "cgroup_quota_cores": quota,
+            "share_per_gpu": CPU_SHARE_PER_GPU, "used": use}
+
+
 def cpu_baseline(params, pts, budget_s, procs):
     """The reference's own C kernels (oracle/_ref, compiled from dadi's
     sources) driven by a restatement of dadi's constant-parameter Python loop,
@@ -211,7 +249,8 @@ def main():
                     help="persistent workgroups (0 = as many as fit)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-procs", type=int, default=0,
-                    help="CPU baseline processes (0 = min(cores, 16))")
+                    help="CPU baseline processes (0 = one per core this job may use: the "
+                         "affinity mask, capped by the cgroup quota and the per-GPU CPU share)")
     ap.add_argument("--evals", type=int, default=512,
                     help="split_mig model evaluations (3 grids each) through the public API")
     ap.add_argument("--evals3d", type=int, default=96,
@@ -256,8 +295,10 @@ def main():
     cpu = None
     if not args.no_cpu:
         # before anything initialises the GPU: the baseline forks its processes
-        procs = args.cpu_procs or min(os.cpu_count() or 1, 16)
+        hc = host_cores()
+        procs = args.cpu_procs or hc["used"]
         cpu = cpu_baseline(params, args.pts, args.cpu_seconds, procs)
+        cpu["host"] = hc
 
     import torch
     import dadi_amd

@@ -83,8 +83,11 @@ class Cache1D:
 
     def integrate_point_pos(self, params, ns, sel_dist, theta, demo_sel_func=None, Npos=1,
                             pts=None, exterior_int=True):
-        """integrate() plus Npos point masses of positive selection
-        (dadi/DFE/Cache1D_mod.py:232-285)."""
+        """(1 - sum ppos) x integrate() plus Npos point masses of positive selection,
+        each its cached spectrum times ppos (dadi/DFE/Cache1D_mod.py:232-285).  A
+        positive gamma that is not cached is evaluated with demo_sel_func (on the
+        GPU) and appended to the cache, multiplied by theta as the reference
+        stores it; without demo_sel_func it raises IndexError."""
         pdf_params = params[:-2 * Npos]
         ppos_l, gammapos_l = params[-2 * Npos::2], params[-2 * Npos + 1::2]
         pdf_fs = self.integrate(pdf_params, None, sel_dist, theta, None,
@@ -95,8 +98,10 @@ class Cache1D:
                 if demo_sel_func is None:
                     raise IndexError('Failed to find requested gammapos={0:.4f} in Cache1D '
                                      'spectra.'.format(gammapos))
+                # as the reference (dadi/DFE/Cache1D_mod.py:275-279), a spectrum
+                # computed here is stored multiplied by theta
                 ex = Numerics.make_extrap_func(demo_sel_func)
-                pos_fs = ex(tuple(self.params) + (gammapos,), self.ns, self.pts)
+                pos_fs = theta * ex(tuple(self.params) + (gammapos,), self.ns, self.pts)
                 self.gammas = np.append(self.gammas, gammapos)
                 self.spectra = np.append(self.spectra, [pos_fs.data], axis=0)
             ii = list(self.gammas).index(gammapos)

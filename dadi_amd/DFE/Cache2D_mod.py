@@ -39,14 +39,27 @@ class Cache2D:
 
     @staticmethod
     def merge(caches):
-        """Combine caches generated with split_jobs (dadi/DFE/Cache2D_mod.py:469-505)."""
-        out = caches[0]
+        """Combine caches generated with split_jobs (dadi/DFE/Cache2D_mod.py:469-505):
+        entries two caches both hold must agree, and the result must be complete
+        (ValueError otherwise).  The first cache is copied, not modified."""
+        import copy
+        out = copy.deepcopy(caches[0])
         ng = len(out.gammas)
         for c in caches[1:]:
             for i in range(ng):
                 for j in range(ng):
-                    if out.spectra[i][j] is None:
-                        out.spectra[i][j] = c.spectra[i][j]
+                    fs = c.spectra[i][j]
+                    if fs is None:
+                        continue
+                    have = out.spectra[i][j]
+                    if have is not None and not np.all(np.asarray(have) == np.asarray(fs)):
+                        raise ValueError("Merged cached conflicts with current.")
+                    out.spectra[i][j] = fs
+        for i in range(ng):
+            for j in range(ng):
+                if out.spectra[i][j] is None:
+                    raise ValueError("Cache is incomplete after merging. First missing entry "
+                                     "is {0},{1}.".format(i, j))
         out.spectra = np.array(out.spectra)
         return out
 
@@ -100,3 +113,91 @@ class Cache2D:
                                            args=[params])
         fs = fs + spectra[-1, 0] * w
         return Spectrum(theta * fs)
+
+    def integrate_point_pos(self, params, ns, biv_seldist, theta, rho=0, pts=None):
+        """Spectrum of a DFE made of four quadrants (dadi/DFE/Cache2D_mod.py:312-430):
+        both gammas negative (integrate() of the bivariate density biv_seldist,
+        exterior terms included, at theta 1), both positive (the cached
+        pair (gammapos1, gammapos2)), and one of each (the cached row or column of
+        the positive gamma weighted by the other population's marginal density).
+        params ends with (ppos1, gammapos1, ppos2, gammapos2); the quadrant
+        weights are p1 p2 + rho (sqrt(p1 p2) - p1 p2), (1-rho) p1 (1-p2),
+        (1-rho) (1-p1) p2 and (1-p1)(1-p2) + rho (1 - sqrt(p1 p2) - (1-p1)(1-p2)).
+        No normalisation: mass outside the cached gammas is taken as unseen.
+        Raises IndexError when a positive gamma is not among the cached ones."""
+        biv_params = params[:-4]
+        ppos1, gammapos1, ppos2, gammapos2 = params[-4:]
+        Nneg = len(self.neg_gammas)
+        weights = biv_seldist(-self.neg_gammas, -self.neg_gammas, biv_params)
+        neg_neg = self.integrate(biv_params, ns, biv_seldist, 1, pts)
+        i1 = np.flatnonzero(self.gammas == gammapos1)
+        i2 = np.flatnonzero(self.gammas == gammapos2)
+        if len(i1) == 0 or len(i2) == 0:
+            raise IndexError('Failed to find requested gammapos1={0:.4f} and/or '
+                             'gammapos2={1:0.4f} in cached spectra. Were they included in '
+                             'additional_gammas during cache generation?'
+                             .format(gammapos1, gammapos2))
+        i1, i2 = int(i1[0]), int(i2[0])
+        pos_pos = self.spectra[i1, i2]
+        # population 1 positive: its cached row, over population 2's marginal
+        marg2 = _trapz(weights, self.neg_gammas, axis=0)
+        pos_neg = _trapz(marg2[:, None, None] * self.spectra[i1, :Nneg], self.neg_gammas,
+                         axis=0)
+        # population 2 positive: its cached column, over population 1's marginal
+        marg1 = _trapz(weights, self.neg_gammas, axis=1)
+        neg_pos = _trapz(marg1[:, None, None] * self.spectra[:Nneg, i2], self.neg_gammas,
+                         axis=0)
+        both = ppos1 * ppos2
+        w_pp = both + rho * (np.sqrt(both) - both)
+        w_pn = (1 - rho) * ppos1 * (1 - ppos2)
+        w_np = (1 - rho) * (1 - ppos1) * ppos2
+        w_nn = (1 - ppos1) * (1 - ppos2) + rho * (1 - np.sqrt(both) - (1 - ppos1) * (1 - ppos2))
+        fs = w_pp * pos_pos + w_pn * pos_neg + w_np * neg_pos + w_nn * neg_neg
+        return theta * fs
+
+    def integrate_symmetric_point_pos(self, params, ns, biv_seldist, theta, pts=None):
+        """integrate_point_pos with the same positive point mass (ppos, gammapos)
+        in both populations; the last density parameter is the correlation rho
+        (dadi/DFE/Cache2D_mod.py:432-467)."""
+        dens = list(params[:-2])
+        ppos, gammapos = params[-2:]
+        return self.integrate_point_pos(dens + [ppos, gammapos, ppos, gammapos], ns,
+                                        biv_seldist, theta, rho=dens[-1], pts=None)
+
+
+def mixture(params, ns, s1, s2, sel_dist1, sel_dist2, theta, pts, exterior_int=True):
+    """(1 - p2d) x the Cache1D spectrum (perfectly correlated gammas) + p2d x the
+    Cache2D spectrum; params = shared density parameters, rho, p2d
+    (dadi/DFE/Cache2D_mod.py:509-547)."""
+    fs1 = s1.integrate(params[:-2], None, sel_dist1, theta, None, exterior_int)
+    fs2 = s2.integrate(params[:-1], None, sel_dist2, theta, None, exterior_int)
+    p2d = params[-1]
+    return (1 - p2d) * fs1 + p2d * fs2
+
+
+def mixture_symmetric_point_pos(params, ns, s1, s2, sel_dist1, sel_dist2, theta, pts=None):
+    """mixture with a symmetric positive point mass; params = shared density
+    parameters, rho, ppos, gammapos, p2d (dadi/DFE/Cache2D_mod.py:549-587).
+    The Cache2D part receives (density parameters, rho, ppos, gammapos, ppos,
+    gammapos), exactly the arguments the reference passes, so its results are
+    the reference's."""
+    dens = list(params[:-4])
+    rho, ppos, gammapos, p2d = params[-4:]
+    fs1 = s1.integrate_point_pos(dens + [ppos, gammapos], None, sel_dist1, theta, Npos=1,
+                                 pts=None)
+    fs2 = s2.integrate_symmetric_point_pos(dens + [rho, ppos, gammapos, ppos, gammapos], None,
+                                           sel_dist2, theta, None)
+    return (1 - p2d) * fs1 + p2d * fs2
+
+
+def mixture_point_pos(params, ns, s1, s2, sel_dist1, sel_dist2, theta, pts=None):
+    """mixture with positive point masses (ppos1, gammapos1) and (ppos2,
+    gammapos2); params = shared density parameters, rho, ppos1, gammapos1,
+    ppos2, gammapos2, p2d (dadi/DFE/Cache2D_mod.py:589-631)."""
+    dens = list(params[:-6])
+    rho, ppos1, gammapos1, ppos2, gammapos2, p2d = params[-6:]
+    fs1 = s1.integrate_point_pos(dens + [ppos1, gammapos1], None, sel_dist1, theta, Npos=1,
+                                 pts=None)
+    fs2 = s2.integrate_point_pos(dens + [rho, ppos1, gammapos1, ppos2, gammapos2], None,
+                                 sel_dist2, theta, None)
+    return (1 - p2d) * fs1 + p2d * fs2

@@ -33,25 +33,36 @@ def normal(xx, mu, sigma):
     return ssd.norm.pdf(xx, loc=mu, scale=sigma)
 
 
-def biv_lognormal(xx, yy, params):
-    """Bivariate lognormal density on the grid xx (rows) x yy (columns);
-    params (mu, sigma, rho) or (mu1, mu2, sigma1, sigma2, rho)
-    (dadi/DFE/PDFs.py biv_lognormal_py)."""
-    xx = np.atleast_1d(xx)
-    yy = np.atleast_1d(yy)
+def _biv_lognormal_params(params):
+    """(mu1, mu2, sigma1, sigma2, rho) from (mu, sigma, rho) or the five values."""
     if len(params) == 3:
-        mu, sigma, rho = params
-        mu1 = mu2 = mu
-        sigma1 = sigma2 = sigma
-    elif len(params) == 5:
-        mu1, mu2, sigma1, sigma2, rho = params
-    else:
-        raise ValueError('Parameter array for bivariate lognormal must have length 3 or 5.')
-    delx = (np.log(xx[:, np.newaxis]) - mu1) / sigma1
-    dely = (np.log(yy[np.newaxis, :]) - mu2) / sigma2
-    norm = 2 * np.pi * sigma1 * sigma2 * np.sqrt(1. - rho ** 2) * np.outer(xx, yy)
-    q = (delx ** 2 - 2. * rho * delx * dely + dely ** 2) / (1. - rho ** 2)
-    return np.squeeze(np.exp(-q / 2.) / norm)
+        return params[0], params[0], params[1], params[1], params[2]
+    if len(params) == 5:
+        return tuple(params)
+    raise ValueError('Parameter array for bivariate lognormal must have length 3 or 5.')
+
+
+def biv_lognormal(xx, yy, params):
+    """Density of (X, Y) = (exp U, exp V) with (U, V) bivariate normal, on the
+    grid xx (rows) x yy (columns); params (mu, sigma, rho) or (mu1, mu2,
+    sigma1, sigma2, rho), as dadi/DFE/PDFs.py:41-43 takes them.
+
+    Written through the standardised coordinates s = (ln x - mu1)/sigma1 and
+    w = ((ln y - mu2)/sigma2 - rho s)/sqrt(1 - rho^2), which are independent
+    standard normals: f(x, y) = phi(s) phi(w) / (sigma1 sigma2 sqrt(1-rho^2) x y),
+    with phi the standard normal density (the Jacobian of (x, y) -> (s, w)).
+    Scalar arguments give a 0-d result."""
+    mu1, mu2, sigma1, sigma2, rho = _biv_lognormal_params(params)
+    x = np.atleast_1d(np.asarray(xx, dtype=float))
+    y = np.atleast_1d(np.asarray(yy, dtype=float))
+    with np.errstate(divide='ignore', invalid='ignore'):
+        s = (np.log(x) - mu1) / sigma1
+        t = (np.log(y) - mu2) / sigma2
+        root = np.sqrt(1. - rho * rho)
+        w = (t[None, :] - rho * s[:, None]) / root
+        jac = (sigma1 * x)[:, None] * (sigma2 * y)[None, :] * root
+        dens = ssd.norm.pdf(s)[:, None] * ssd.norm.pdf(w) / jac
+    return np.squeeze(dens)
 
 
 def biv_ind_gamma(xx, yy, params):

@@ -17,10 +17,27 @@ def _masked(a):
     return numpy.ma.masked_array(numpy.asarray(a, dtype=float))
 
 
+_FOLDS = ('folded', 'folded_ancestral', 'folded_major')
+
+
+def _refuse_folded(model, data):
+    """The reference folds an unfolded model to match folded data before
+    comparing them (dadi/Inference.py:493-501, 676-683).  Spectrum folding is not
+    part of this package, so such a comparison raises instead of returning a
+    likelihood of mismatched spectra."""
+    for attr in _FOLDS:
+        if getattr(data, attr, False) and not getattr(model, attr, False):
+            raise NotImplementedError(
+                'data spectrum is %s but the model is not; folding the model '
+                '(Spectrum.fold*) is not implemented in dadi_amd' % attr.replace('_', ' '))
+
+
 def ll_per_bin(model, data, missing_model_cutoff=1e-6):
     """Poisson log-likelihood of each entry: -model + data*log(model) - lnGamma(data+1),
     masked where either spectrum is masked or the model is not positive
-    (dadi/Inference.py:482-548)."""
+    (dadi/Inference.py:482-548).  Folded data with an unfolded model raises
+    NotImplementedError."""
+    _refuse_folded(model, data)
     model = _masked(model)
     data = _masked(data)
     result = -model.data + data.data * numpy.ma.log(model) - gammaln(data + 1.)
@@ -43,7 +60,9 @@ def ll(model, data):
 
 
 def optimal_sfs_scaling(model, data):
-    """sum(data)/sum(model) over entries masked in neither (dadi/Inference.py:668-685)."""
+    """sum(data)/sum(model) over entries masked in neither (dadi/Inference.py:668-685).
+    Folded data with an unfolded model raises NotImplementedError."""
+    _refuse_folded(model, data)
     model = _masked(model)
     data = _masked(data)
     mask = numpy.ma.getmaskarray(model) | numpy.ma.getmaskarray(data)

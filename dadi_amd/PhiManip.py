@@ -273,3 +273,85 @@ def phi_3D_to_4D_device(phi, f1, f2, xx, yy, zz, aa):
     z = _dev_grid(zz, phi)
     ad_w = f1 * x[:, None, None] + f2 * y[None, :, None] + (1 - f1 - f2) * z[None, None, :]
     return _new_axis_device(phi, ad_w, aa)
+
+
+def phi_2D_to_3D_split_1_device(xx, phi_2D):
+    """phi_2D_to_3D_split_1 for a device tensor."""
+    return phi_2D_to_3D_admix_device(phi_2D, 1, xx, xx, xx)
+
+
+def phi_4D_to_5D_device(phi, f1, f2, f3, xx, yy, zz, aa, bb):
+    """phi_4D_to_5D for a device tensor."""
+    if f1 + f2 + f3 > 1:
+        raise ValueError('Admixture proportions are non-sensible.')
+    x = _dev_grid(xx, phi)
+    y = _dev_grid(yy, phi)
+    z = _dev_grid(zz, phi)
+    a = _dev_grid(aa, phi)
+    ad_w = f1 * x[:, None, None, None] + f2 * y[None, :, None, None] \
+        + f3 * z[None, None, :, None] + (1 - f1 - f2 - f3) * a[None, None, None, :]
+    return _new_axis_device(phi, ad_w, bb)
+
+
+def _pairwise_last(a):
+    """Sum over the last axis in the order numpy's pairwise summation uses for
+    contiguous doubles: fewer than 8 terms one after another from 0; up to 128
+    terms in 8 interleaved partial sums combined as ((r0+r1)+(r2+r3)) +
+    ((r4+r5)+(r6+r7)), then the remainder; longer runs split in two at a
+    multiple of 8 below the half."""
+    n = a.shape[-1]
+    if n < 8:
+        import torch
+        res = torch.zeros_like(a[..., 0])
+        for i in range(n):
+            res = res + a[..., i]
+        return res
+    if n <= 128:
+        r = [a[..., j] for j in range(8)]
+        i = 8
+        while i < n - n % 8:
+            r = [r[j] + a[..., i + j] for j in range(8)]
+            i += 8
+        res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]))
+        while i < n:
+            res = res + a[..., i]
+            i += 1
+        return res
+    n2 = n // 2
+    n2 -= n2 % 8
+    return _pairwise_last(a[..., :n2]) + _pairwise_last(a[..., n2:])
+
+
+def _trapz_device(phi, xx, axis):
+    """Numerics.trapz(phi, xx, axis) on a device tensor with numpy's operations
+    and summation order: the last axis is summed pairwise, any other axis
+    term after term, so results are bit-identical to the host version."""
+    x = _dev_grid(xx, phi)
+    dx = x[1:] - x[:-1]
+    n = phi.shape[axis]
+    shape = [1] * phi.dim()
+    shape[axis] = n - 1
+    terms = dx.reshape(shape) * (phi.narrow(axis, 1, n - 1) + phi.narrow(axis, 0, n - 1)) / 2.0
+    if axis == phi.dim() - 1:
+        return _pairwise_last(terms)
+    terms = terms.movedim(axis, 0)
+    acc = terms[0]
+    for k in range(1, n - 1):
+        acc = acc + terms[k]
+    return acc.contiguous()
+
+
+def remove_pop_device(phi, xx, popnum):
+    """remove_pop for a device tensor (dadi/PhiManip.py:1036-1050); no
+    Demes.cache event."""
+    return _trapz_device(phi, xx, popnum - 1)
+
+
+def filter_pops_device(phi, xx, tokeep):
+    """filter_pops for a device tensor (dadi/PhiManip.py:1052-1072)."""
+    toremove = list(range(1, phi.dim() + 1))
+    for pop_ii in tokeep:
+        toremove.remove(pop_ii)
+    for pop_ii in sorted(toremove)[::-1]:
+        phi = remove_pop_device(phi, xx, pop_ii)
+    return phi

@@ -25,8 +25,10 @@
 #include <cstdint>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "dadi_hip.h"
 
@@ -2617,8 +2619,109 @@ int set_lds(const void *fn, size_t bytes)
     return DH_OK;
 }
 
+// Cache of instantiated step graphs, keyed by everything captured into them.
+// A caller holds a reference to its entry from the lookup until its
+// launches are enqueued, so eviction never destroys an executable that is
+// about to be launched.  After launching, the caller records an event on its
+// stream (one event per stream: launches on a stream complete in order).
+// Evicted entries wait in a list and are destroyed, at a later insertion,
+// once no caller holds them and every recorded event has completed; nothing
+// here waits for the device.
+struct GraphEntry {
+    hipGraphExec_t exec = nullptr;
+    unsigned long long stamp = 0;          // last use, for least-recently-used eviction
+    std::mutex mu;                         // guards streams/events
+    std::vector<hipStream_t> streams;
+    std::vector<hipEvent_t> events;
+    GraphEntry() = default;
+    GraphEntry(const GraphEntry &) = delete;
+    GraphEntry &operator=(const GraphEntry &) = delete;
+    ~GraphEntry()
+    {
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
+        if (exec) (void)hipGraphExecDestroy(exec);
+    }
+    // completion marker of this caller's launches on st
+    hipError_t note_launch(hipStream_t st)
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        for (size_t i = 0; i < streams.size(); ++i)
+            if (streams[i] == st) return hipEventRecord(events[i], st);
+        hipEvent_t e;
+        hipError_t ec = hipEventCreateWithFlags(&e, hipEventDisableTiming);
+        if (ec != hipSuccess) return ec;
+        streams.push_back(st);
+        events.push_back(e);
+        return hipEventRecord(e, st);
+    }
+    // every launch recorded so far has completed (no waiting)
+    bool idle()
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        for (hipEvent_t e : events)
+            if (hipEventQuery(e) != hipSuccess) return false;
+        return true;
+    }
+};
+using GraphRef = std::shared_ptr<GraphEntry>;
+
+constexpr size_t DH_GRAPH_CACHE = 64;
 std::mutex g_graph_mu;
-std::map<std::string, hipGraphExec_t> g_graphs;
+std::map<std::string, GraphRef> g_graphs;
+std::vector<GraphRef> g_graph_retired;
+unsigned long long g_graph_clock = 0;
+
+// Entry for `key`, or null; the returned reference keeps it alive.
+GraphRef graph_lookup(const std::string &key)
+{
+    std::lock_guard<std::mutex> lk(g_graph_mu);
+    auto it = g_graphs.find(key);
+    if (it == g_graphs.end()) return nullptr;
+    it->second->stamp = ++g_graph_clock;
+    return it->second;
+}
+
+// Adds a freshly instantiated executable (the cache owns it from here on) and
+// returns its entry; when another thread inserted the same key meanwhile,
+// that entry is returned and `exec` destroyed (it was never launched).
+GraphRef graph_insert(const std::string &key, hipGraphExec_t exec)
+{
+    GraphRef mine = std::make_shared<GraphEntry>();
+    mine->exec = exec;
+    std::vector<GraphRef> drop;   // destroyed after the lock is released
+    GraphRef out;
+    {
+        std::lock_guard<std::mutex> lk(g_graph_mu);
+        // destroy retired entries nobody holds whose launches have completed
+        for (size_t i = 0; i < g_graph_retired.size();) {
+            GraphRef &r = g_graph_retired[i];
+            if (r.use_count() == 1 && r->idle()) {
+                drop.push_back(std::move(r));
+                r = std::move(g_graph_retired.back());
+                g_graph_retired.pop_back();
+            } else {
+                ++i;
+            }
+        }
+        auto it = g_graphs.find(key);
+        if (it != g_graphs.end()) {
+            out = it->second;
+            drop.push_back(std::move(mine));
+        } else {
+            while (g_graphs.size() >= DH_GRAPH_CACHE) {
+                auto lru = g_graphs.begin();
+                for (auto j = g_graphs.begin(); j != g_graphs.end(); ++j)
+                    if (j->second->stamp < lru->second->stamp) lru = j;
+                g_graph_retired.push_back(std::move(lru->second));
+                g_graphs.erase(lru);
+            }
+            out = mine;
+            g_graphs.emplace(key, mine);
+        }
+        out->stamp = ++g_graph_clock;
+    }
+    return out;
+}
 
 }  // namespace
 
@@ -2904,13 +3007,9 @@ int dh_integrate_graph_range(const dh_instance *inst_host, long long *step_base_
     key.append((const char *)&total_steps, sizeof(total_steps));
     key.append((const char *)&nsteps_per_graph, sizeof(int));
     key.append((const char *)&waves, sizeof(int));
-    hipGraphExec_t exec = nullptr;
-    {
-        std::lock_guard<std::mutex> lk(g_graph_mu);
-        auto it = g_graphs.find(key);
-        if (it != g_graphs.end()) exec = it->second;
-    }
-    if (!exec) {
+    GraphRef entry = graph_lookup(key);
+    if (!entry) {
+        hipGraphExec_t exec = nullptr;
         hipStream_t cap;
         DH_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
         DH_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
@@ -2938,19 +3037,13 @@ int dh_integrate_graph_range(const dh_instance *inst_host, long long *step_base_
         (void)hipGraphDestroy(graph);
         if (ei != hipSuccess)
             return fail(DH_EHIP, std::string("instantiate: ") + hipGetErrorString(ei));
-        std::lock_guard<std::mutex> lk(g_graph_mu);
-        if (g_graphs.size() > 64) {
-            // replays of these graphs may still be queued on other streams
-            if (hipDeviceSynchronize() != hipSuccess) return fail(DH_EHIP, "device sync");
-            for (auto &kv : g_graphs) (void)hipGraphExecDestroy(kv.second);
-            g_graphs.clear();
-        }
-        g_graphs[key] = exec;
+        entry = graph_insert(key, exec);
     }
     hipLaunchKernelGGL(k_set_ll, dim3(1), dim3(1), 0, st, step_base_dev, first_step);
     const long long last = first_step + nsteps < total_steps ? first_step + nsteps : total_steps;
     const long long reps = (last - first_step + nsteps_per_graph - 1) / nsteps_per_graph;
-    for (long long r = 0; r < reps; ++r) DH_HIP(hipGraphLaunch(exec, st));
+    for (long long r = 0; r < reps; ++r) DH_HIP(hipGraphLaunch(entry->exec, st));
+    DH_HIP(entry->note_launch(st));
     return DH_OK;
 }
 
@@ -2994,13 +3087,9 @@ int dh_integrate_batch(const dh_sweep_item *items_dev, const int *wg_item_dev, i
     key.append((const char *)&step_base_dev, sizeof(step_base_dev));
     const int ints[] = {ninst, total_wgs, D, order, nmax, waves, nsteps_per_graph};
     key.append((const char *)ints, sizeof(ints));
-    hipGraphExec_t exec = nullptr;
-    {
-        std::lock_guard<std::mutex> lk(g_graph_mu);
-        auto it = g_graphs.find(key);
-        if (it != g_graphs.end()) exec = it->second;
-    }
-    if (!exec) {
+    GraphRef entry = graph_lookup(key);
+    if (!entry) {
+        hipGraphExec_t exec = nullptr;
         hipStream_t cap;
         DH_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
         DH_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
@@ -3030,19 +3119,14 @@ int dh_integrate_batch(const dh_sweep_item *items_dev, const int *wg_item_dev, i
         (void)hipGraphDestroy(graph);
         if (ei != hipSuccess)
             return fail(DH_EHIP, std::string("instantiate: ") + hipGetErrorString(ei));
-        std::lock_guard<std::mutex> lk(g_graph_mu);
-        if (g_graphs.size() > 64) {
-            if (hipDeviceSynchronize() != hipSuccess) return fail(DH_EHIP, "device sync");
-            for (auto &kv : g_graphs) (void)hipGraphExecDestroy(kv.second);
-            g_graphs.clear();
-        }
-        g_graphs[key] = exec;
+        entry = graph_insert(key, exec);
     }
     hipLaunchKernelGGL(k_set_ll, dim3(1), dim3(1), 0, st, step_base_dev, first_step);
     const long long last =
         first_step + nsteps < max_total_steps ? first_step + nsteps : max_total_steps;
     const long long reps = (last - first_step + nsteps_per_graph - 1) / nsteps_per_graph;
-    for (long long r = 0; r < reps; ++r) DH_HIP(hipGraphLaunch(exec, st));
+    for (long long r = 0; r < reps; ++r) DH_HIP(hipGraphLaunch(entry->exec, st));
+    DH_HIP(entry->note_launch(st));
     return DH_OK;
 }
 

@@ -10,8 +10,9 @@ Contents, each a restatement of the reference function it cites:
 * ``default_grid``            dadi/Numerics.py:215-233
 * ``phi_1D_snm``/``phi_1D_genic``/``phi_1D``   dadi/PhiManip.py:166-195, 110-164, 12-108
 * ``phi_1D_to_2D``            dadi/PhiManip.py:214-234
-* ``phi_2D_to_3D_admix``/``phi_3D_to_4D``/``phi_4D_to_5D``
-                              dadi/PhiManip.py:268-300, 302-370, 372-495
+* ``phi_2D_to_3D_admix``/``_split_1``/``_split_2``/``phi_3D_to_4D``/``phi_4D_to_5D``
+                              dadi/PhiManip.py:236-300, 302-370, 372-495
+* ``remove_pop``/``filter_pops``  dadi/PhiManip.py:1036-1072
 * ``compute_dt``              dadi/Integration.py:147-176
 * ``inject_value``            dadi/Integration.py:70-145
 * ``integrate_const``         dadi/Integration.py:986-1242 (constant parameters)
@@ -24,6 +25,7 @@ Contents, each a restatement of the reference function it cites:
 * ``quadratic_extrap``/``linear_extrap``/``extrapolate``
                               dadi/Numerics.py:334-475, 644-669
 * ``ll``/``ll_multinom``      dadi/Inference.py:466-575, 668-685
+* ``marginalize``             the entry sums of Spectrum.marginalize
 
 The sweeps run either through this repository's C restatement
 (``oracle/libdadi_oracle.so``, backend ``"oracle"``) or through the reference C
@@ -208,6 +210,35 @@ def phi_2D_to_3D_admix(phi, f1, xx, yy, zz):
 def phi_2D_to_3D_split_2(xx, phi_2D):
     """dadi/PhiManip.py:236-250."""
     return phi_2D_to_3D_admix(phi_2D, 0, xx, xx, xx)
+
+
+def phi_2D_to_3D_split_1(xx, phi_2D):
+    """dadi/PhiManip.py:252-266."""
+    return phi_2D_to_3D_admix(phi_2D, 1, xx, xx, xx)
+
+
+def _trapz_axis(yy, xx, axis):
+    """dadi/Numerics.py:303-332 with a grid: sum of dx (y_i + y_{i+1}) / 2."""
+    yy = np.asarray(yy, dtype=np.float64)
+    dx = np.diff(xx)
+    shape = [1] * yy.ndim
+    shape[axis] = len(dx)
+    lo = np.take(yy, np.arange(len(xx) - 1), axis=axis)
+    hi = np.take(yy, np.arange(1, len(xx)), axis=axis)
+    return np.sum(dx.reshape(shape) * (hi + lo) / 2.0, axis=axis)
+
+
+def remove_pop(phi, xx, popnum):
+    """dadi/PhiManip.py:1036-1050: integrate population popnum (from 1) out."""
+    return _trapz_axis(phi, xx, popnum - 1)
+
+
+def filter_pops(phi, xx, tokeep):
+    """dadi/PhiManip.py:1052-1072: remove every population not in tokeep,
+    highest number first."""
+    for pop in sorted(set(range(1, np.ndim(phi) + 1)) - set(tokeep))[::-1]:
+        phi = remove_pop(phi, xx, pop)
+    return phi
 
 
 def phi_3D_to_4D(phi, f1, f2, xx, yy, zz, aa):
@@ -698,6 +729,12 @@ def from_phi_direct(phi, ns, xxs, het_ascertained=None):
     raise ValueError("direct sampling is defined for 1-4 populations")
 
 
+def marginalize(fs, over):
+    """Spectrum.marginalize's data (dadi/Spectrum_mod.py): the entries summed
+    over the listed axes."""
+    return np.sum(np.asarray(fs), axis=tuple(over))
+
+
 def corner_mask(shape):
     m = np.zeros(shape, dtype=bool)
     m[(0,) * len(shape)] = True
@@ -721,7 +758,13 @@ def quadratic_extrap(ys, xs):
 
 
 def extrapolate(results, xs, log=False, fail_mag=10):
-    """Extrapolation step of make_extrap_func, dadi/Numerics.py:417-469."""
+    """Extrapolation step of make_extrap_func, dadi/Numerics.py:417-469.
+
+    The reference applies it to Spectrum objects, i.e. numpy masked arrays,
+    whose log10 masks non-positive arguments: an entry whose extrapolation is
+    zero or of the opposite sign to the finest grid's value never counts as
+    failed (it keeps the extrapolated value), while an infinite ratio does.
+    The same rule is applied here on plain arrays."""
     res = [np.asarray(r, dtype=np.float64) for r in results]
     with np.errstate(all="ignore"):
         if log:
@@ -740,7 +783,8 @@ def extrapolate(results, xs, log=False, fail_mag=10):
             best = res[int(np.argmin(xs))]
             if log:
                 best = np.exp(best)
-            failed = np.abs(np.log10(ex / best)) > fail_mag
+            ratio = ex / best
+            failed = (ratio > 0) & (np.abs(np.log10(np.where(ratio > 0, ratio, 1.0))) > fail_mag)
             ex = np.where(failed, best, ex)
     return ex
 

@@ -51,3 +51,33 @@ def test_grid_and_godambe_batched_equal_serial(setup):
     d.Godambe.cache.clear()
     b = d.Godambe.FIM_uncert(func_ex, pts, popt, data, batch=True)
     assert np.array_equal(a, b) and np.all(np.isfinite(a))
+
+
+def test_optimize_log_batched_agrees_with_oracle_model(setup, oracle):
+    """The batched GPU optimisation against the same BFGS driven by an
+    oracle-backed split_mig (the oracle's integration and sampling, evaluated
+    one point at a time on the CPU): the spectra differ only by the sampling's
+    summation order (~1e-13 relative), so the optimum agrees to 1e-6 relative
+    and the log-likelihood to 1e-6 absolute (dadi/Inference.py:80-176 for the
+    driver)."""
+    d, func_ex, pts, data = setup
+
+    def oracle_split_mig(params, ns, pts_):
+        nu1, nu2, T, m = params
+        xx = oracle.default_grid(pts_)
+        phi = oracle.phi_1D_to_2D(xx, oracle.phi_1D(xx))
+        phi = oracle.integrate_const(phi, xx, T, [nu1, nu2], [[m], [m]], [0, 0], [0.5, 0.5])
+        fs = d.Spectrum(oracle.from_phi(phi, ns, xx))
+        fs.extrap_x = xx[1]
+        return fs
+
+    oracle_ex = d.Numerics.make_extrap_log_func(oracle_split_mig)
+    kw = dict(lower_bound=[1e-2, 1e-2, 1e-3, 0], upper_bound=[20, 20, 5, 10], maxiter=3,
+              full_output=True)
+    p0 = [1.0, 1.0, 0.5, 1.0]
+    want = d.Inference.optimize_log(p0, data, oracle_ex, pts, batch=False, **kw)
+    got = d.Inference.optimize_log(p0, data, func_ex, pts, batch=True, **kw)
+    np.testing.assert_allclose(got[0], want[0], rtol=1e-6)
+    assert abs(got[1] - want[1]) <= 1e-6
+    # and the optimum moved away from the start, so the comparison has teeth
+    assert np.max(np.abs(np.log(got[0]) - np.log(p0))) > 1e-2

@@ -181,3 +181,23 @@ def test_cache2d_integrate_accounts_for_all_mass(params):
     inner = c.integrate(params, None, sel_dist, 1.0, None, exterior_int=False)
     sq = (d1.cdf(hi) - d1.cdf(lo)) * (d2.cdf(hi) - d2.cdf(lo))
     np.testing.assert_allclose(np.asarray(inner), K * sq, rtol=5e-3)
+
+
+@pytest.mark.parametrize("attr", ["folded", "folded_ancestral", "folded_major"])
+def test_folded_data_with_unfolded_model_raises(attr):
+    """The reference folds the model to match folded data
+    (dadi/Inference.py:493-501, 676-683); without folding support the
+    likelihoods and the theta scaling refuse instead of comparing mismatched
+    spectra.  Folded model with folded data, and unfolded with unfolded, work."""
+    import numpy as np
+    import dadi_amd
+    rng = np.random.default_rng(0)
+    model = dadi_amd.Spectrum(rng.uniform(1, 5, (6, 6)))
+    data = dadi_amd.Spectrum(rng.poisson(3.0, (6, 6)).astype(float))
+    setattr(data, attr, True)
+    for f in (dadi_amd.Inference.ll, dadi_amd.Inference.ll_multinom,
+              dadi_amd.Inference.optimal_sfs_scaling, dadi_amd.Inference.ll_per_bin):
+        with pytest.raises(NotImplementedError):
+            f(model, data)
+    setattr(model, attr, True)
+    assert np.isfinite(dadi_amd.Inference.ll(model, data))

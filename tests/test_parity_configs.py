@@ -1,0 +1,232 @@
+"""
+GPU parity at the sizes of the benchmark configurations, against the oracle:
+
+* 2-population split_mig at pts (80, 90, 100), ns (20, 20): extrapolated
+  spectrum within 1e-10 relative on entries above 1e-14 of the mass, Poisson
+  log-likelihood against a committed integer data fixture within 1e-9;
+* the 3-population chain (1 -> 2 -> 3 populations) at pts (40, 50, 60), ns
+  (10, 10, 10), extrapolated, within 1e-10;
+* DFE batch (a): two_epoch_sel at pts (200, 220, 240), n = 50, over gammas
+  spanning -2000 .. -1e-4: every integration of the batched launch bit-exact,
+  the cached spectra within 1e-10;
+* DFE batch (b): a Cache2D of split_mig_sel over a 3 x 3 gamma grid at pts
+  (60, 70, 80), ns (20, 20): entries equal to one-at-a-time evaluations and
+  within 1e-10 of the oracle;
+* the device-resident splits and removals equal to the host versions.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REL = 1e-10      # extrapolated spectra, relative, on entries above 1e-14 of the mass
+LL_ABS = 1e-9    # Poisson log-likelihood, absolute
+
+
+@pytest.fixture(scope="module")
+def dadi_amd():
+    import dadi_amd as d
+    assert d.cuda_enabled(True), "HIP backend failed to initialise"
+    return d
+
+
+def _close(got, want, what):
+    got = np.asarray(got, dtype=np.float64)
+    mask = np.zeros(want.shape, dtype=bool)
+    mask[(0,) * want.ndim] = mask[tuple(s - 1 for s in want.shape)] = True
+    big = (want > 1e-14 * want[~mask].sum()) & ~mask
+    rel = np.abs(got - want)[big] / np.abs(want[big])
+    assert rel.max() <= REL, (what, float(rel.max()))
+    return float(rel.max())
+
+
+def _oracle_extrap(oracle, spectra, pts_l, log):
+    xs = [oracle.default_grid(p)[1] for p in pts_l]
+    return oracle.extrapolate(spectra, xs, log=log)
+
+
+# ---------------------------------------------------------------------------
+# configuration 2: split_mig at pts (80, 90, 100)
+# ---------------------------------------------------------------------------
+def test_split_mig_at_config_grids_spectrum_and_ll(dadi_amd, oracle):
+    fx = np.load(os.path.join(HERE, "golden", "split_mig_pts80_90_100_ll.npz"))
+    pts_l = tuple(int(p) for p in fx["pts"])
+    ns = tuple(int(v) for v in fx["ns"])
+    assert pts_l == (80, 90, 100) and ns == (20, 20)
+    func = dadi_amd.Numerics.make_extrap_log_func(dadi_amd.Demographics2D.split_mig)
+    for params, data in zip(fx["params"], fx["data"]):
+        nu1, nu2, T, m = (float(v) for v in params)
+        print("split_mig", params.tolist(), flush=True)
+        model = func((nu1, nu2, T, m), ns, pts_l)
+        res = []
+        for pts in pts_l:
+            xx = oracle.default_grid(pts)
+            phi = oracle.phi_1D_to_2D(xx, oracle.phi_1D(xx))
+            phi = oracle.integrate_const(phi, xx, T, [nu1, nu2], [[m], [m]], [0, 0], [0.5, 0.5])
+            res.append(oracle.from_phi(phi, ns, xx))
+        want = _oracle_extrap(oracle, res, pts_l, log=True)
+        _close(model.data, want, params)
+        # the likelihood of the committed data under 1000 x model
+        data_fs = dadi_amd.Spectrum(data.astype(float))
+        ll_gpu = dadi_amd.Inference.ll(1000 * model, data_fs)
+        ll_ref = oracle.ll(1000 * want, data)
+        assert abs(ll_gpu - ll_ref) <= LL_ABS, (ll_gpu, ll_ref)
+        llm_gpu = dadi_amd.Inference.ll_multinom(model, data_fs)
+        assert abs(llm_gpu - oracle.ll_multinom(want, data)) <= LL_ABS
+
+
+# ---------------------------------------------------------------------------
+# configuration 3: the 3-population chain at pts (40, 50, 60)
+# ---------------------------------------------------------------------------
+def _chain_draws(oracle, seed=303, keep=2, max_steps=300):
+    """Seeded draws of the chain's distribution (sizes logU[0.1, 10],
+    migration U[0, 5], epoch times U[0.01, 1]) whose 3-population epoch takes
+    at most max_steps steps, so the CPU oracle finishes in seconds."""
+    rng = np.random.default_rng(seed)
+    lu = lambda: float(np.exp(rng.uniform(np.log(0.1), np.log(10))))
+    out = []
+    while len(out) < keep:
+        p = (lu(), rng.uniform(0.01, 1), lu(), lu(), rng.uniform(0.01, 1), rng.uniform(0, 5),
+             lu(), lu(), lu(), rng.uniform(0.01, 1), rng.uniform(0, 5))
+        dt = min(oracle.compute_dt(nu, [p[10], p[10]], 0, 0.5) for nu in p[6:9])
+        if np.ceil(p[9] / dt) <= max_steps:
+            out.append(tuple(float(v) for v in p))
+    return out
+
+
+def _oracle_chain(oracle, p, pts):
+    nuA, TA, nu1, nu2, T2, m2, nu1b, nu2b, nu3, T3, m3 = p
+    xx = oracle.default_grid(pts)
+    phi = oracle.integrate_const(oracle.phi_1D(xx), xx, TA, [nuA], [[]], [0], [0.5])
+    phi = oracle.phi_1D_to_2D(xx, phi)
+    phi = oracle.integrate_const(phi, xx, T2, [nu1, nu2], [[m2], [m2]], [0, 0], [0.5, 0.5])
+    phi = oracle.phi_2D_to_3D_split_2(xx, phi)
+    phi = oracle.integrate_const(phi, xx, T3, [nu1b, nu2b, nu3], [[m3, m3]] * 3, [0] * 3,
+                                 [0.5] * 3)
+    return phi, xx
+
+
+def test_three_pop_chain_at_config_grids(dadi_amd, oracle):
+    pts_l, ns = (40, 50, 60), (10, 10, 10)
+    D3 = dadi_amd.Demographics3D
+    func = dadi_amd.Numerics.make_extrap_log_func(D3.three_pop_chain)
+    for p in _chain_draws(oracle):
+        print("three_pop_chain", p, flush=True)
+        res = []
+        for pts in pts_l:
+            phi, xx = _oracle_chain(oracle, p, pts)
+            # the integrated phi itself, bit for bit, at the finest grid
+            if pts == pts_l[-1]:
+                got_phi, _ = D3.three_pop_chain_phi(p, pts)
+                assert np.array_equal(got_phi, phi)
+            res.append(oracle.from_phi(phi, ns, xx))
+        want = _oracle_extrap(oracle, res, pts_l, log=True)
+        _close(func(p, ns, pts_l).data, want, p)
+
+
+# ---------------------------------------------------------------------------
+# configuration 4a: 1D DFE cache at pts (200, 220, 240), n = 50
+# ---------------------------------------------------------------------------
+def test_dfe_two_epoch_sel_at_config_grids(dadi_amd, oracle):
+    from dadi_amd import engine
+    from dadi_amd.DFE import DemogSelModels, Cache1D
+    pts_l, n = (200, 220, 240), 50
+    nu, T = 2.0, 0.1
+    gammas = -np.logspace(np.log10(2000), np.log10(1e-4), 8)
+
+    def integrate(g, pts):
+        xx = dadi_amd.Numerics.default_grid(pts)
+        phi = dadi_amd.PhiManip.phi_1D(xx, gamma=g)
+        return dadi_amd.Integration.one_pop(phi, xx, T, nu, gamma=g)
+
+    tasks = [(float(g), pts) for g in gammas for pts in pts_l]
+    got = engine.map_batched(integrate, tasks, workers=len(tasks))
+    for (g, pts), phi in zip(tasks, got):
+        assert not isinstance(phi, Exception), phi
+        xx = oracle.default_grid(pts)
+        want = oracle.integrate_const(oracle.phi_1D(xx, gamma=g), xx, T, [nu], [[]], [g],
+                                      [0.5])
+        assert np.array_equal(phi, want), (g, pts)
+    cache = Cache1D((nu, T), (n,), DemogSelModels.two_epoch_sel, pts_l,
+                    gamma_bounds=(1e-4, 2000), gamma_pts=8)
+    assert np.allclose(cache.gammas, gammas, rtol=0, atol=0)
+    for ii, g in enumerate(gammas):
+        res = []
+        for pts in pts_l:
+            xx = oracle.default_grid(pts)
+            phi = oracle.integrate_const(oracle.phi_1D(xx, gamma=g), xx, T, [nu], [[]], [g],
+                                         [0.5])
+            res.append(oracle.from_phi(phi, [n], xx))
+        want = _oracle_extrap(oracle, res, pts_l, log=False)
+        _close(cache.spectra[ii], want, g)
+
+
+# ---------------------------------------------------------------------------
+# configuration 4b: 2D DFE cache over gamma pairs at pts (60, 70, 80)
+# ---------------------------------------------------------------------------
+def test_dfe_cache2d_split_mig_sel_at_config_grids(dadi_amd, oracle):
+    from dadi_amd.DFE import DemogSelModels, Cache2D
+    params = (0.33, 3.49, 0.1, 0.71)
+    ns, pts_l = (20, 20), (60, 70, 80)
+    cache = Cache2D(params, ns, DemogSelModels.split_mig_sel, pts_l, gamma_bounds=(1e-2, 20),
+                    gamma_pts=3)
+    seq = dadi_amd.Numerics.make_extrap_func(DemogSelModels.split_mig_sel)
+    nu1, nu2, T, m = params
+    for i, g1 in enumerate(cache.gammas):
+        for j, g2 in enumerate(cache.gammas):
+            one = seq(params + (float(g1), float(g2)), ns, pts_l)
+            assert np.array_equal(np.asarray(one.data), cache.spectra[i, j]), (g1, g2)
+            if (i + j) % 2:
+                continue      # the oracle on every other pair keeps the CPU time short
+            res = []
+            for pts in pts_l:
+                xx = oracle.default_grid(pts)
+                phi = oracle.phi_1D_to_2D(xx, oracle.phi_1D(xx, gamma=g1))
+                phi = oracle.integrate_const(phi, xx, T, [nu1, nu2], [[m], [m]], [g1, g2],
+                                             [0.5, 0.5])
+                res.append(oracle.from_phi(phi, ns, xx))
+            _close(cache.spectra[i, j], _oracle_extrap(oracle, res, pts_l, log=False),
+                   (g1, g2))
+
+
+# ---------------------------------------------------------------------------
+# device-resident splits and removals
+# ---------------------------------------------------------------------------
+def test_device_splits_and_removals_equal_host(dadi_amd):
+    import torch
+    P = dadi_amd.PhiManip
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(41)
+    for n in (9, 20):
+        xx = dadi_amd.Numerics.default_grid(n)
+        yy = np.sort(np.concatenate(([0.0, 1.0], rng.uniform(0, 1, n - 2))))
+        t = lambda a: torch.from_numpy(a).to(dev)
+        p1 = rng.uniform(0.1, 2, n)
+        assert np.array_equal(P.phi_1D_to_2D_device(xx, t(p1)).cpu().numpy(),
+                              P.phi_1D_to_2D(xx, p1))
+        p2 = rng.uniform(0.1, 2, (n, n))
+        for f in (0.0, 0.3, 1.0):
+            assert np.array_equal(P.phi_2D_to_3D_admix_device(t(p2), f, xx, yy, xx).cpu().numpy(),
+                                  P.phi_2D_to_3D_admix(p2, f, xx, yy, xx))
+        assert np.array_equal(P.phi_2D_to_3D_split_1_device(xx, t(p2)).cpu().numpy(),
+                              P.phi_2D_to_3D_split_1(xx, p2))
+        assert np.array_equal(P.phi_2D_to_3D_split_2_device(xx, t(p2)).cpu().numpy(),
+                              P.phi_2D_to_3D_split_2(xx, p2))
+        p3 = rng.uniform(0.1, 2, (n, n, n))
+        for f1, f2 in ((0.0, 0.0), (0.25, 0.4), (1.0, 0.0)):
+            assert np.array_equal(
+                P.phi_3D_to_4D_device(t(p3), f1, f2, xx, yy, xx, yy).cpu().numpy(),
+                P.phi_3D_to_4D(p3, f1, f2, xx, yy, xx, yy))
+        if n <= 9:
+            p4 = rng.uniform(0.1, 2, (n,) * 4)
+            assert np.array_equal(
+                P.phi_4D_to_5D_device(t(p4), 0.1, 0.2, 0.3, xx, xx, yy, yy, xx).cpu().numpy(),
+                P.phi_4D_to_5D(p4, 0.1, 0.2, 0.3, xx, xx, yy, yy, xx))
+            assert np.array_equal(P.filter_pops_device(t(p4), xx, [2, 4]).cpu().numpy(),
+                                  P.filter_pops(p4, xx, [2, 4]))
+        for pop in (1, 2, 3):
+            assert np.array_equal(P.remove_pop_device(t(p3), xx, pop).cpu().numpy(),
+                                  P.remove_pop(p3, xx, pop))

@@ -134,3 +134,35 @@ def test_graph_range_rejects_ragged_quanta(dadi_amd):
     rc = ctx.lib.dh_integrate_graph_range(inst, base.data_ptr(), base.data_ptr(), 100, 0, 10,
                                           4, 2, ctx.stream())
     assert rc == -1   # DH_EINVAL
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["three_epoch", "three_pop_chain", "four_pop_chain"])
+def test_served_programs_equal_model_functions(dadi_amd, kind):
+    """Served 1D, 3D-chain and 4D-chain requests (phi resident on the device
+    across epochs and device-side splits) equal the sequential model functions
+    (Demographics1D.three_epoch, Demographics3D.three_pop_chain /
+    four_pop_chain), alone and inside a mix of other requests."""
+    from dadi_amd import serve
+    if kind == "three_epoch":
+        params, ns, pts = (0.4, 2.5, 0.05, 0.02), (17,), 150
+        prog = serve.prog_three_epoch(params, ns, pts)
+        fs = dadi_amd.Demographics1D.three_epoch(params, ns, pts)
+    elif kind == "three_pop_chain":
+        params = (1.5, 0.02, 0.7, 2.2, 0.01, 1.3, 0.5, 1.9, 3.1, 0.004, 0.8)
+        ns, pts = (6, 7, 5), 24
+        prog = serve.prog_three_pop_chain(params, ns, pts)
+        fs = dadi_amd.Demographics3D.three_pop_chain(params, ns, pts)
+    else:
+        params = (1.5, 0.02, 0.7, 2.2, 0.01, 1.3, 0.5, 1.9, 3.1, 0.004, 0.8,
+                  0.9, 1.4, 0.6, 2.0, 0.003, 0.5)
+        ns, pts = (4, 5, 3, 4), 14
+        prog = serve.prog_four_pop_chain(params, ns, pts)
+        fs = dadi_amd.Demographics3D.four_pop_chain(params, ns, pts)
+    alone = serve.Request(0, prog, 0.0)
+    serve.Server().run([alone], realtime=False)
+    assert np.array_equal(alone.result, np.asarray(fs.data))
+    mixed = serve.Request(1, prog, 0.0)
+    st = serve.Server(quantum=5).run(_small_mix(13, 10) + [mixed], realtime=False)
+    assert st["failed"] == 0
+    assert np.array_equal(mixed.result, np.asarray(fs.data))
