@@ -2398,6 +2398,29 @@ __global__ void k_sample_prep(const double *__restrict__ in, long long P, int n,
     cT[e] = c1;
 }
 
+// Error-free transformations (Knuth's TwoSum; TwoProduct through the fused
+// multiply-add, which is exact): a + b = s + e and a * b = p + e exactly.
+__device__ __forceinline__ void two_sum(double a, double b, double &s, double &e)
+{
+    s = a + b;
+    const double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+__device__ __forceinline__ void two_prod(double a, double b, double &p, double &e)
+{
+    p = a * b;
+    e = fma(a, b, -p);
+}
+
+// out[k][p] = sum_i c1[p][i] db1[k][i] + fac_k sum_i s[p][i] db2[k][i]
+// (dadi/Spectrum_mod.py:1527-1532, fac_k = (k+1)/((m+1)(m+2))).  The sums are
+// compensated (Ogita, Rump and Oishi's Dot2, SIAM J. Sci. Comput. 26, 2005):
+// each product and partial sum carries its exact rounding error, so the result
+// is the exactly rounded value of the formula up to an error of order
+// eps^2 times the sum of the magnitudes of the terms.  Entries next to the
+// spectrum's corners cancel strongly (terms ~1e4 times the result), where an
+// uncompensated sum in any order is off by ~1e-12 relative before
+// extrapolation amplifies it.
 __global__ void k_sample_contract(const double *__restrict__ sT, const double *__restrict__ cT,
                                   long long P, int n, int m, const double *__restrict__ db1,
                                   const double *__restrict__ db2, double *__restrict__ out)
@@ -2406,14 +2429,21 @@ __global__ void k_sample_contract(const double *__restrict__ sT, const double *_
     if (e >= (long long)(m + 1) * P) return;
     const long long k = e / P, p = e - k * P;
     const double *d1 = db1 + k * (n - 1), *d2 = db2 + k * (n - 1);
-    double t1 = 0., t2 = 0.;
+    double s1 = 0., c1 = 0., s2 = 0., c2 = 0.;
     for (int i = 0; i < n - 1; ++i) {
-        t1 = t1 + cT[(long long)i * P + p] * d1[i];
-        t2 = t2 + sT[(long long)i * P + p] * d2[i];
+        double q, eq, es;
+        two_prod(cT[(long long)i * P + p], d1[i], q, eq);
+        two_sum(s1, q, s1, es);
+        c1 = c1 + (es + eq);
+        two_prod(sT[(long long)i * P + p], d2[i], q, eq);
+        two_sum(s2, q, s2, es);
+        c2 = c2 + (es + eq);
     }
-    // dadi/Spectrum_mod.py:1531-1532: term2 *= (k+1)/((m+1)(m+2)); term1 + term2
     const double fac = (double)(k + 1) / (double)((m + 1) * (m + 2));
-    out[e] = t1 + t2 * fac;
+    double q, eq, t, et;
+    two_prod(s2, fac, q, eq);
+    two_sum(s1, q, t, et);
+    out[e] = t + (et + ((c1 + eq) + c2 * fac));
 }
 
 // Direct (trapezoid) sampling: out[k][p] = sum_i A[k][i] * in[p][i], where

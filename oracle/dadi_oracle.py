@@ -20,6 +20,7 @@ Contents, each a restatement of the reference function it cites:
 * ``one_pop_X``               dadi/Integration.py:1244-1352 (X chromosome, constant)
 * ``from_phi``                dadi/Spectrum_mod.py:25-41, 1307-1348, 1509-1548,
                               1774-1871, 1938-2062
+* ``from_phi_exact``          the same formula with compensated (once-rounded) sums
 * ``from_phi_direct``         dadi/Spectrum_mod.py:1263-1279, 1350-1380, 1550-1592,
                               1594-1643 (trapezoid samplers, heterozygote ascertainment)
 * ``quadratic_extrap``/``linear_extrap``/``extrapolate``
@@ -639,6 +640,94 @@ def _contract_last(phi, n, xx):
     term2 = np.dot(s, db2.T)
     term2 *= np.arange(1, n + 2) / ((n + 1) * (n + 2))
     return term1 + term2
+
+
+def _two_sum(a, b):
+    """Knuth's TwoSum: a + b = s + e exactly."""
+    s = a + b
+    bb = s - a
+    return s, (a - (s - bb)) + (b - bb)
+
+
+def _split(a):
+    """Veltkamp's split of a double into two 26-bit halves."""
+    c = 134217729.0 * a
+    hi = c - (c - a)
+    return hi, a - hi
+
+
+def _two_prod(a, b):
+    """Dekker's TwoProduct: a * b = p + e exactly (no fused multiply-add
+    needed; exact while the product stays clear of under- and overflow)."""
+    p = a * b
+    ah, al = _split(a)
+    bh, bl = _split(b)
+    return p, al * bl - (((p - ah * bh) - al * bh) - ah * bl)
+
+
+def _dot2(A, B):
+    """Compensated dot products over the last axis, A (..., L) against the rows
+    of B (K, L), as (s, c) with s + c the result (Ogita, Rump and Oishi, Dot2)."""
+    shp = A.shape[:-1] + (B.shape[0],)
+    s = np.zeros(shp)
+    c = np.zeros(shp)
+    for i in range(A.shape[-1]):
+        q, eq = _two_prod(A[..., i:i + 1], B[:, i])
+        s, es = _two_sum(s, q)
+        c = c + (es + eq)
+    return s, c
+
+
+def _contract_last_exact(phi, n, xx):
+    """_contract_last with compensated sums: term1 + term2 of
+    dadi/Spectrum_mod.py:1527-1532 rounded once (up to eps^2-level error),
+    instead of numpy's BLAS summation order."""
+    db1, db2 = cached_dbeta(n, xx)
+    s = (phi[..., 1:] - phi[..., :-1]) / (xx[1:] - xx[:-1])
+    c1 = (phi[..., :-1] - s * xx[:-1]) / (n + 1)
+    s1, k1 = _dot2(c1, db1)
+    s2, k2 = _dot2(s, db2)
+    fac = np.arange(1, n + 2) / ((n + 1) * (n + 2))
+    q, eq = _two_prod(s2, fac)
+    t, et = _two_sum(s1, q)
+    return t + (et + ((k1 + eq) + k2 * fac))
+
+
+def from_phi_exact(phi, ns, xx):
+    """from_phi with every sampling sum rounded once (compensated dot
+    products) rather than in numpy's BLAS order: the value the reference's
+    formula defines for these inputs, against which any summation order is an
+    approximation.  Its closeness to exact rational sums is checked in the
+    tests."""
+    phi = np.asarray(phi, dtype=np.float64)
+    out = phi
+    for k in range(phi.ndim - 1, -1, -1):
+        if phi.ndim == 1:
+            out = _contract_last_exact(out[None, :], ns[0], xx)[0]
+            break
+        out = _contract_last_exact(out, ns[k], xx)
+        out = np.moveaxis(out, -1, 0)
+    return out
+
+
+def contract_fraction(phi_rows, n, xx, ks):
+    """Exact rational value of the sampling contraction of each row of
+    phi_rows (P, pts) at sample entries ks, with the reference's double
+    inputs (s, c1, the betainc tables, fac) taken as exact; for checking the
+    compensated sums on small cases."""
+    from fractions import Fraction as F
+    db1, db2 = cached_dbeta(n, xx)
+    phi_rows = np.atleast_2d(phi_rows)
+    s = (phi_rows[:, 1:] - phi_rows[:, :-1]) / (xx[1:] - xx[:-1])
+    c1 = (phi_rows[:, :-1] - s * xx[:-1]) / (n + 1)
+    fac = np.arange(1, n + 2) / ((n + 1) * (n + 2))
+    out = np.empty((len(phi_rows), len(ks)))
+    for p in range(len(phi_rows)):
+        for j, k in enumerate(ks):
+            t1 = sum(F(float(a)) * F(float(b)) for a, b in zip(c1[p], db1[k]))
+            t2 = sum(F(float(a)) * F(float(b)) for a, b in zip(s[p], db2[k]))
+            out[p, j] = float(t1 + t2 * F(float(fac[k])))
+    return out
 
 
 def from_phi(phi, ns, xx):

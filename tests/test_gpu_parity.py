@@ -3,10 +3,12 @@ GPU parity: the HIP path through the C ABI against the reference's own C
 kernels' outputs (golden vectors) and the oracle.
 
 Integration results must be bit-identical (same operations, same order, no
-FMA contraction, IEEE division).  Sampling uses a different summation order
-than numpy's BLAS dot, so spectra are compared with rtol 1e-12; extrapolated
-spectra against the stated 1e-10 on entries above 1e-14 of the total mass,
-and the Poisson log-likelihood to 1e-9 absolute.
+FMA contraction, IEEE division).  The device's sampling sums are compensated
+(exactly rounded up to eps^2-level terms), numpy's BLAS dot sums in its own
+order: spectra are compared with rtol 1e-12 against the reference-order
+oracle and 1e-15 against its exactly rounded sampler; extrapolated spectra
+against the stated 1e-10 on entries above 1e-14 of the total mass, and the
+Poisson log-likelihood to 1e-9 absolute.
 """
 import numpy as np
 import pytest
@@ -166,6 +168,9 @@ def test_sampling_matches_oracle(dadi_amd, oracle, D, n, ns):
     fs = dadi_amd.Spectrum.from_phi(phi, ns, [xx] * D)
     want = oracle.from_phi(phi, ns, xx)
     np.testing.assert_allclose(fs.data, want, rtol=1e-12, atol=1e-300)
+    # the device sums are compensated: the exactly rounded values, to rounding
+    np.testing.assert_allclose(fs.data, oracle.from_phi_exact(phi, ns, xx), rtol=1e-15,
+                               atol=1e-300)
     assert fs.mask[(0,) * D] and fs.mask[tuple(ns)]
     assert fs.extrap_x == xx[1]
 
@@ -173,25 +178,31 @@ def test_sampling_matches_oracle(dadi_amd, oracle, D, n, ns):
 def test_split_mig_extrapolated_spectrum_and_ll(dadi_amd, oracle):
     """End to end on the configuration's model shape at small grids:
     extrapolated spectrum within 1e-10 relative (entries above 1e-14 of the
-    mass) and Poisson ll within 1e-9 absolute of the oracle."""
+    mass) and Poisson ll within 1e-9 absolute of the oracle (its exactly
+    rounded sampling sums)."""
     params = (1.7, 0.35, 0.21, 1.3)
     ns = (12, 12)
     pts_l = (30, 36, 42)
     func = dadi_amd.Numerics.make_extrap_log_func(dadi_amd.Demographics2D.split_mig)
     model = func(params, ns, pts_l)
-    res, xs = [], []
+    res, ref, xs = [], [], []
     for pts in pts_l:
         xx = oracle.default_grid(pts)
         phi = oracle.phi_1D_to_2D(xx, oracle.phi_1D(xx))
         phi = oracle.integrate_const(phi, xx, params[2], [params[0], params[1]],
                                      [[params[3]], [params[3]]], [0, 0], [0.5, 0.5])
-        res.append(oracle.from_phi(phi, ns, xx))
+        res.append(oracle.from_phi_exact(phi, ns, xx))
+        ref.append(oracle.from_phi(phi, ns, xx))
         xs.append(xx[1])
     want = oracle.extrapolate(res, xs, log=True)
     mask = oracle.corner_mask(want.shape)
     big = (want > 1e-14 * want[~mask].sum()) & ~mask
     rel = np.abs(model.data - want)[big] / np.abs(want[big])
     assert rel.max() <= 1e-10
+    # numpy's summation order (dadi's own) differs from the exactly rounded
+    # sums by its rounding, amplified by the extrapolation
+    want_ref = oracle.extrapolate(ref, xs, log=True)
+    assert (np.abs(model.data - want_ref)[big] / np.abs(want_ref[big])).max() <= 2e-9
     rng = np.random.default_rng(1234)
     data = rng.poisson(1000 * want)
     data_fs = dadi_amd.Spectrum(data)

@@ -345,3 +345,51 @@ def test_product_host_splits_equal_oracle(oracle):
         for pop in (1, 2, 3):
             assert np.array_equal(P.remove_pop(p3, xx, pop), oracle.remove_pop(p3, xx, pop))
         assert np.array_equal(P.filter_pops(p4, xx, [2, 4]), oracle.filter_pops(p4, xx, [2, 4]))
+
+
+# ---------------------------------------------------------------------------
+# exactly rounded sampling sums
+# ---------------------------------------------------------------------------
+def _short_split_mig_phi(oracle, pts, T=0.02):
+    xx = oracle.default_grid(pts)
+    phi = oracle.phi_1D_to_2D(xx, oracle.phi_1D(xx))
+    return oracle.integrate_const(phi, xx, T, [0.16, 1.35], [[2.3], [2.3]], [0, 0],
+                                  [0.5, 0.5]), xx
+
+
+def test_compensated_sampling_equals_exact_rational_sums(oracle):
+    """from_phi_exact (compensated dot products) against exact rational
+    arithmetic on the same double inputs, contraction by contraction, on a
+    split_mig density shortly after the split: its entries next to the
+    spectrum's corners cancel by ~4 orders of magnitude.  The compensated
+    sums are the exactly rounded values (within one rounding); numpy's BLAS
+    order is not, and after extrapolation its error reaches the 1e-10 bar at
+    pts 80-100 (here only measured, as the BLAS order depends on the host)."""
+    phi, xx = _short_split_mig_phi(oracle, 30)
+    ns = (6, 8)
+    first = oracle._contract_last_exact(phi, ns[1], xx)            # (30, 9)
+    want1 = oracle.contract_fraction(phi, ns[1], xx, range(ns[1] + 1))
+    np.testing.assert_allclose(first, want1, rtol=2.3e-16, atol=0)
+    mid = np.moveaxis(first, -1, 0)                                   # (9, 30)
+    want2 = oracle.contract_fraction(mid, ns[0], xx, range(ns[0] + 1))
+    got = oracle.from_phi_exact(phi, ns, xx)                          # (7, 9)
+    np.testing.assert_allclose(got, want2.T, rtol=2.3e-16, atol=0)
+    # the reference-order sampler agrees to rounding but not exactly
+    ref = oracle.from_phi(phi, ns, xx)
+    assert np.allclose(ref, got, rtol=1e-11, atol=0)
+    # 1D and 3D go through the same contraction
+    p1 = phi[:, 5]
+    np.testing.assert_allclose(oracle.from_phi_exact(p1, [7], xx),
+                               oracle.contract_fraction(p1, 7, xx, range(8))[0],
+                               rtol=2.3e-16, atol=0)
+    assert np.allclose(oracle.from_phi_exact(p1, [7], xx), oracle.from_phi(p1, [7], xx),
+                       rtol=1e-12, atol=0)
+    x3 = oracle.default_grid(13)
+    p3 = np.random.default_rng(2).uniform(0.1, 2, (13, 13, 13))
+    assert np.allclose(oracle.from_phi_exact(p3, [3, 4, 5], x3),
+                       oracle.from_phi(p3, [3, 4, 5], x3), rtol=1e-13, atol=0)
+    # planted mistake: the slope term without its (k+1)/((n+1)(n+2)) factor
+    db1, db2 = oracle.cached_dbeta(ns[1], xx)
+    s = (phi[:, 1:] - phi[:, :-1]) / (xx[1:] - xx[:-1])
+    c1 = (phi[:, :-1] - s * xx[:-1]) / (ns[1] + 1)
+    assert not np.allclose(c1 @ db1.T + s @ db2.T, want1, rtol=1e-3)
