@@ -97,6 +97,10 @@ _SIGS = {
     "dh_integrate_factored": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp,
                                ctypes.POINTER(DhLaunch), _vp], ctypes.c_int),
     "dh_factored_slice_doubles": ([ctypes.c_int, ctypes.c_int], ctypes.c_longlong),
+    "dh_integrate_factored_static": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int, ctypes.POINTER(DhLaunch), _vp],
+                                     ctypes.c_int),
+    "dh_static_scratch_doubles": ([], ctypes.c_longlong),
     "dh_sweep_axis": ([ctypes.POINTER(DhInstance), ctypes.c_int, ctypes.c_int, _vp,
                        ctypes.c_longlong, _vp, ctypes.c_longlong, ctypes.c_int, _vp],
                       ctypes.c_int),

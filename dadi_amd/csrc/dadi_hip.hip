@@ -2753,6 +2753,42 @@ GraphRef graph_insert(const std::string &key, hipGraphExec_t exec)
     return out;
 }
 
+// Scratch of dh_integrate_factored_static (the entry point for callers that
+// cannot allocate): the queue counter in the first DH_STATIC_HEAD doubles,
+// then the factorisation workspace.  Every device the library runs on holds
+// its own copy until the process ends; host code reaches it only through the
+// per-device address from hipGetSymbolAddress, never through the name.
+constexpr long long DH_STATIC_HEAD = 32;                       // 256 bytes
+constexpr long long DH_STATIC_WORKSPACE = 4LL << 20;           // doubles (32 MiB)
+constexpr int DH_MAX_DEVICES = 64;
+double *g_scratch_dev[DH_MAX_DEVICES] = {};
+std::mutex g_scratch_mu;
+
+}  // namespace
+
+static __device__ __align__(256) double scratch[DH_STATIC_HEAD + DH_STATIC_WORKSPACE];
+
+namespace {
+
+// Address of `scratch` on the current device, resolved once per device.
+int static_scratch(double **out)
+{
+    int dev = 0;
+    DH_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= DH_MAX_DEVICES) return fail(DH_ERANGE, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(DH_EHIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                     hipGetErrorString(e));
+        g_scratch_dev[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return DH_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2808,7 +2844,8 @@ int dh_init(int device)
         wk = 100000;   // 100 MHz, the CDNA constant-rate counter
     g_wall_khz = wk;
     g_inited = true;
-    return DH_OK;
+    double *sp = nullptr;
+    return static_scratch(&sp);
 }
 
 int dh_integrate_nd(dh_instance *table_dev, int ninst, int D, int order, int nmax,
@@ -2914,23 +2951,32 @@ long long dh_factored_slice_doubles(int D, int nmax)
     return fz_slice(D, nmax);
 }
 
-int dh_integrate_factored(dh_instance *table_dev, int ninst, int D, int order, int nmax,
-                          int *counter_dev, const dh_launch *cfg, void *stream)
+}  // extern "C"
+
+namespace {
+
+// Checks shared by the two factored entry points (0 = go ahead, 1 = nothing
+// to do, < 0 = error).
+int factored_args(const dh_instance *table_dev, int ninst, int D, int order, int nmax,
+                  const dh_launch *cfg)
 {
     if (int e = check_init()) return e;
     if (D < 2 || D > DH_MAXD) return fail(DH_EINVAL, "D must be 2..5");
     if (order < 0 || order > 2) return fail(DH_EINVAL, "bad order");
-    if (ninst <= 0) return DH_OK;
-    if (!table_dev || !counter_dev || !cfg) return fail(DH_EINVAL, "null pointer");
+    if (ninst <= 0) return 1;
+    if (!table_dev || !cfg) return fail(DH_EINVAL, "null pointer");
     if (nmax < 3) return fail(DH_EINVAL, "n must be >= 3");
     if (cfg->delj) return fail(DH_EINVAL, "the delta trick is not available on this path");
-    const int waves = cfg->waves;
-    if (waves < 1 || waves > 8) return fail(DH_EINVAL, "waves must be 1..8");
-    long long groups = cfg->groups > 0 ? cfg->groups : std::min<long long>(ninst, g_num_cu);
-    const long long slice = fz_slice(D, nmax);
-    if (!cfg->gamma_dev || cfg->gamma_doubles < groups * slice)
-        return fail(DH_ERANGE, "factorisation workspace too small: need " +
-                                   std::to_string(groups * slice) + " doubles");
+    if (cfg->waves < 1 || cfg->waves > 8) return fail(DH_EINVAL, "waves must be 1..8");
+    return DH_OK;
+}
+
+// Resets the queue head and launches the factored kernel over `groups`
+// workgroups, each with its `slice` of the workspace `ws`.
+int factored_launch(dh_instance *table_dev, int ninst, int D, int order, int nmax,
+                    int *counter_dev, const dh_launch *cfg, double *ws, long long groups,
+                    long long slice, void *stream)
+{
     const void *fn = pick_fz(D, order);
     const size_t lds =
         sizeof(double) * (size_t)nmax * (LDS_GRID_TABLES + LDS_TABLES_PER_AXIS * (size_t)D);
@@ -2938,11 +2984,64 @@ int dh_integrate_factored(dh_instance *table_dev, int ninst, int D, int order, i
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, counter_dev, 0);
     long long ms = cfg->max_steps, ticks = budget_ticks(cfg);
-    double *ws = cfg->gamma_dev;
     long long sl = slice;
     void *args[] = {&table_dev, &ninst, &counter_dev, &ms, &ws, &sl, &ticks};
-    DH_HIP(hipLaunchKernel(fn, dim3((unsigned)groups), dim3(64 * waves), args, lds, st));
+    DH_HIP(hipLaunchKernel(fn, dim3((unsigned)groups), dim3(64 * cfg->waves), args, lds, st));
     return DH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dh_integrate_factored(dh_instance *table_dev, int ninst, int D, int order, int nmax,
+                          int *counter_dev, const dh_launch *cfg, void *stream)
+{
+    if (int e = factored_args(table_dev, ninst, D, order, nmax, cfg)) return e < 0 ? e : DH_OK;
+    if (!counter_dev) return fail(DH_EINVAL, "null pointer");
+    long long groups = cfg->groups > 0 ? cfg->groups : std::min<long long>(ninst, g_num_cu);
+    const long long slice = fz_slice(D, nmax);
+    if (!cfg->gamma_dev || cfg->gamma_doubles < groups * slice)
+        return fail(DH_ERANGE, "factorisation workspace too small: need " +
+                                   std::to_string(groups * slice) + " doubles");
+    return factored_launch(table_dev, ninst, D, order, nmax, counter_dev, cfg, cfg->gamma_dev,
+                           groups, slice, stream);
+}
+
+long long dh_static_scratch_doubles(void) { return DH_STATIC_WORKSPACE; }
+
+int dh_integrate_factored_static(dh_instance *table_dev, int ninst, int D, int order,
+                                 int nmax, const dh_launch *cfg, void *stream)
+{
+    if (int e = factored_args(table_dev, ninst, D, order, nmax, cfg)) return e < 0 ? e : DH_OK;
+    if (cfg->gamma_dev)
+        return fail(DH_EINVAL, "gamma_dev must be NULL: this entry point uses the library's "
+                               "static scratch");
+    const long long slice = fz_slice(D, nmax);
+    const long long fit = DH_STATIC_WORKSPACE / slice;
+    if (fit < 1)
+        return fail(DH_ERANGE, "one workgroup's factorisation workspace (" +
+                                   std::to_string(slice) + " doubles for " + std::to_string(D) +
+                                   " populations on " + std::to_string(nmax) +
+                                   " points) exceeds the static scratch of " +
+                                   std::to_string(DH_STATIC_WORKSPACE) + " doubles");
+    long long groups;
+    if (cfg->groups > 0) {
+        groups = cfg->groups;
+        if (groups > fit)
+            return fail(DH_ERANGE, std::to_string(groups) + " workgroups need " +
+                                       std::to_string(groups * slice) +
+                                       " doubles of workspace; the static scratch holds " +
+                                       std::to_string(DH_STATIC_WORKSPACE) + " (at most " +
+                                       std::to_string(fit) + " workgroups)");
+    } else {
+        groups = std::min<long long>(std::min<long long>(ninst, g_num_cu), fit);
+    }
+    double *sp = nullptr;
+    if (int e = static_scratch(&sp)) return e;
+    int *counter = reinterpret_cast<int *>(sp);
+    return factored_launch(table_dev, ninst, D, order, nmax, counter, cfg, sp + DH_STATIC_HEAD,
+                           groups, slice, stream);
 }
 
 size_t dh_1d_wave_lds_bytes(int nmax)

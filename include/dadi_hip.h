@@ -6,7 +6,8 @@
  * Conventions
  *   - Every pointer named *_dev is a device pointer; the caller owns all
  *     memory (the Python layer allocates it as torch tensors).  The library
- *     never allocates device memory.
+ *     never allocates device memory; its only device storage is the static
+ *     scratch array of dh_integrate_factored_static.
  *   - Every entry point returns 0 on success and a negative DH_E* code on
  *     failure; dh_last_error() then holds a message.  Errors are per host
  *     thread.
@@ -223,6 +224,26 @@ int dh_integrate_factored(dh_instance *table_dev, int ninst, int D, int order, i
                           int *counter_dev, const dh_launch *cfg, void *stream);
 /* Workspace doubles one workgroup of dh_integrate_factored needs (0 if D or nmax is out of range). */
 long long dh_factored_slice_doubles(int D, int nmax);
+/*
+ * dh_integrate_factored without any caller-provided device memory besides
+ * the instance table, phi, grids and schedules: the factorisation workspace
+ * and the queue counter live in a static device array of the library
+ * (dh_static_scratch_doubles() doubles of workspace per GPU, 32 MiB, held
+ * until the process ends).  Allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * hipGraph capture, tight serving loops, harnesses that forbid allocation).
+ * cfg->gamma_dev must be NULL.  cfg->groups = 0 launches min(ninst, compute
+ * units, workgroups whose workspaces fit); an explicit cfg->groups, or a
+ * single workgroup's dh_factored_slice_doubles(D, nmax), beyond the scratch
+ * is rejected with DH_ERANGE.  Results are those of dh_integrate_factored.
+ * Every call on a GPU shares that GPU's scratch: calls that overlap on one
+ * GPU (other streams or host threads) race and must be serialised by the
+ * caller.
+ */
+int dh_integrate_factored_static(dh_instance *table_dev, int ninst, int D, int order,
+                                 int nmax, const dh_launch *cfg, void *stream);
+/* Workspace doubles of the static scratch of dh_integrate_factored_static. */
+long long dh_static_scratch_doubles(void);
 /* LDS per workgroup of dh_integrate_1d_wave, bytes. */
 size_t dh_1d_wave_lds_bytes(int nmax);
 
