@@ -101,6 +101,9 @@ _SIGS = {
                                       ctypes.c_int, ctypes.POINTER(DhLaunch), _vp],
                                      ctypes.c_int),
     "dh_static_scratch_doubles": ([], ctypes.c_longlong),
+    "dh_integrate_resident": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp,
+                               ctypes.POINTER(DhLaunch), _vp], ctypes.c_int),
+    "dh_resident_rows": ([ctypes.c_int], ctypes.c_int),
     "dh_sweep_axis": ([ctypes.POINTER(DhInstance), ctypes.c_int, ctypes.c_int, _vp,
                        ctypes.c_longlong, _vp, ctypes.c_longlong, ctypes.c_int, _vp],
                       ctypes.c_int),

@@ -1515,6 +1515,500 @@ k_integrate_packed(dh_instance *__restrict__ tab, int ninst, int *__restrict__ c
 }
 
 // ---------------------------------------------------------------------------
+// Kernel 1c: 2-population integrations with phi resident in registers.
+// One workgroup of ceil(NP/64) waves per integration, pulled from the queue.
+// Lane L holds line L of the axis being swept: the column phi[., L] for the x
+// sweep, the row phi[L, .] for the y sweep (the other coordinate of both
+// lines is x_L: the two axes share the grid).  phi is read from memory when
+// the workgroup takes an integration up and written back when it puts it
+// down; in between every step runs on chip.  The Thomas forward pass keeps
+// u' over phi in registers and gamma_j in accumulation registers, the back
+// substitution finishes the line in place, and between the sweeps the lines
+// are transposed through an LDS tile.  The arithmetic is row0/fwd_core's, so
+// the results are the other kernels' bits.  A kernel built for NP rows takes
+// n in (NP - 8, NP]: rows below NP - 8 are known to exist and not to be the
+// last, the last 8 are guarded.
+// LDS: x, dx (2 NP) | per axis the coefficient tables of every row j,
+//      interleaved as xInt_j, dfactor_j, W0_j, W1_j, S_j (5 NP each: the grid
+//      tables are repeated per axis so that one address reaches a row's five
+//      values) | transpose tile 64 x (NP + 1).
+// ---------------------------------------------------------------------------
+constexpr int RR_MAX_NP = 112;
+constexpr int RR_SPAN = 8;   // n values one kernel covers
+
+template <int NP>
+constexpr int rr_waves() { return (NP + 63) / 64; }
+
+template <int NP>
+constexpr size_t rr_lds_doubles() { return 12 * (size_t)NP + 64 * (size_t)(NP + 1); }
+
+// Returns `off` unchanged, through an empty asm that also consumes `dep`: no
+// instruction is emitted, but whatever is addressed through the result
+// cannot be loaded before `dep` is computed.  Row j's coefficient tables are
+// addressed through the u of row j - RR_AHEAD, so their loads (and the
+// coefficient arithmetic on them) start a fixed distance ahead of the
+// recurrence instead of all at once; hoisted, every row's coefficients would
+// be live together, and within a segment they are even invariant across
+// steps.  `off` is the byte offset of the axis' table block in LDS, so each
+// table read is one LDS load with an immediate offset.
+constexpr int RR_AHEAD = 2;
+
+// A zero the compiler cannot see through, produced anew at each use: values
+// formed from it inside the step loop (the line setup of a sweep) stay inside
+// the loop instead of being hoisted out and kept live across steps.
+__device__ __forceinline__ int rr_opaque_zero()
+{
+    int z;
+    asm volatile("s_mov_b32 %0, 0" : "=s"(z));
+    return z;
+}
+
+__device__ __forceinline__ int rr_after(int off, double dep)
+{
+    asm volatile("" : "+v"(off) : "v"(dep));
+    return off;
+}
+
+// A double held in two accumulation registers (the unified register file's
+// second half), moved with v_accvgpr_write/read: gamma_j is written once in
+// the forward pass and read once in the back substitution.
+struct RrAcc {
+    unsigned lo, hi;
+};
+
+__device__ __forceinline__ RrAcc rr_acc_put(double x)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+    RrAcc r;
+    asm volatile("v_accvgpr_write_b32 %0, %1" : "=a"(r.lo) : "v"((unsigned)b));
+    asm volatile("v_accvgpr_write_b32 %0, %1" : "=a"(r.hi) : "v"((unsigned)(b >> 32)));
+    return r;
+}
+
+__device__ __forceinline__ double rr_acc_get(const RrAcc &r)
+{
+    unsigned lo, hi;
+    asm volatile("v_accvgpr_read_b32 %0, %1" : "=v"(lo) : "a"(r.lo));
+    asm volatile("v_accvgpr_read_b32 %0, %1" : "=v"(hi) : "a"(r.hi));
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// Rows of a resident line kept in vector registers; the rest (the line's
+// last rows) live in accumulation registers beside gamma.  About 78 vector
+// registers stay free for the row arithmetic and the line's coefficients.
+template <int NP>
+constexpr int rr_vrows() { return NP <= 88 ? NP : 88; }
+
+// This lane's line: rows 0..NP-1, row j in v[j] (j < PV) or a[j - PV].
+// Indices are compile-time constants after unrolling, so get/set fold to a
+// register access (or two accumulation-register moves).
+template <int NP>
+struct RrLine {
+    static constexpr int PV = rr_vrows<NP>();
+    double v[PV];
+    RrAcc a[NP > PV ? NP - PV : 1];
+    __device__ __forceinline__ double get(int j) const
+    {
+        return j < PV ? v[j < PV ? j : 0] : rr_acc_get(a[j >= PV ? j - PV : 0]);
+    }
+    __device__ __forceinline__ void set(int j, double x)
+    {
+        if (j < PV)
+            v[j < PV ? j : 0] = x;
+        else
+            a[j >= PV ? j - PV : 0] = rr_acc_put(x);
+    }
+};
+
+// One implicit half-step of this lane's line (rows 0..n-1); injection first
+// (row 0 or 1 of this line, or none).
+constexpr int RR_TS = 5;   // interleaved coefficient tables: stride per row
+#ifndef RR_FENCE
+#define RR_FENCE() __builtin_amdgcn_sched_barrier(0)
+#endif
+
+// Grid tables of the interleaved layout (x and dx apart, xInt and dfactor in
+// `blk`), with build_grid_tables' operations.
+__device__ __forceinline__ void rr_grid_tables(const double *__restrict__ xx, int n, double *sx,
+                                               double *sdx, double *blk, int tid, int nthr)
+{
+    for (int i = tid; i < n; i += nthr) sx[i] = xx[i];
+    __syncthreads();
+    for (int i = tid; i < n - 1; i += nthr) {
+        sdx[i] = sx[i + 1] - sx[i];
+        blk[i * RR_TS] = 0.5 * (sx[i + 1] + sx[i]);
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += nthr) {
+        double v;
+        if (i == 0) v = 2. / sdx[0];
+        else if (i == n - 1) v = 2. / sdx[n - 2];
+        else v = 2. / (sdx[i] + sdx[i - 1]);
+        blk[i * RR_TS + 1] = v;
+    }
+}
+
+// W0, W1, S of axis a in the interleaved layout, with build_axis_tables'
+// operations (2 populations: no beta factor).
+__device__ __forceinline__ void rr_axis_tables(const dh_segment *sp, int a, int n,
+                                               const double *sx, const double *sdx, double *blk,
+                                               int tid, int nthr)
+{
+    const double nu = sp->nu[a], g = sp->gamma[a], h = sp->h[a];
+    const int chrom = sp->chrom;
+    for (int i = tid; i < n - 1; i += nthr) {
+        const double vi = d_vfunc(sx[i], nu), vj = d_vfunc(sx[i + 1], nu);
+        blk[i * RR_TS + 2] = vi / (2 * sdx[i]);
+        blk[i * RR_TS + 3] = vj / (2 * sdx[i]);
+        blk[i * RR_TS + 4] = d_sel(blk[i * RR_TS], g, h, chrom);
+    }
+}
+
+// Next row's tables become this row's.
+__device__ __forceinline__ void rr_take(double &xi, double &df, double &w0, double &w1,
+                                        double &sl, double nxi, double ndf, double nw0,
+                                        double nw1, double nsl)
+{
+    xi = nxi;
+    df = ndf;
+    w0 = nw0;
+    w1 = nw1;
+    sl = nsl;
+}
+
+// Loads row j's tables (after `dep`: see rr_after).
+template <int NP>
+__device__ __forceinline__ void rr_fetch(const double *lds, int &off, double dep, int j,
+                                         double &xi, double &df, double &w0, double &w1,
+                                         double &sl)
+{
+    off = rr_after(off, dep);
+    const double *T = (const double *)((const char *)lds + off) + j * RR_TS;
+    xi = T[0];
+    df = T[1];
+    w0 = T[2];
+    w1 = T[3];
+    sl = T[4];
+}
+
+// A row j that is not the last: coefficients of row j with the carry (pa,
+// pc) of interface j-1 (row_coeffs), bet_j, u_j (fwd_core), then
+// gamma_{j+1} = c_j / bet_j into g, and the carry advanced to interface j.
+template <int ORDER>
+__device__ __forceinline__ void rr_mid_row(double pj, double xi, double df, double w0,
+                                           double w1, double sl, double c, double m, double dt,
+                                           double inv_dt, double &pa, double &pc, double &g,
+                                           double &bet, double &u)
+{
+    const double a = -df * pa;
+    const double M = m * (c - xi) + sl;
+    const double na = M * 0.5 + w0;
+    const double nc = -M * (1 - 0.5) + w1;
+    double b;
+    if constexpr (ORDER == DH_ORDER_TV)
+        b = (inv_dt + df * pc) + df * na;
+    else
+        b = (df * na + df * pc) + inv_dt;
+    bet = b - a * g;
+    const double r = (ORDER == DH_ORDER_PRE) ? inv_dt * pj : pj / dt;
+    u = (r - a * u) / bet;
+    g = (-df * nc) / bet;
+    pa = na;
+    pc = nc;
+}
+
+// The last row: its boundary term (a select, which gives the bits of the
+// conditional add), bet and u.
+template <int ORDER>
+__device__ __forceinline__ void rr_last_row(double pj, double df, double dt, double inv_dt,
+                                            bool bcl_on, double bcl, double &pa, double &pc,
+                                            double g, double &bet, double &u)
+{
+    const double a = -df * pa;
+    double b;
+    if constexpr (ORDER == DH_ORDER_TV) {
+        b = inv_dt + df * pc;
+        b = bcl_on ? b + bcl : b;
+    } else {
+        b = df * pc;
+        b = bcl_on ? b + bcl : b;
+        b = b + inv_dt;
+    }
+    bet = b - a * g;
+    const double r = (ORDER == DH_ORDER_PRE) ? inv_dt * pj : pj / dt;
+    u = (r - a * u) / bet;
+    pa = 0.;
+    pc = 0.;
+}
+
+// One implicit half-step of this lane's line (rows 0..n-1), the axis'
+// interleaved tables at lds + off (bytes); injection first (row 0 or 1 of
+// this line, or none).  The operations are those of row0 / row_coeffs /
+// fwd_core (compute_abc_nobc, dadi/integration_shared.c:72-93, and
+// dadi/tridiag.c:22-33), regrouped into one block per row j: the row's
+// coefficients, bet_j = b_j - a_j gamma_j, then the two quotients by bet_j,
+// u_j = (r_j - a_j u_{j-1}) / bet_j and gamma_{j+1} = c_j / bet_j, side by
+// side (c_j = -dfactor_j ctemp_j needs only row j's tables).  A block starts
+// with the next row's table loads, and a scheduling fence keeps them there,
+// one block ahead of their use.
+template <int NP, int ORDER>
+__device__ __forceinline__ void rr_sweep(RrLine<NP> &P, RrAcc (&G)[NP], int n,
+                                         const double *lds, int off, double c, double m,
+                                         double dt, double inv_dt, bool bcf_on, double bcf,
+                                         bool bcl_on, double bcl, int inj_row, double inj)
+{
+    constexpr int SURE = NP - RR_SPAN;   // rows 0..SURE exist; rows < SURE are not the last
+    if (inj_row == 0) P.set(0, P.get(0) + inj);
+    if (inj_row == 1) P.set(1, P.get(1) + inj);
+    const double *T = (const double *)((const char *)lds + off);
+    // row 0's tables, then row 1's in flight
+    double xi = T[0], df = T[1], w0 = T[2], w1 = T[3], sl = T[4];
+    double nxi = T[RR_TS], ndf = T[RR_TS + 1], nw0 = T[RR_TS + 2], nw1 = T[RR_TS + 3],
+           nsl = T[RR_TS + 4];
+    double bet, u, pa, pc, g;
+    {
+        // row 0 (row0)
+        const double M = m * (c - xi) + sl;
+        pa = M * 0.5 + w0;
+        pc = -M * (1 - 0.5) + w1;
+        double b;
+        if constexpr (ORDER == DH_ORDER_TV) {
+            b = inv_dt + df * pa;
+            b = bcf_on ? b + bcf : b;
+        } else {
+            b = df * pa;
+            b = bcf_on ? b + bcf : b;
+            b = b + inv_dt;
+        }
+        const double p0 = P.get(0);
+        const double r = (ORDER == DH_ORDER_PRE) ? inv_dt * p0 : p0 / dt;
+        bet = b;
+        u = r / bet;
+        P.set(0, u);
+        g = (-df * pc) / bet;   // gamma_1 = c_0 / bet_0
+        G[1] = rr_acc_put(g);
+    }
+    // rows 1 .. SURE-1: present and not the last
+#pragma unroll
+    for (int j = 1; j < SURE; ++j) {
+        RR_FENCE();
+        rr_take(xi, df, w0, w1, sl, nxi, ndf, nw0, nw1, nsl);
+        rr_fetch<NP>(lds, off, u, j + 1, nxi, ndf, nw0, nw1, nsl);
+        rr_mid_row<ORDER>(P.get(j), xi, df, w0, w1, sl, c, m, dt, inv_dt, pa, pc, g, bet, u);
+        P.set(j, u);
+        G[j + 1] = rr_acc_put(g);
+    }
+    // rows SURE .. NP-1: row SURE is present, the others if j < n; the last
+    // present row takes the boundary term
+#pragma unroll
+    for (int j = SURE; j < NP; ++j) {
+        RR_FENCE();
+        rr_take(xi, df, w0, w1, sl, nxi, ndf, nw0, nw1, nsl);
+        if (j + 1 < NP) rr_fetch<NP>(lds, off, u, j + 1, nxi, ndf, nw0, nw1, nsl);
+        if (j == SURE || j < n) {
+            if (j < n - 1) {
+                rr_mid_row<ORDER>(P.get(j), xi, df, w0, w1, sl, c, m, dt, inv_dt, pa, pc, g,
+                                  bet, u);
+                P.set(j, u);
+                if (j + 1 < NP) G[j + 1] = rr_acc_put(g);
+            } else {
+                rr_last_row<ORDER>(P.get(j), df, dt, inv_dt, bcl_on, bcl, pa, pc, g, bet, u);
+                P.set(j, u);
+                if (j + 1 < NP) G[j + 1] = rr_acc_put(0.);   // defined on every path
+            }
+        } else if (j + 1 < NP) {
+            G[j + 1] = rr_acc_put(0.);
+        }
+    }
+#pragma unroll
+    for (int j = NP - 2; j >= 0; --j) {
+        if (j < SURE || j <= n - 2) {
+            const double v = P.get(j) - rr_acc_get(G[j + 1]) * u;
+            P.set(j, v);
+            u = v;
+        }
+    }
+}
+
+// Transposes the n x n matrix whose line L lane L holds (row k of the line =
+// M[k][L] -> M[L][k]) through the LDS tile (64 rows of pitch NP + 1).  With
+// two waves it runs in two halves, each lane exporting rows 0..63 and then
+// 64..n-1; wave 0 imports its lines after the first half, so it first keeps
+// its rows 64..n-1 in X (accumulation registers, free between sweeps).
+// `wave` is uniform (from readfirstlane).
+template <int NP>
+__device__ __forceinline__ void rr_transpose(RrLine<NP> &P, RrAcc (&X)[NP > 64 ? NP - 64 : 1],
+                                             int n, int L, int wave, double *__restrict__ tile)
+{
+    constexpr int LDW = NP + 1;
+    if constexpr (NP <= 64) {
+        if (L < n) {
+#pragma unroll
+            for (int k = 0; k < NP; ++k)
+                if (k < n) tile[k * LDW + L] = P.get(k);
+        }
+        wave_sync();
+        if (L < n) {
+#pragma unroll
+            for (int k = 0; k < NP; ++k)
+                if (k < n) P.set(k, tile[L * LDW + k]);
+        }
+        wave_sync();
+    } else {
+        static_assert(NP <= 128, "two waves hold at most 128 lines");
+        if (L < n) {
+#pragma unroll
+            for (int k = 0; k < 64; ++k) tile[k * LDW + L] = P.get(k);
+        }
+        __syncthreads();
+        // both waves fill X (only wave 0 needs it): defined on every path, X
+        // is dead outside the transpose
+#pragma unroll
+        for (int q = 0; q < NP - 64; ++q) X[q] = rr_acc_put(P.get(64 + q));
+        if (wave == 0) {
+#pragma unroll
+            for (int k = 0; k < NP; ++k)
+                if (k < n) P.set(k, tile[L * LDW + k]);
+        }
+        __syncthreads();
+        if (L < n) {
+            if (wave == 0) {
+#pragma unroll
+                for (int q = 0; q < NP - 64; ++q)
+                    if (64 + q < n) tile[q * LDW + L] = rr_acc_get(X[q]);
+            } else {
+#pragma unroll
+                for (int q = 0; q < NP - 64; ++q)
+                    if (64 + q < n) tile[q * LDW + L] = P.get(64 + q);
+            }
+        }
+        __syncthreads();
+        if (wave == 1 && L < n) {
+#pragma unroll
+            for (int k = 0; k < NP; ++k)
+                if (k < n) P.set(k, tile[(L - 64) * LDW + k]);
+        }
+        __syncthreads();
+    }
+}
+
+template <int NP, int ORDER>
+__global__ void __launch_bounds__(64 * rr_waves<NP>()) __attribute__((amdgpu_waves_per_eu(1, 1)))
+k_integrate_rr(dh_instance *__restrict__ tab, int ninst, int *__restrict__ counter,
+               long long max_steps, long long max_ticks)
+{
+    extern __shared__ double lds[];
+    __shared__ int s_next, s_stop;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int L = tid;
+    double *sx = lds, *sdx = sx + NP;
+    double *blk0 = sdx + NP, *blk1 = blk0 + 5 * NP;   // per axis: xInt, dfactor, W0, W1, S
+    double *tile = blk1 + 5 * NP;
+    for (;;) {
+        __syncthreads();
+        if (tid == 0) s_next = atomicAdd(counter, 1);
+        __syncthreads();
+        const int id = __builtin_amdgcn_readfirstlane(s_next);
+        if (id >= ninst) break;
+        dh_instance *ip = tab + id;
+        const int n = __builtin_amdgcn_readfirstlane(ip->n);
+        if (ip->D != 2 || n <= NP - RR_SPAN || n > NP) {
+            // not this kernel's shape: left untouched, reported as not done
+            if (tid == 0) ip->status = -1;
+            continue;
+        }
+        __builtin_assume(n > NP - RR_SPAN && n <= NP);
+        rr_grid_tables(gbl(ip->xx_dev), n, sx, sdx, blk0, tid, nthr);
+        __syncthreads();
+        for (int i = tid; i < n; i += nthr) {
+            blk1[i * RR_TS] = blk0[i * RR_TS];
+            blk1[i * RR_TS + 1] = blk0[i * RR_TS + 1];
+        }
+        long long seg = ip->cur_seg, step = ip->cur_step;
+        const long long nseg = ip->nseg;
+        long long budget = max_steps > 0 ? max_steps : LLONG_MAX;
+        const int frozen = ip->frozen, injm = ip->inject;
+        int first_axis = 0;
+        while (first_axis < 2 && ((frozen >> first_axis) & 1)) ++first_axis;
+        double *phi = gbl(ip->phi_dev);
+        const long long ld = ip->ld > 0 ? ip->ld : n;
+        RrLine<NP> P;
+        RrAcc G[NP];
+        RrAcc X[NP > 64 ? NP - 64 : 1];
+        if (L < n) {
+#pragma unroll
+            for (int i = 0; i < NP; ++i)
+                if (i < n) P.set(i, phi[i * ld + L]);
+        }
+        // all loads issued here: their addresses need not stay live
+        asm volatile("" ::: "memory");
+        const long long t0 = max_ticks > 0 ? (long long)wall_clock64() : 0;
+        bool out_of_time = false;
+        if (first_axis == 2) seg = nseg;   // every axis frozen: the identity
+        while (seg < nseg && budget > 0 && !out_of_time) {
+            const dh_segment *sp = gbl(ip->seg_dev) + seg;
+            __syncthreads();
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                if ((frozen >> a) & 1) continue;
+                rr_axis_tables(sp, a, n, sx, sdx, a ? blk1 : blk0, tid, nthr);
+            }
+            __syncthreads();
+            const double dt = sp->dt, inv_dt = 1. / dt, inj = sp->inject;
+            const int line = L < n ? L : 0;
+            long long nst = sp->nsteps - step;
+            if (nst > budget) nst = budget;
+            for (long long k = 0; k < nst; ++k) {
+#pragma unroll 1
+                for (int a = 0; a < 2; ++a) {
+                    if (!((frozen >> a) & 1)) {
+                        // the line's boundary terms, injection row and other
+                        // coordinate, formed again for each sweep (through an
+                        // opaque zero, so they are not kept live across steps)
+                        LineSetup<2, ORDER> Ls;
+                        const double mg[1] = {sp->mig[a][0]};
+                        line_setup<2, ORDER>(Ls, line + rr_opaque_zero(), n, a, sx, mg,
+                                             sp->nu[a], sp->gamma[a], sp->h[a], sdx[0],
+                                             sdx[n - 2], first_axis == a, injm, sp->chrom);
+                        const int off = (int)((const char *)(a ? blk1 : blk0) - (const char *)lds);
+                        rr_sweep<NP, ORDER>(P, G, n, lds, off, Ls.c[0], mg[0], dt, inv_dt,
+                                            Ls.bcf_on, Ls.bcf, Ls.bcl_on, Ls.bcl, Ls.inj_row,
+                                            inj);
+                    }
+                    rr_transpose<NP>(P, X, n, L, wave, tile);
+                }
+                if (max_ticks > 0) {
+                    if (tid == 0) s_stop = (long long)wall_clock64() - t0 >= max_ticks;
+                    __syncthreads();
+                    if (s_stop) {
+                        nst = k + 1;
+                        out_of_time = true;
+                        break;
+                    }
+                }
+            }
+            budget -= nst;
+            step += nst;
+            if (step >= sp->nsteps) {
+                ++seg;
+                step = 0;
+            }
+        }
+        if (L < n) {
+#pragma unroll
+            for (int i = 0; i < NP; ++i)
+                if (i < n) phi[i * ld + L] = P.get(i);
+        }
+        if (tid == 0) {
+            ip->cur_seg = seg;
+            ip->cur_step = step;
+            ip->status = seg >= nseg ? 1 : 0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Kernel 2: 1D instances, one lane each, lanes pull instances independently.
 // Scratch layout (coalesced over lanes): [q][i][slot], q = xInt, df, W0, W1,
 // S, gamma.
@@ -2540,6 +3034,35 @@ const void *pick_sweep(int D, int order)
 template <int D, int ORDER>
 const void *fz_kernel() { return reinterpret_cast<const void *>(&k_integrate_fz<D, ORDER>); }
 
+// register-resident 2D kernel: the row counts NP it is built for (each takes
+// n in (NP - 8, NP]); the coefficient orders of 2 populations (TV, PRE)
+template <int NP>
+const void *rr_kernel(int order)
+{
+    switch (order) {
+    case DH_ORDER_TV: return reinterpret_cast<const void *>(&k_integrate_rr<NP, DH_ORDER_TV>);
+    case DH_ORDER_PRE: return reinterpret_cast<const void *>(&k_integrate_rr<NP, DH_ORDER_PRE>);
+    }
+    return nullptr;
+}
+
+template <int NP>
+size_t rr_lds_bytes_of() { return sizeof(double) * rr_lds_doubles<NP>(); }
+
+// kernel and LDS bytes for n rows (null if n has no kernel)
+const void *pick_rr(int n, int order, size_t *lds, int *waves)
+{
+#define DH_RR(np)                                                   \
+    if (n > (np) - RR_SPAN && n <= (np)) {                          \
+        *lds = rr_lds_bytes_of<np>();                               \
+        *waves = rr_waves<np>();                                    \
+        return rr_kernel<np>(order);                                \
+    }
+    DH_RR(104)
+#undef DH_RR
+    return nullptr;
+}
+
 const void *pick_fz(int D, int order)
 {
     switch (D * 4 + order) {
@@ -3006,6 +3529,39 @@ int dh_integrate_factored(dh_instance *table_dev, int ninst, int D, int order, i
                                    std::to_string(groups * slice) + " doubles");
     return factored_launch(table_dev, ninst, D, order, nmax, counter_dev, cfg, cfg->gamma_dev,
                            groups, slice, stream);
+}
+
+int dh_resident_rows(int n)
+{
+    size_t lds = 0;
+    int waves = 0;
+    return pick_rr(n, DH_ORDER_PRE, &lds, &waves) ? 1 : 0;
+}
+
+int dh_integrate_resident(dh_instance *table_dev, int ninst, int order, int n,
+                          int *counter_dev, const dh_launch *cfg, void *stream)
+{
+    if (int e = check_init()) return e;
+    if (order != DH_ORDER_TV && order != DH_ORDER_PRE)
+        return fail(DH_EINVAL, "order must be TV or PRE (2 populations)");
+    if (ninst <= 0) return DH_OK;
+    if (!table_dev || !counter_dev || !cfg) return fail(DH_EINVAL, "null pointer");
+    size_t lds = 0;
+    int waves = 0;
+    const void *fn = pick_rr(n, order, &lds, &waves);
+    if (!fn)
+        return fail(DH_ERANGE, "no register-resident kernel for n = " + std::to_string(n));
+    if (int e = set_lds(fn, lds)) return e;
+    // one workgroup per SIMD pair: each wave needs a SIMD's whole register file
+    const int per_cu = std::max(1, 4 / waves);
+    long long groups = cfg->groups > 0 ? cfg->groups
+                                       : std::min<long long>(ninst, (long long)per_cu * g_num_cu);
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, counter_dev, 0);
+    long long ms = cfg->max_steps, ticks = budget_ticks(cfg);
+    void *args[] = {&table_dev, &ninst, &counter_dev, &ms, &ticks};
+    DH_HIP(hipLaunchKernel(fn, dim3((unsigned)groups), dim3(64 * waves), args, lds, st));
+    return DH_OK;
 }
 
 long long dh_static_scratch_doubles(void) { return DH_STATIC_WORKSPACE; }

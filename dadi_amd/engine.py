@@ -61,6 +61,9 @@ class Context:
     #: each integration's sweeps over several workgroups; 0 disables it
     spread_max_inst = 512
     spread_waves = 0          # 0: chosen per batch
+    #: 2D batches whose grid size has a register-resident kernel use it when
+    #: no other launch shape is asked for; False keeps them on the others
+    resident_auto = True
 
     def one_d_wave(self, ninst, nmax):
         """Whether a 1D batch runs one wave per integration (LDS-resident,
@@ -561,13 +564,40 @@ class PackedBatch:
         (device copy; the caller resets phi)."""
         self.tab.copy_(self.tab0)
 
-    def launch(self, waves=None, groups=0, max_steps=0, stream=None, chunk=None,
-               recompute=None, gamma_global=None, slots=None):
-        """One persistent launch over the table (max_steps > 0: at most that
-        many steps per instance).  Shape arguments left None are autotuned /
-        chosen by rule of thumb; slots=1 forbids packing."""
+    def resident_ok(self):
+        """Whether the register-resident 2D kernel (dh_integrate_resident) can
+        run this batch: two populations, one grid size it is built for, the
+        TV or PRE coefficient order, no delta trick."""
+        if self.D != 2 or self.delj or self.order not in (_lib.ORDER_TV, _lib.ORDER_PRE):
+            return False
+        if any(j.n != self.nmax for j in self.jobs):
+            return False
+        return bool(self.ctx.lib.dh_resident_rows(self.nmax))
+
+    def launch_resident(self, groups=0, max_steps=0, max_us=0, stream=None):
+        """One dh_integrate_resident launch over the table."""
         ctx = self.ctx
         stream = ctx.stream() if stream is None else stream
+        cfg = DhLaunch(1, groups, max_steps, 0, 0)
+        cfg.max_us = int(max_us)
+        check(ctx.lib.dh_integrate_resident(self.tab.data_ptr(), len(self.jobs), self.order,
+                                            self.nmax, self.counter.data_ptr(), cfg, stream),
+              "dh_integrate_resident")
+
+    def launch(self, waves=None, groups=0, max_steps=0, stream=None, chunk=None,
+               recompute=None, gamma_global=None, slots=None, resident=None):
+        """One persistent launch over the table (max_steps > 0: at most that
+        many steps per instance).  Shape arguments left None are autotuned /
+        chosen by rule of thumb; slots=1 forbids packing.  resident: True runs
+        the register-resident 2D kernel, False never does; None uses it when
+        it can run the batch and no other launch shape is asked for."""
+        ctx = self.ctx
+        stream = ctx.stream() if stream is None else stream
+        if resident or (resident is None and ctx.resident_auto and self.resident_ok() and
+                        all(v is None for v in (waves, chunk, recompute, gamma_global, slots))):
+            if not self.resident_ok():
+                raise ValueError("the register-resident kernel cannot run this batch")
+            return self.launch_resident(groups, max_steps, 0, stream)
         if self.D == 1:
             if ctx.one_d_wave(len(self.jobs), self.nmax):
                 cfg = DhLaunch(1, groups, max_steps, 0, 0)

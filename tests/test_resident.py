@@ -1,0 +1,120 @@
+"""
+The register-resident 2D kernel (dh_integrate_resident): phi held in
+registers for a whole integration, lines transposed through LDS between the
+sweeps.  Its results must be the bits of the other kernels and of the oracle:
+split_mig draws on the grid sizes it is built for, both coefficient orders
+(constant parameters and time-varying ones), frozen populations, integrations
+run in quanta of a few steps, and an instance of another shape left alone.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def eng():
+    import dadi_amd
+    assert dadi_amd.cuda_enabled(True)
+    from dadi_amd import engine
+    return engine
+
+
+def _draws(rng, k, tmax):
+    out = []
+    for _ in range(k):
+        nu1, nu2 = np.exp(rng.uniform(np.log(0.1), np.log(10), 2))
+        out.append((nu1, nu2, rng.uniform(0.001, tmax), rng.uniform(0, 5)))
+    return out
+
+
+def _jobs(eng, oracle, rng, n, k, tv=False, frozen_every=0, tmax=0.02):
+    from dadi_amd import schedule as S, _lib
+    ctx = eng.context()
+    xx = oracle.default_grid(n)
+    phi0 = oracle.phi_1D_to_2D(xx, oracle.phi_1D(xx)) + rng.uniform(0, 0.1, (n, n))
+    jobs, refs = [], []
+    for i, (nu1, nu2, T, m) in enumerate(_draws(rng, k, tmax)):
+        fz = [False, False]
+        if frozen_every and i % frozen_every == 1:
+            fz = [i % 2 == 0, i % 2 == 1]
+        migs = [[0.0 if fz[0] else m], [0.0 if fz[1] else m]]
+        g = [float(rng.uniform(-3, 3)), float(rng.uniform(-3, 3))]
+        h = [0.5, float(rng.uniform(0.1, 0.9))]
+        if tv:
+            c = lambda v: (lambda t, v=v: v)
+            nu1f = (lambda a: (lambda t: a * (1 + 5 * t)))(nu1)
+            seg = S.tv_segments(2, xx, 0.0, T, [nu1f, c(nu2)], [[c(v) for v in ms] for ms in migs],
+                                [c(v) for v in g], [c(v) for v in h], c(1.0), None)
+            order = _lib.ORDER_TV
+            ref = oracle.integrate_tv(phi0, xx, T, [nu1f, nu2], migs, g, h, frozen=fz)
+        else:
+            dt = min(S.compute_dt(nu1, migs[0], g[0], h[0]), S.compute_dt(nu2, migs[1], g[1], h[1]))
+            seg = S.const_segments(2, xx, 0.0, T, dt, [nu1, nu2], migs, g, h, 1.0)
+            order = _lib.ORDER_PRE
+            ref = oracle.integrate_const(phi0, xx, T, [nu1, nu2], migs, g, h, frozen=fz)
+        fzm, inj = S.masks(2, fz, [False, False])
+        jobs.append(eng.Job(ctx.empty_phi((n, n)), xx, seg, order, fzm, inj))
+        jobs[-1].phi.copy_(ctx.to_dev(phi0))
+        refs.append(ref)
+    return jobs, refs, phi0
+
+
+@pytest.mark.parametrize("n", [97, 100, 104])
+def test_resident_equals_packed_and_oracle(eng, oracle, n):
+    ctx = eng.context()
+    assert ctx.lib.dh_resident_rows(n) == 1
+    rng = np.random.default_rng(n)
+    jobs, refs, phi0 = _jobs(eng, oracle, rng, n, 24, frozen_every=5)
+    pb = eng.PackedBatch(ctx, jobs)
+    assert pb.resident_ok()
+    pb.launch(resident=True)
+    got = [j.phi.cpu().numpy() for j in jobs]
+    for g, r in zip(got, refs):
+        assert np.array_equal(g, r)
+    # the packed kernel on the same table gives the same bits
+    for j in jobs:
+        j.phi.copy_(ctx.to_dev(phi0))
+    pb.restart()
+    pb.launch(resident=False, slots=3)
+    for j, g in zip(jobs, got):
+        assert np.array_equal(j.phi.cpu().numpy(), g)
+
+
+def test_resident_time_varying_and_quanta(eng, oracle):
+    ctx = eng.context()
+    rng = np.random.default_rng(3)
+    jobs, refs, _ = _jobs(eng, oracle, rng, 100, 6, tv=True, tmax=0.004)
+    pb = eng.PackedBatch(ctx, jobs)
+    # a few steps per launch, resuming from the recorded progress
+    for _ in range(10000):
+        pb.launch(resident=True, max_steps=7)
+        st = pb.tab.cpu().numpy().view(eng.INST_DTYPE)["status"]
+        if np.all(st == 1):
+            break
+    assert np.all(st == 1)
+    for j, r in zip(jobs, refs):
+        assert np.array_equal(j.phi.cpu().numpy(), r)
+
+
+def test_resident_rejects_other_shapes(eng, oracle):
+    """An instance whose n the kernel is not built for is left untouched
+    with status -1; the others complete."""
+    from dadi_amd import _lib
+    ctx = eng.context()
+    rng = np.random.default_rng(4)
+    jobs, refs, phi0 = _jobs(eng, oracle, rng, 100, 3)
+    other, _, phi_o = _jobs(eng, oracle, rng, 60, 1)
+    pb = eng.PackedBatch(ctx, jobs)
+    tab = pb.tab.cpu().numpy().view(eng.INST_DTYPE).copy()
+    # point instance 1 at a 60-point phi (its record says n = 60)
+    ot = eng.PackedBatch(ctx, other).tab.cpu().numpy().view(eng.INST_DTYPE)
+    tab[1] = ot[0]
+    pb.tab.copy_(ctx.to_dev(tab.view(np.uint8)))
+    pb.launch_resident()
+    st = pb.tab.cpu().numpy().view(eng.INST_DTYPE)["status"]
+    assert list(st) == [1, -1, 1]
+    assert np.array_equal(other[0].phi.cpu().numpy(), phi_o)
+    assert np.array_equal(jobs[0].phi.cpu().numpy(), refs[0])
+    assert np.array_equal(jobs[2].phi.cpu().numpy(), refs[2])
+    assert ctx.lib.dh_resident_rows(60) in (0, 1)
