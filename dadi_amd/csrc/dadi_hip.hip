@@ -1515,7 +1515,8 @@ k_integrate_packed(dh_instance *__restrict__ tab, int ninst, int *__restrict__ c
 }
 
 // ---------------------------------------------------------------------------
-// Kernel 1c: 2-population integrations with phi resident in registers.
+// Kernel 1c: 2-population integrations with phi resident in registers
+// (n from 33 to 104 points, in kernels for 8 consecutive n each).
 // One workgroup of ceil(NP/64) waves per integration, pulled from the queue.
 // Lane L holds line L of the axis being swept: the column phi[., L] for the x
 // sweep, the row phi[L, .] for the y sweep (the other coordinate of both
@@ -3058,7 +3059,7 @@ const void *pick_rr(int n, int order, size_t *lds, int *waves)
         *waves = rr_waves<np>();                                    \
         return rr_kernel<np>(order);                                \
     }
-    DH_RR(104)
+    DH_RR(40) DH_RR(48) DH_RR(56) DH_RR(64) DH_RR(72) DH_RR(80) DH_RR(88) DH_RR(96) DH_RR(104)
 #undef DH_RR
     return nullptr;
 }
@@ -3535,7 +3536,7 @@ int dh_resident_rows(int n)
 {
     size_t lds = 0;
     int waves = 0;
-    return pick_rr(n, DH_ORDER_PRE, &lds, &waves) ? 1 : 0;
+    return pick_rr(n, DH_ORDER_PRE, &lds, &waves) ? (n + RR_SPAN - 1) / RR_SPAN * RR_SPAN : 0;
 }
 
 int dh_integrate_resident(dh_instance *table_dev, int ninst, int order, int n,

@@ -442,6 +442,9 @@ def run_single(phi, xx, seg, order, frozen, inject, ctx=None, mode="auto", waves
         if mode == "wg" and D == 2 and not delj and ctx.factored_max_inst > 0 and \
                 ctx.factorable([seg]):
             mode = "factored"
+        if D == 2 and not delj and ctx.resident_auto and \
+                order in (_lib.ORDER_TV, _lib.ORDER_PRE) and ctx.lib.dh_resident_rows(n):
+            mode = "resident"
     stream = ctx.stream()
     if mode == "wg":
         # one instance: the full gamma column (less VALU) when it fits
@@ -456,6 +459,15 @@ def run_single(phi, xx, seg, order, frozen, inject, ctx=None, mode="auto", waves
         cfg.delj = dj
         check(ctx.lib.dh_integrate_nd(tab.data_ptr(), 1, D, order, n, counter.data_ptr(),
                                       cfg, stream), "dh_integrate_nd")
+        keep = (tab, counter, segd)
+    elif mode == "resident":
+        rec = _inst_record(phid.data_ptr(), xxd.data_ptr(), segd.data_ptr(), len(seg), D, n,
+                           order, frozen, inject, ld)
+        tab = ctx.to_dev(rec.view(np.uint8))
+        counter = t.zeros(1, dtype=t.int32, device=ctx.device)
+        cfg = DhLaunch(1, 1, 0, 0, 0)
+        check(ctx.lib.dh_integrate_resident(tab.data_ptr(), 1, order, n, counter.data_ptr(),
+                                            cfg, stream), "dh_integrate_resident")
         keep = (tab, counter, segd)
     elif mode == "factored":
         rec = _inst_record(phid.data_ptr(), xxd.data_ptr(), segd.data_ptr(), len(seg), D, n,
@@ -566,13 +578,12 @@ class PackedBatch:
 
     def resident_ok(self):
         """Whether the register-resident 2D kernel (dh_integrate_resident) can
-        run this batch: two populations, one grid size it is built for, the
-        TV or PRE coefficient order, no delta trick."""
+        run this batch: two populations, grid sizes within one of its kernels'
+        ranges, the TV or PRE coefficient order, no delta trick."""
         if self.D != 2 or self.delj or self.order not in (_lib.ORDER_TV, _lib.ORDER_PRE):
             return False
-        if any(j.n != self.nmax for j in self.jobs):
-            return False
-        return bool(self.ctx.lib.dh_resident_rows(self.nmax))
+        rows = self.ctx.lib.dh_resident_rows(self.nmax)
+        return rows > 0 and all(self.ctx.lib.dh_resident_rows(j.n) == rows for j in self.jobs)
 
     def launch_resident(self, groups=0, max_steps=0, max_us=0, stream=None):
         """One dh_integrate_resident launch over the table."""

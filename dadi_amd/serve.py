@@ -166,7 +166,7 @@ class Server:
 
     def __init__(self, budget_bytes=8 << 30, quantum=200, ctx=None, quantum_ms=8.0,
                  wide_lines=512, wide_streams=2, steps_per_graph=16, wide_waves=2,
-                 factored=True, wide_mode="graph"):
+                 factored=True, wide_mode="graph", resident=True):
         import torch
         self.torch = torch
         self.ctx = ctx or engine.context()
@@ -178,6 +178,7 @@ class Server:
         self.spg = max(1, int(steps_per_graph))
         self.wide_waves = int(wide_waves)
         self.factored = bool(factored)
+        self.resident = bool(resident)
         if wide_mode not in ("spread", "graph"):
             raise ValueError("wide_mode is 'spread' or 'graph'")
         self.wide_mode = wide_mode
@@ -369,13 +370,25 @@ class Server:
         self.wide_launches += 1
         return st, ("spread", D, order), q, ev0
 
+    def _class_of(self, r):
+        """The launch class of a request's current epoch: (populations,
+        order), and for 2 populations whose grid has a register-resident
+        kernel also that kernel's row count (one launch serves one such
+        kernel)."""
+        D, order = int(r.rec["D"][0]), int(r.rec["order"][0])
+        if D == 2 and self.resident and order in (_lib.ORDER_TV, _lib.ORDER_PRE):
+            rows = self.ctx.lib.dh_resident_rows(int(r.rec["n"][0]))
+            if rows:
+                return (D, order, rows)
+        return (D, order)
+
     def _launch(self, group):
-        """One quantum for every in-flight epoch of one (D, order) class."""
+        """One quantum for every in-flight epoch of one class (_class_of)."""
         t = self.torch
         ctx = self.ctx
         D, order = group[0].rec["D"][0], group[0].rec["order"][0]
         tab = np.concatenate([r.rec for r in group])
-        key = (int(D), int(order))
+        key = self._class_of(group[0])
         q = self._steps_for(key)
         st = self._stream(key)
         # splits and uploads of this quantum were issued on the current stream
@@ -407,6 +420,12 @@ class Server:
                 _lib.check(ctx.lib.dh_integrate_1d(tabd.data_ptr(), len(group), int(order), nmax,
                                                    counter.data_ptr(), scr.data_ptr(), scr.numel(),
                                                    cfg, h), "dh_integrate_1d")
+            elif len(key) == 3:
+                cfg = _lib.DhLaunch(1, 0, 0, 0, 0)   # a time budget, not a step count
+                cfg.max_us = budget_us
+                _lib.check(ctx.lib.dh_integrate_resident(tabd.data_ptr(), len(group), int(order),
+                                                         nmax, counter.data_ptr(), cfg, h),
+                           "dh_integrate_resident")
             elif self.factored and ctx.factorable([r.seg for r in group]):
                 ws = self.scratch.setdefault(key, {})
                 ctx.factored_launch(tabd.data_ptr(), len(group), int(D), int(order), nmax,
@@ -451,8 +470,7 @@ class Server:
             elif status == "wide":
                 waiting.setdefault(("wide", r.rid), []).append(r)
             else:
-                key = (int(r.rec["D"][0]), int(r.rec["order"][0]))
-                waiting.setdefault(key, []).append(r)
+                waiting.setdefault(self._class_of(r), []).append(r)
 
         while pending or waiting or inflight:
             now = clock() - t0

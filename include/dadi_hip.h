@@ -246,15 +246,15 @@ int dh_integrate_factored_static(dh_instance *table_dev, int ninst, int D, int o
 long long dh_static_scratch_doubles(void);
 
 /*
- * Runs ninst 2-population instances on n points each (every instance of the
- * call has the same n), one workgroup per instance at a time, pulled from a
- * device-side queue, with phi resident in registers: a workgroup reads phi
- * when it takes an instance up and writes it back when it puts it down, and
- * every time step in between runs on chip (each lane holds one line of phi;
- * the lines are transposed through LDS between the two sweeps).  Results are
- * the bits of dh_integrate_nd.  Available for the n where
- * dh_resident_rows(n) is 1; order DH_ORDER_TV or DH_ORDER_PRE.  An instance
- * of another shape is left untouched with status -1.  cfg->groups (0 = one
+ * Runs ninst 2-population instances, one workgroup per instance at a time,
+ * pulled from a device-side queue, with phi resident in registers: a
+ * workgroup reads phi when it takes an instance up and writes it back when it
+ * puts it down, and every time step in between runs on chip (each lane holds
+ * one line of phi; the lines are transposed through LDS between the two
+ * sweeps).  Results are the bits of dh_integrate_nd.  A kernel serves 8
+ * consecutive grid sizes: with R = dh_resident_rows(n), every instance must
+ * have between R - 7 and R points; order DH_ORDER_TV or DH_ORDER_PRE.  An
+ * instance of another shape is left untouched with status -1.  cfg->groups (0 = one
  * workgroup per pair of SIMDs) and cfg->max_steps / cfg->max_us as for
  * dh_integrate_nd; no workspace, no delta trick.
  * Replaces, like dh_integrate_nd, the per-step loop of
@@ -263,7 +263,9 @@ long long dh_static_scratch_doubles(void);
  */
 int dh_integrate_resident(dh_instance *table_dev, int ninst, int order, int n,
                           int *counter_dev, const dh_launch *cfg, void *stream);
-/* 1 if dh_integrate_resident has a kernel for n points, else 0. */
+/* Rows R of the dh_integrate_resident kernel for n points (it serves
+ * n = R-7 .. R; R is a multiple of 8), or 0 when there is none (n outside
+ * 33..104). */
 int dh_resident_rows(int n);
 /* LDS per workgroup of dh_integrate_1d_wave, bytes. */
 size_t dh_1d_wave_lds_bytes(int nmax);

@@ -60,10 +60,10 @@ def _jobs(eng, oracle, rng, n, k, tv=False, frozen_every=0, tmax=0.02):
     return jobs, refs, phi0
 
 
-@pytest.mark.parametrize("n", [97, 100, 104])
+@pytest.mark.parametrize("n", [33, 40, 57, 64, 65, 80, 90, 97, 100, 104])
 def test_resident_equals_packed_and_oracle(eng, oracle, n):
     ctx = eng.context()
-    assert ctx.lib.dh_resident_rows(n) == 1
+    assert ctx.lib.dh_resident_rows(n) == (n + 7) // 8 * 8
     rng = np.random.default_rng(n)
     jobs, refs, phi0 = _jobs(eng, oracle, rng, n, 24, frozen_every=5)
     pb = eng.PackedBatch(ctx, jobs)
@@ -97,6 +97,34 @@ def test_resident_time_varying_and_quanta(eng, oracle):
         assert np.array_equal(j.phi.cpu().numpy(), r)
 
 
+def test_resident_sizes():
+    """The grid sizes with a kernel: 33 to 104 points."""
+    import dadi_amd
+    assert dadi_amd.cuda_enabled(True)
+    from dadi_amd import engine
+    ctx = engine.context()
+    have = [n for n in range(3, 200) if ctx.lib.dh_resident_rows(n)]
+    assert have == list(range(33, 105))
+    assert {ctx.lib.dh_resident_rows(n) for n in have} == set(range(40, 105, 8))
+
+
+def test_resident_mixed_sizes_in_one_kernel(eng, oracle):
+    """Instances on different grid sizes served by one kernel (97 to 104
+    points) run in one launch, each to the oracle's bits."""
+    ctx = eng.context()
+    rng = np.random.default_rng(9)
+    jobs, refs = [], []
+    for n in (97, 100, 104, 101):
+        j, r, _ = _jobs(eng, oracle, rng, n, 3)
+        jobs += j
+        refs += r
+    pb = eng.PackedBatch(ctx, jobs)
+    assert pb.resident_ok()
+    pb.launch(resident=True)
+    for j, r in zip(jobs, refs):
+        assert np.array_equal(j.phi.cpu().numpy(), r)
+
+
 def test_resident_rejects_other_shapes(eng, oracle):
     """An instance whose n the kernel is not built for is left untouched
     with status -1; the others complete."""
@@ -117,4 +145,4 @@ def test_resident_rejects_other_shapes(eng, oracle):
     assert np.array_equal(other[0].phi.cpu().numpy(), phi_o)
     assert np.array_equal(jobs[0].phi.cpu().numpy(), refs[0])
     assert np.array_equal(jobs[2].phi.cpu().numpy(), refs[2])
-    assert ctx.lib.dh_resident_rows(60) in (0, 1)
+    assert ctx.lib.dh_resident_rows(60) == 64
