@@ -1627,6 +1627,9 @@ constexpr int RR_TS = 5;   // interleaved coefficient tables: stride per row
 #ifndef RR_FENCE
 #define RR_FENCE() __builtin_amdgcn_sched_barrier(0)
 #endif
+#ifndef RR_FENCE_EVERY
+#define RR_FENCE_EVERY 4   // rows per scheduling block
+#endif
 
 // Grid tables of the interleaved layout (x and dx apart, xInt and dfactor in
 // `blk`), with build_grid_tables' operations.
@@ -1677,6 +1680,58 @@ __device__ __forceinline__ void rr_take(double &xi, double &df, double &w0, doub
     sl = nsl;
 }
 
+// Quotients by bet_j, two per row, from one reciprocal.  x / b compiles to
+//   y = rcp(b'); y = fma(y, fma(-b', y, 1), y) (twice); q = x' * y;
+//   e = fma(-b', q, x'); result = div_fixup(div_fmas(e, y, q), b, x)
+// where b' and x' come from div_scale.  div_scale rescales, and div_fmas and
+// div_fixup adjust, only near the ends of the exponent range (a zero,
+// denormal or non-finite operand, b beyond 2^1022, x below 2^-969, x / b
+// denormal or above 2^767); elsewhere b' = b, x' = x and the result is
+// fma(e, y, q).  rr_recip and rr_quot are that sequence with b' = b and
+// x' = x, so y is shared by both quotients of a row.  rr_in records whether
+// the operands were inside the range where this gives the bits of x / b:
+// with |b| in [2^-60, 2^100] and the quotient in [2^-900, 2^600] (the
+// quotient is accurate to an ulp or two, so these margins cover its error)
+// every condition above holds.  So does x = +0 with such a b: div_fixup
+// returns the zero of sign(b), and so does fma(e, y, q) (q = +0 * y, e = +0);
+// for x = -0 and b > 0 it would give +0, not -0.  A lane that leaves the
+// range marks its integration for the exactly dividing kernel.
+__device__ __forceinline__ double rr_recip(double b)
+{
+    double y = __builtin_amdgcn_rcp(b);
+    double t = __builtin_fma(-b, y, 1.0);
+    y = __builtin_fma(y, t, y);
+    t = __builtin_fma(-b, y, 1.0);
+    return __builtin_fma(y, t, y);
+}
+
+__device__ __forceinline__ double rr_quot(double x, double b, double y)
+{
+    const double q = x * y;
+    const double e = __builtin_fma(-b, q, x);
+    return __builtin_fma(e, y, q);
+}
+
+__device__ __forceinline__ bool rr_in(double x, double lo, double hi)
+{
+    return (__builtin_fabs(x) >= lo) & (__builtin_fabs(x) <= hi);
+}
+
+constexpr double RR_DEN_LO = 0x1p-60, RR_DEN_HI = 0x1p100;
+constexpr double RR_QUO_LO = 0x1p-900, RR_QUO_HI = 0x1p600;
+
+// okm &= the lanes where the quotient q of x by an in-range b is exact: q in
+// the quotient range, or x = +0.  A ballot of each compare, folded into okm
+// at once (an empty asm holds okm in scalar registers, so the masks are not
+// kept for a later fold).
+__device__ __forceinline__ void rr_keep_exact(unsigned long long &okm, double q, double x)
+{
+    okm &= (__builtin_amdgcn_ballot_w64(__builtin_fabs(q) >= RR_QUO_LO) &
+            __builtin_amdgcn_ballot_w64(__builtin_fabs(q) <= RR_QUO_HI)) |
+           __builtin_amdgcn_ballot_w64(__builtin_amdgcn_class(x, 64));   // class 64: +0
+    asm volatile("" : "+s"(okm));
+}
+
 // Loads row j's tables (after `dep`: see rr_after).
 template <int NP>
 __device__ __forceinline__ void rr_fetch(const double *lds, int &off, double dep, int j,
@@ -1692,41 +1747,58 @@ __device__ __forceinline__ void rr_fetch(const double *lds, int &off, double dep
     sl = T[4];
 }
 
-// A row j that is not the last: coefficients of row j with the carry (pa,
-// pc) of interface j-1 (row_coeffs), bet_j, u_j (fwd_core), then
-// gamma_{j+1} = c_j / bet_j into g, and the carry advanced to interface j.
+// Row 0's coefficients (row0): the carry (pa, pc) of interface 0 and
+// b_0 = bet_0, the boundary term by a select (which gives the bits of the
+// conditional add).
 template <int ORDER>
-__device__ __forceinline__ void rr_mid_row(double pj, double xi, double df, double w0,
-                                           double w1, double sl, double c, double m, double dt,
-                                           double inv_dt, double &pa, double &pc, double &g,
-                                           double &bet, double &u)
+__device__ __forceinline__ double rr_row0(double xi, double df, double w0, double w1, double sl,
+                                          double c, double m, double inv_dt, bool bcf_on,
+                                          double bcf, double &pa, double &pc)
 {
-    const double a = -df * pa;
+    const double M = m * (c - xi) + sl;
+    pa = M * 0.5 + w0;
+    pc = -M * (1 - 0.5) + w1;
+    double b;
+    if constexpr (ORDER == DH_ORDER_TV) {
+        b = inv_dt + df * pa;
+        b = bcf_on ? b + bcf : b;
+    } else {
+        b = df * pa;
+        b = bcf_on ? b + bcf : b;
+        b = b + inv_dt;
+    }
+    return b;
+}
+
+// Coefficients of a row j that is not the last (row_coeffs) from its tables
+// and the carry (pa, pc) of interface j-1: a_j, b_j and c_j = -dfactor_j
+// ctemp_j, the carry advanced to interface j.
+template <int ORDER>
+__device__ __forceinline__ void rr_coef_mid(double xi, double df, double w0, double w1,
+                                            double sl, double c, double m, double inv_dt,
+                                            double &pa, double &pc, double &a, double &b,
+                                            double &cj)
+{
+    a = -df * pa;
     const double M = m * (c - xi) + sl;
     const double na = M * 0.5 + w0;
     const double nc = -M * (1 - 0.5) + w1;
-    double b;
     if constexpr (ORDER == DH_ORDER_TV)
         b = (inv_dt + df * pc) + df * na;
     else
         b = (df * na + df * pc) + inv_dt;
-    bet = b - a * g;
-    const double r = (ORDER == DH_ORDER_PRE) ? inv_dt * pj : pj / dt;
-    u = (r - a * u) / bet;
-    g = (-df * nc) / bet;
+    cj = -df * nc;
     pa = na;
     pc = nc;
 }
 
-// The last row: its boundary term (a select, which gives the bits of the
-// conditional add), bet and u.
+// Coefficients of the last row: a_j and b_j with its boundary term (a
+// select, which gives the bits of the conditional add).
 template <int ORDER>
-__device__ __forceinline__ void rr_last_row(double pj, double df, double dt, double inv_dt,
-                                            bool bcl_on, double bcl, double &pa, double &pc,
-                                            double g, double &bet, double &u)
+__device__ __forceinline__ void rr_coef_last(double df, double inv_dt, bool bcl_on, double bcl,
+                                             double pa, double pc, double &a, double &b)
 {
-    const double a = -df * pa;
-    double b;
+    a = -df * pa;
     if constexpr (ORDER == DH_ORDER_TV) {
         b = inv_dt + df * pc;
         b = bcl_on ? b + bcl : b;
@@ -1735,85 +1807,167 @@ __device__ __forceinline__ void rr_last_row(double pj, double df, double dt, dou
         b = bcl_on ? b + bcl : b;
         b = b + inv_dt;
     }
-    bet = b - a * g;
+}
+
+// Row j that is not the last, from its coefficients (a, b, cj) and gamma_j
+// in g: bet_j = b_j - a_j gamma_j, u_j (fwd_core), gamma_{j+1} = c_j / bet_j
+// into g.  FAST: the quotients by rr_recip / rr_quot (and pj / dt from
+// y_dt, the reciprocal of dt), gamma_{j+1} only if fast_g (else by
+// division: the last rows, where c_j = -0 at x = 1 when the advection term
+// is zero); u_j's exactness goes into okm (bet_j and gamma_{j+1} are the same
+// at every step of a segment: rr_coeffs_in_range checks them once).
+template <int ORDER, bool FAST>
+__device__ __forceinline__ void rr_solve_mid(double pj, double a, double b, double cj,
+                                             double dt, double inv_dt, double y_dt, bool fast_g,
+                                             double &g, double &u, unsigned long long &okm)
+{
+    const double bet = b - a * g;
+    if constexpr (FAST) {
+        double r;
+        const double y = rr_recip(bet);
+        if constexpr (ORDER == DH_ORDER_PRE) {
+            r = inv_dt * pj;
+        } else {
+            r = rr_quot(pj, dt, y_dt);
+            rr_keep_exact(okm, r, pj);
+        }
+        const double x = r - a * u;
+        u = rr_quot(x, bet, y);
+        rr_keep_exact(okm, u, x);
+        g = fast_g ? rr_quot(cj, bet, y) : cj / bet;
+    } else {
+        const double r = (ORDER == DH_ORDER_PRE) ? inv_dt * pj : pj / dt;
+        u = (r - a * u) / bet;
+        g = cj / bet;
+    }
+}
+
+// The last row: bet and u.
+template <int ORDER>
+__device__ __forceinline__ void rr_solve_last(double pj, double a, double b, double dt,
+                                              double inv_dt, double g, double &u)
+{
+    const double bet = b - a * g;
     const double r = (ORDER == DH_ORDER_PRE) ? inv_dt * pj : pj / dt;
     u = (r - a * u) / bet;
-    pa = 0.;
-    pc = 0.;
 }
+
+// Whether every bet_j of rows 1..n-2 of this lane's line (the axis' tables
+// at T) is inside rr_recip / rr_quot's exact range, and every gamma_{j+1}
+// of rows 1..fast_rows-1 (those rr_sweep computes by rr_quot) is exact, by
+// the operations of rr_sweep's rows: within a segment these values are the
+// same at every step.
+template <int ORDER>
+__device__ __forceinline__ bool rr_coeffs_in_range(int n, int fast_rows, const double *T,
+                                                   double c, double m, double inv_dt,
+                                                   bool bcf_on, double bcf)
+{
+    double pa, pc;
+    const double bet0 = rr_row0<ORDER>(T[0], T[1], T[2], T[3], T[4], c, m, inv_dt, bcf_on,
+                                       bcf, pa, pc);
+    double g = (-T[1] * pc) / bet0;
+    bool in = true;
+#pragma unroll 1
+    for (int j = 1; j < n - 1; ++j) {
+        const double *R = T + j * RR_TS;
+        double a, b, cj;
+        rr_coef_mid<ORDER>(R[0], R[1], R[2], R[3], R[4], c, m, inv_dt, pa, pc, a, b, cj);
+        const double bet = b - a * g;
+        in = in & rr_in(bet, RR_DEN_LO, RR_DEN_HI);
+        if (j < fast_rows) {
+            g = rr_quot(cj, bet, rr_recip(bet));
+            in = in & (rr_in(g, RR_QUO_LO, RR_QUO_HI) |
+                       (__builtin_bit_cast(unsigned long long, cj) == 0));
+        } else {
+            g = cj / bet;
+        }
+    }
+    return in;
+}
+
+// Rows 1..rr_fast_g_rows-1 of an NP-row kernel compute gamma_{j+1} by
+// rr_quot; the rest divide (row n-2 is at least NP - 9).
+template <int NP>
+constexpr int rr_fast_g_rows() { return NP - RR_SPAN - 1; }
 
 // One implicit half-step of this lane's line (rows 0..n-1), the axis'
 // interleaved tables at lds + off (bytes); injection first (row 0 or 1 of
 // this line, or none).  The operations are those of row0 / row_coeffs /
 // fwd_core (compute_abc_nobc, dadi/integration_shared.c:72-93, and
-// dadi/tridiag.c:22-33), regrouped into one block per row j: the row's
-// coefficients, bet_j = b_j - a_j gamma_j, then the two quotients by bet_j,
-// u_j = (r_j - a_j u_{j-1}) / bet_j and gamma_{j+1} = c_j / bet_j, side by
-// side (c_j = -dfactor_j ctemp_j needs only row j's tables).  A block starts
-// with the next row's table loads, and a scheduling fence keeps them there,
-// one block ahead of their use.
-template <int NP, int ORDER>
+// dadi/tridiag.c:22-33), regrouped into one block per row j, pipelined:
+// block j takes row j+1's tables (loaded in block j-1) and starts the loads
+// of row j+2's, solves row j (bet_j = b_j - a_j gamma_j, then the two
+// quotients by bet_j, u_j = (r_j - a_j u_{j-1}) / bet_j and
+// gamma_{j+1} = c_j / bet_j), and forms row j+1's coefficients, which need
+// neither u nor gamma, beside that chain.  A scheduling fence between blocks
+// keeps each block's loads a block ahead of their use.
+template <int NP, int ORDER, bool FAST>
 __device__ __forceinline__ void rr_sweep(RrLine<NP> &P, RrAcc (&G)[NP], int n,
                                          const double *lds, int off, double c, double m,
-                                         double dt, double inv_dt, bool bcf_on, double bcf,
-                                         bool bcl_on, double bcl, int inj_row, double inj)
+                                         double dt, double inv_dt, double y_dt, bool bcf_on,
+                                         double bcf, bool bcl_on, double bcl, int inj_row,
+                                         double inj, unsigned long long &okm)
 {
     constexpr int SURE = NP - RR_SPAN;   // rows 0..SURE exist; rows < SURE are not the last
     if (inj_row == 0) P.set(0, P.get(0) + inj);
     if (inj_row == 1) P.set(1, P.get(1) + inj);
     const double *T = (const double *)((const char *)lds + off);
-    // row 0's tables, then row 1's in flight
-    double xi = T[0], df = T[1], w0 = T[2], w1 = T[3], sl = T[4];
-    double nxi = T[RR_TS], ndf = T[RR_TS + 1], nw0 = T[RR_TS + 2], nw1 = T[RR_TS + 3],
-           nsl = T[RR_TS + 4];
-    double bet, u, pa, pc, g;
+    double u, pa, pc, g;
+    double ca, cb, cc;   // coefficients a, b, c of the next row to solve
+    // tables of the row after it (in flight)
+    double nxi, ndf, nw0, nw1, nsl;
     {
-        // row 0 (row0)
-        const double M = m * (c - xi) + sl;
-        pa = M * 0.5 + w0;
-        pc = -M * (1 - 0.5) + w1;
-        double b;
-        if constexpr (ORDER == DH_ORDER_TV) {
-            b = inv_dt + df * pa;
-            b = bcf_on ? b + bcf : b;
-        } else {
-            b = df * pa;
-            b = bcf_on ? b + bcf : b;
-            b = b + inv_dt;
-        }
+        // row 0 (row0), then row 1's coefficients; row 2's tables in flight
+        const double df0 = T[1];
+        const double bet = rr_row0<ORDER>(T[0], df0, T[2], T[3], T[4], c, m, inv_dt, bcf_on,
+                                          bcf, pa, pc);
         const double p0 = P.get(0);
         const double r = (ORDER == DH_ORDER_PRE) ? inv_dt * p0 : p0 / dt;
-        bet = b;
         u = r / bet;
         P.set(0, u);
-        g = (-df * pc) / bet;   // gamma_1 = c_0 / bet_0
+        g = (-df0 * pc) / bet;   // gamma_1 = c_0 / bet_0
         G[1] = rr_acc_put(g);
+        rr_coef_mid<ORDER>(T[RR_TS], T[RR_TS + 1], T[RR_TS + 2], T[RR_TS + 3], T[RR_TS + 4], c,
+                           m, inv_dt, pa, pc, ca, cb, cc);
+        rr_fetch<NP>(lds, off, u, 2, nxi, ndf, nw0, nw1, nsl);
     }
-    // rows 1 .. SURE-1: present and not the last
+    // rows 1 .. SURE-1: present and not the last; row j+1 is not the last
+    // either, except possibly for j = SURE-1
 #pragma unroll
     for (int j = 1; j < SURE; ++j) {
-        RR_FENCE();
+        if (j % RR_FENCE_EVERY == 0) RR_FENCE();
+        double xi, df, w0, w1, sl;
         rr_take(xi, df, w0, w1, sl, nxi, ndf, nw0, nw1, nsl);
-        rr_fetch<NP>(lds, off, u, j + 1, nxi, ndf, nw0, nw1, nsl);
-        rr_mid_row<ORDER>(P.get(j), xi, df, w0, w1, sl, c, m, dt, inv_dt, pa, pc, g, bet, u);
+        if (j + 2 < NP) rr_fetch<NP>(lds, off, u, j + 2, nxi, ndf, nw0, nw1, nsl);
+        rr_solve_mid<ORDER, FAST>(P.get(j), ca, cb, cc, dt, inv_dt, y_dt,
+                                  j < rr_fast_g_rows<NP>(), g, u, okm);
         P.set(j, u);
         G[j + 1] = rr_acc_put(g);
+        if (j + 1 < SURE || j + 1 < n - 1)
+            rr_coef_mid<ORDER>(xi, df, w0, w1, sl, c, m, inv_dt, pa, pc, ca, cb, cc);
+        else
+            rr_coef_last<ORDER>(df, inv_dt, bcl_on, bcl, pa, pc, ca, cb);
     }
     // rows SURE .. NP-1: row SURE is present, the others if j < n; the last
     // present row takes the boundary term
 #pragma unroll
     for (int j = SURE; j < NP; ++j) {
-        RR_FENCE();
+        if (j % RR_FENCE_EVERY == 0) RR_FENCE();
+        double xi, df, w0, w1, sl;
         rr_take(xi, df, w0, w1, sl, nxi, ndf, nw0, nw1, nsl);
-        if (j + 1 < NP) rr_fetch<NP>(lds, off, u, j + 1, nxi, ndf, nw0, nw1, nsl);
+        if (j + 2 < NP) rr_fetch<NP>(lds, off, u, j + 2, nxi, ndf, nw0, nw1, nsl);
         if (j == SURE || j < n) {
             if (j < n - 1) {
-                rr_mid_row<ORDER>(P.get(j), xi, df, w0, w1, sl, c, m, dt, inv_dt, pa, pc, g,
-                                  bet, u);
+                rr_solve_mid<ORDER, FAST>(P.get(j), ca, cb, cc, dt, inv_dt, y_dt, false, g, u,
+                                          okm);
                 P.set(j, u);
                 if (j + 1 < NP) G[j + 1] = rr_acc_put(g);
+                if (j + 1 < n - 1)
+                    rr_coef_mid<ORDER>(xi, df, w0, w1, sl, c, m, inv_dt, pa, pc, ca, cb, cc);
+                else
+                    rr_coef_last<ORDER>(df, inv_dt, bcl_on, bcl, pa, pc, ca, cb);
             } else {
-                rr_last_row<ORDER>(P.get(j), df, dt, inv_dt, bcl_on, bcl, pa, pc, g, bet, u);
+                rr_solve_last<ORDER>(P.get(j), ca, cb, dt, inv_dt, g, u);
                 P.set(j, u);
                 if (j + 1 < NP) G[j + 1] = rr_acc_put(0.);   // defined on every path
             }
@@ -1893,7 +2047,19 @@ __device__ __forceinline__ void rr_transpose(RrLine<NP> &P, RrAcc (&X)[NP > 64 ?
     }
 }
 
-template <int NP, int ORDER>
+// Whether a lane of this wave that holds a line left the exact range.
+__device__ __forceinline__ int rr_wave_out(unsigned long long okm, int L, int n)
+{
+    return (~okm & __builtin_amdgcn_ballot_w64(L < n)) != 0;
+}
+
+// FAST: the quotients of rows 1..n-2 by rr_recip / rr_quot.  A workgroup
+// that finds a lane outside their exact range (every 32 steps and before it
+// stores phi) leaves the instance as it took it up and sets bit 0 of its
+// `reserved` word; the exactly dividing kernel (FAST false) runs only the
+// instances with that bit set, from the same state, so every instance ends
+// with the bits of exact division.
+template <int NP, int ORDER, bool FAST>
 __global__ void __launch_bounds__(64 * rr_waves<NP>()) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_integrate_rr(dh_instance *__restrict__ tab, int ninst, int *__restrict__ counter,
                long long max_steps, long long max_ticks)
@@ -1920,6 +2086,9 @@ k_integrate_rr(dh_instance *__restrict__ tab, int ninst, int *__restrict__ count
             continue;
         }
         __builtin_assume(n > NP - RR_SPAN && n <= NP);
+        // the fast kernel leaves instances marked for exact division to the
+        // other kernel, which runs only those
+        if (FAST == ((__builtin_amdgcn_readfirstlane(ip->reserved) & 1) != 0)) continue;
         rr_grid_tables(gbl(ip->xx_dev), n, sx, sdx, blk0, tid, nthr);
         __syncthreads();
         for (int i = tid; i < n; i += nthr) {
@@ -1945,9 +2114,12 @@ k_integrate_rr(dh_instance *__restrict__ tab, int ninst, int *__restrict__ count
         // all loads issued here: their addresses need not stay live
         asm volatile("" ::: "memory");
         const long long t0 = max_ticks > 0 ? (long long)wall_clock64() : 0;
-        bool out_of_time = false;
+        bool out_of_time = false, failed = false;
+        // FAST: the lanes whose quotients so far were in range (lanes past
+        // the last line hold no line and count as in range)
+        unsigned long long okm = __builtin_amdgcn_ballot_w64(true);
         if (first_axis == 2) seg = nseg;   // every axis frozen: the identity
-        while (seg < nseg && budget > 0 && !out_of_time) {
+        while (seg < nseg && budget > 0 && !out_of_time && !failed) {
             const dh_segment *sp = gbl(ip->seg_dev) + seg;
             __syncthreads();
 #pragma unroll
@@ -1957,7 +2129,24 @@ k_integrate_rr(dh_instance *__restrict__ tab, int ninst, int *__restrict__ count
             }
             __syncthreads();
             const double dt = sp->dt, inv_dt = 1. / dt, inj = sp->inject;
+            const double y_dt = rr_recip(dt);
             const int line = L < n ? L : 0;
+            if constexpr (FAST) {
+                bool in = ORDER == DH_ORDER_PRE || rr_in(dt, RR_DEN_LO, RR_DEN_HI);
+#pragma unroll 1
+                for (int a = 0; a < 2; ++a) {
+                    if ((frozen >> a) & 1) continue;
+                    LineSetup<2, ORDER> Ls;
+                    const double mg[1] = {sp->mig[a][0]};
+                    line_setup<2, ORDER>(Ls, line, n, a, sx, mg, sp->nu[a], sp->gamma[a],
+                                         sp->h[a], sdx[0], sdx[n - 2], first_axis == a, injm,
+                                         sp->chrom);
+                    in = in & rr_coeffs_in_range<ORDER>(n, rr_fast_g_rows<NP>(),
+                                                        a ? blk1 : blk0, Ls.c[0], mg[0],
+                                                        inv_dt, Ls.bcf_on, Ls.bcf);
+                }
+                okm &= __builtin_amdgcn_ballot_w64(in || L >= n);
+            }
             long long nst = sp->nsteps - step;
             if (nst > budget) nst = budget;
             for (long long k = 0; k < nst; ++k) {
@@ -1973,11 +2162,16 @@ k_integrate_rr(dh_instance *__restrict__ tab, int ninst, int *__restrict__ count
                                              sp->nu[a], sp->gamma[a], sp->h[a], sdx[0],
                                              sdx[n - 2], first_axis == a, injm, sp->chrom);
                         const int off = (int)((const char *)(a ? blk1 : blk0) - (const char *)lds);
-                        rr_sweep<NP, ORDER>(P, G, n, lds, off, Ls.c[0], mg[0], dt, inv_dt,
-                                            Ls.bcf_on, Ls.bcf, Ls.bcl_on, Ls.bcl, Ls.inj_row,
-                                            inj);
+                        rr_sweep<NP, ORDER, FAST>(P, G, n, lds, off, Ls.c[0], mg[0], dt, inv_dt,
+                                                  y_dt, Ls.bcf_on, Ls.bcf, Ls.bcl_on, Ls.bcl,
+                                                  Ls.inj_row, inj, okm);
                     }
                     rr_transpose<NP>(P, X, n, L, wave, tile);
+                }
+                if (FAST && (k & 31) == 31 && __syncthreads_or(rr_wave_out(okm, L, n))) {
+                    nst = k + 1;
+                    failed = true;
+                    break;
                 }
                 if (max_ticks > 0) {
                     if (tid == 0) s_stop = (long long)wall_clock64() - t0 >= max_ticks;
@@ -1995,6 +2189,11 @@ k_integrate_rr(dh_instance *__restrict__ tab, int ninst, int *__restrict__ count
                 ++seg;
                 step = 0;
             }
+        }
+        if (FAST && (failed || __syncthreads_or(rr_wave_out(okm, L, n)))) {
+            // phi and the progress are as taken up: the exact kernel redoes them
+            if (tid == 0) ip->reserved |= 1;
+            continue;
         }
         if (L < n) {
 #pragma unroll
@@ -3036,13 +3235,16 @@ template <int D, int ORDER>
 const void *fz_kernel() { return reinterpret_cast<const void *>(&k_integrate_fz<D, ORDER>); }
 
 // register-resident 2D kernel: the row counts NP it is built for (each takes
-// n in (NP - 8, NP]); the coefficient orders of 2 populations (TV, PRE)
-template <int NP>
+// n in (NP - 8, NP]); the coefficient orders of 2 populations (TV, PRE); the
+// shared-reciprocal (fast) and the exactly dividing form
+template <int NP, bool FAST>
 const void *rr_kernel(int order)
 {
     switch (order) {
-    case DH_ORDER_TV: return reinterpret_cast<const void *>(&k_integrate_rr<NP, DH_ORDER_TV>);
-    case DH_ORDER_PRE: return reinterpret_cast<const void *>(&k_integrate_rr<NP, DH_ORDER_PRE>);
+    case DH_ORDER_TV:
+        return reinterpret_cast<const void *>(&k_integrate_rr<NP, DH_ORDER_TV, FAST>);
+    case DH_ORDER_PRE:
+        return reinterpret_cast<const void *>(&k_integrate_rr<NP, DH_ORDER_PRE, FAST>);
     }
     return nullptr;
 }
@@ -3051,13 +3253,13 @@ template <int NP>
 size_t rr_lds_bytes_of() { return sizeof(double) * rr_lds_doubles<NP>(); }
 
 // kernel and LDS bytes for n rows (null if n has no kernel)
-const void *pick_rr(int n, int order, size_t *lds, int *waves)
+const void *pick_rr(int n, int order, bool fast, size_t *lds, int *waves)
 {
-#define DH_RR(np)                                                   \
-    if (n > (np) - RR_SPAN && n <= (np)) {                          \
-        *lds = rr_lds_bytes_of<np>();                               \
-        *waves = rr_waves<np>();                                    \
-        return rr_kernel<np>(order);                                \
+#define DH_RR(np)                                                                   \
+    if (n > (np) - RR_SPAN && n <= (np)) {                                          \
+        *lds = rr_lds_bytes_of<np>();                                               \
+        *waves = rr_waves<np>();                                                    \
+        return fast ? rr_kernel<np, true>(order) : rr_kernel<np, false>(order);     \
     }
     DH_RR(40) DH_RR(48) DH_RR(56) DH_RR(64) DH_RR(72) DH_RR(80) DH_RR(88) DH_RR(96) DH_RR(104)
 #undef DH_RR
@@ -3536,7 +3738,8 @@ int dh_resident_rows(int n)
 {
     size_t lds = 0;
     int waves = 0;
-    return pick_rr(n, DH_ORDER_PRE, &lds, &waves) ? (n + RR_SPAN - 1) / RR_SPAN * RR_SPAN : 0;
+    return pick_rr(n, DH_ORDER_PRE, true, &lds, &waves) ? (n + RR_SPAN - 1) / RR_SPAN * RR_SPAN
+                                                         : 0;
 }
 
 int dh_integrate_resident(dh_instance *table_dev, int ninst, int order, int n,
@@ -3549,19 +3752,25 @@ int dh_integrate_resident(dh_instance *table_dev, int ninst, int order, int n,
     if (!table_dev || !counter_dev || !cfg) return fail(DH_EINVAL, "null pointer");
     size_t lds = 0;
     int waves = 0;
-    const void *fn = pick_rr(n, order, &lds, &waves);
-    if (!fn)
+    const void *fast = pick_rr(n, order, true, &lds, &waves);
+    const void *exact = pick_rr(n, order, false, &lds, &waves);
+    if (!fast || !exact)
         return fail(DH_ERANGE, "no register-resident kernel for n = " + std::to_string(n));
-    if (int e = set_lds(fn, lds)) return e;
+    if (int e = set_lds(fast, lds)) return e;
+    if (int e = set_lds(exact, lds)) return e;
     // one workgroup per SIMD pair: each wave needs a SIMD's whole register file
     const int per_cu = std::max(1, 4 / waves);
     long long groups = cfg->groups > 0 ? cfg->groups
                                        : std::min<long long>(ninst, (long long)per_cu * g_num_cu);
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, counter_dev, 0);
     long long ms = cfg->max_steps, ticks = budget_ticks(cfg);
     void *args[] = {&table_dev, &ninst, &counter_dev, &ms, &ticks};
-    DH_HIP(hipLaunchKernel(fn, dim3((unsigned)groups), dim3(64 * waves), args, lds, st));
+    // the fast kernel, then the exact one over the instances it handed over
+    // (those whose quotients left the fast kernel's exact range)
+    for (const void *fn : {fast, exact}) {
+        hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, counter_dev, 0);
+        DH_HIP(hipLaunchKernel(fn, dim3((unsigned)groups), dim3(64 * waves), args, lds, st));
+    }
     return DH_OK;
 }
 

@@ -89,7 +89,8 @@ typedef struct dh_instance {
     int inject;  /* bit k set: population k receives new mutations */
     int status;  /* 1 when the schedule has completed */
     int ld;      /* row pitch of the last axis in doubles (0 = n) */
-    int reserved;
+    int reserved; /* 0 from the caller; bit 0 is set by dh_integrate_resident for an
+                     instance it integrates with exact division from then on */
 } dh_instance;
 
 /* Launch shape of the workgroup-per-instance kernel. */
@@ -254,7 +255,14 @@ long long dh_static_scratch_doubles(void);
  * sweeps).  Results are the bits of dh_integrate_nd.  A kernel serves 8
  * consecutive grid sizes: with R = dh_resident_rows(n), every instance must
  * have between R - 7 and R points; order DH_ORDER_TV or DH_ORDER_PRE.  An
- * instance of another shape is left untouched with status -1.  cfg->groups (0 = one
+ * instance of another shape is left untouched with status -1.
+ * Two kernels run in turn on the stream: the first divides by bet_j through
+ * one shared reciprocal per row (the operations of IEEE division where no
+ * operand or quotient is near the ends of the exponent range, checked at
+ * run time); an instance where a check fails is left as it was taken up,
+ * marked (bit 0 of `reserved`) and integrated by the second, exactly
+ * dividing, kernel, in this call and every later one.  A time budget
+ * (cfg->max_us) applies to each of the two kernels.  cfg->groups (0 = one
  * workgroup per pair of SIMDs) and cfg->max_steps / cfg->max_us as for
  * dh_integrate_nd; no workspace, no delta trick.
  * Replaces, like dh_integrate_nd, the per-step loop of

@@ -5,6 +5,9 @@ sweeps.  Its results must be the bits of the other kernels and of the oracle:
 split_mig draws on the grid sizes it is built for, both coefficient orders
 (constant parameters and time-varying ones), frozen populations, integrations
 run in quanta of a few steps, and an instance of another shape left alone.
+The fast kernel's shared-reciprocal division must hand every integration
+whose quotients leave its exact range (tiny or zero phi) to the exactly
+dividing kernel, and both must give the oracle's bits.
 """
 import numpy as np
 import pytest
@@ -146,3 +149,92 @@ def test_resident_rejects_other_shapes(eng, oracle):
     assert np.array_equal(jobs[0].phi.cpu().numpy(), refs[0])
     assert np.array_equal(jobs[2].phi.cpu().numpy(), refs[2])
     assert ctx.lib.dh_resident_rows(60) == 64
+
+
+def _mark_exact(eng, ctx, pb, which):
+    tab = pb.tab.cpu().numpy().view(eng.INST_DTYPE).copy()
+    for i in which:
+        tab[i]["reserved"] |= 1
+    pb.tab.copy_(ctx.to_dev(tab.view(np.uint8)))
+
+
+@pytest.mark.parametrize("tv", [False, True])
+def test_resident_exact_kernel_equals_fast(eng, oracle, tv):
+    """Instances marked for exact division (run by the second kernel only)
+    give the bits of the fast kernel and of the oracle."""
+    ctx = eng.context()
+    rng = np.random.default_rng(17 + tv)
+    jobs, refs, phi0 = _jobs(eng, oracle, rng, 90, 8, tv=tv, frozen_every=3, tmax=0.005)
+    pb = eng.PackedBatch(ctx, jobs)
+    _mark_exact(eng, ctx, pb, range(0, 8, 2))
+    pb.launch(resident=True)
+    tab = pb.tab.cpu().numpy().view(eng.INST_DTYPE)
+    assert list(tab["status"]) == [1] * 8
+    assert list(tab["reserved"] & 1) == [1, 0] * 4
+    for j, r in zip(jobs, refs):
+        assert np.array_equal(j.phi.cpu().numpy(), r)
+        assert np.array_equal(np.signbit(j.phi.cpu().numpy()), np.signbit(r))
+
+
+@pytest.mark.parametrize("tv", [False, True])
+def test_resident_tiny_values_go_to_exact_kernel(eng, oracle, tv):
+    """phi with entries far below 2^-900, or negative zeros, makes quotients
+    leave the fast kernel's exact range: those integrations are marked and
+    redone by the exactly dividing kernel.  Positive zeros (phi after a
+    split is zero off the diagonal) and ordinary values stay on the fast
+    kernel.  Every result is the oracle's, also when run in quanta of a few
+    steps."""
+    from dadi_amd import schedule as S, _lib
+    ctx = eng.context()
+    rng = np.random.default_rng(23 + tv)
+    n = 72
+    xx = oracle.default_grid(n)
+    base = oracle.phi_1D_to_2D(xx, oracle.phi_1D(xx))
+    starts = []
+    for i in range(8):
+        p = base + rng.uniform(0, 0.1, (n, n))
+        if i % 4 == 1:
+            p[:, n // 2:] *= 1e-290          # tiny quotients
+        elif i % 4 == 2:
+            p = base.copy()                  # +0 off the diagonal
+        elif i % 4 == 3:
+            p[:, : n // 2] = -0.0            # -0 numerators
+        starts.append(p)
+    for quanta in (0, 40):
+        jobs, refs = [], []
+        for i, (nu1, nu2, T, m) in enumerate(_draws(rng, 8, 0.003)):
+            migs = [[m], [m]] if i < 4 else [[0.0], [0.0]]
+            g = [0.0, float(rng.uniform(-1, 1))] if i < 4 else [0.0, 0.0]
+            h = [0.5, 0.5]
+            if tv:
+                c = lambda v: (lambda t, v=v: v)
+                nu1f = (lambda a: (lambda t: a * (1 + 5 * t)))(nu1)
+                seg = S.tv_segments(2, xx, 0.0, T, [nu1f, c(nu2)],
+                                    [[c(v) for v in ms] for ms in migs],
+                                    [c(v) for v in g], [c(v) for v in h], c(1.0), None)
+                order = _lib.ORDER_TV
+                ref = oracle.integrate_tv(starts[i], xx, T, [nu1f, nu2], migs, g, h)
+            else:
+                dt = min(S.compute_dt(nu1, migs[0], g[0], h[0]),
+                         S.compute_dt(nu2, migs[1], g[1], h[1]))
+                seg = S.const_segments(2, xx, 0.0, T, dt, [nu1, nu2], migs, g, h, 1.0)
+                order = _lib.ORDER_PRE
+                ref = oracle.integrate_const(starts[i], xx, T, [nu1, nu2], migs, g, h)
+            fzm, inj = S.masks(2, [False, False], [False, False])
+            jobs.append(eng.Job(ctx.empty_phi((n, n)), xx, seg, order, fzm, inj))
+            jobs[-1].phi.copy_(ctx.to_dev(starts[i]))
+            refs.append(ref)
+        pb = eng.PackedBatch(ctx, jobs)
+        for _ in range(10000):
+            pb.launch(resident=True, max_steps=quanta)
+            tab = pb.tab.cpu().numpy().view(eng.INST_DTYPE)
+            if np.all(tab["status"] == 1):
+                break
+        assert np.all(tab["status"] == 1)
+        marked = [int(v) for v in tab["reserved"] & 1]
+        assert marked[0::4] == [0, 0] and marked[1::4] == [1, 1], marked
+        assert marked[2::4] == [0, 0] and marked[3::4] == [1, 1], marked
+        for j, r in zip(jobs, refs):
+            got = j.phi.cpu().numpy()
+            assert np.array_equal(got, r)
+            assert np.array_equal(np.signbit(got), np.signbit(r))   # zeros' signs too
