@@ -8,9 +8,12 @@ T ~ U[0.01, 2], m ~ U[0, 5]; phi0 = phi_1D_to_2D(phi_1D) on default_grid(100);
 one unit = one two_pops(phi0, xx, T, nu1, nu2, m12=m, m21=m) integration (the
 constant-parameter path, bit-identical to dadi's CPU result).
 
-A step = one persistent launch integrating --batch instances (resident in
-HBM, reset from phi0 by a device copy before each launch).  value =
-batch * steps / wall time of the timed steps.
+A step = one launch integrating --batch instances (resident in HBM, reset
+from phi0 by a device copy before each launch): the register-resident kernel
+(each integration's phi held in a workgroup's registers from start to end)
+where the grid size has one, else the packed streaming kernel.  value =
+batch * steps / wall time of the timed steps.  The packed kernel is also
+timed on the same batch for comparison (and must give the same bits).
 
 Prints one JSON line.  --gpus > 1 is refused: this build uses one GPU.
 """
@@ -215,18 +218,32 @@ def model_workloads(args, rng):
 
 def measured_traffic(pts, batch, seed, kernel):
     """HBM bytes per launch of this exact workload and kernel, from the
-    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (each counter scaled
-    by what it reads for a device copy of known size); None when no summary
-    matches."""
-    path = os.path.join(ROOT, "profiles", "traffic_split_mig_2d_pts%d.json" % pts)
-    try:
-        rec = json.load(open(path))
-    except (OSError, ValueError):
-        return None
-    if rec.get("pts") != pts or rec.get("batch") != batch or rec.get("seed") != seed or \
-            rec.get("kernel") != kernel:
-        return None
-    return (rec["fetch_kib"] * rec["fetch_scale"] + rec["write_kib"] * rec["write_scale"]) * 1024
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summaries
+    (profiles/traffic_*.json, each counter scaled by what it reads for a
+    device copy of known size); None when no summary matches."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "traffic_*.json"))):
+        try:
+            rec = json.load(open(path))
+        except (OSError, ValueError):
+            continue
+        if rec.get("pts") == pts and rec.get("batch") == batch and rec.get("seed") == seed \
+                and rec.get("kernel") == kernel:
+            return (rec["fetch_kib"] * rec["fetch_scale"] +
+                    rec["write_kib"] * rec["write_scale"]) * 1024
+    return None
+
+
+def kernel_of(pb, packed=False):
+    """Name (as rocprofv3 lists it, without arguments) of the kernel that
+    pb.launch() runs for a 2-population PRE-order batch: the
+    register-resident kernel where it applies, else the factored kernel for
+    few integrations, else the packed one; packed=True: the packed kernel."""
+    if not packed and pb.resident_ok():
+        return "k_integrate_rr<%d, 1, true>" % pb.ctx.lib.dh_resident_rows(pb.nmax)
+    if not packed and pb.latency_bound():
+        return "k_integrate_fz<2, 1>"
+    return "k_integrate_packed<2, 1>"
 
 
 def main():
@@ -242,7 +259,13 @@ def main():
     ap.add_argument("--recompute", type=int, default=-1)
     ap.add_argument("--gamma-global", type=int, default=-1)
     ap.add_argument("--slots", type=int, default=-1,
-                    help="instances per workgroup (1 = unpacked kernel; -1 = automatic)")
+                    help="packed kernel: instances per workgroup (1 = unpacked kernel; "
+                         "-1 = automatic)")
+    ap.add_argument("--kernel", choices=("auto", "resident", "packed"), default="auto",
+                    help="batched launch: the register-resident kernel where the grid size "
+                         "has one (auto), always it, or the packed streaming kernel")
+    ap.add_argument("--compare-packed", type=int, default=1,
+                    help="also time one packed-kernel launch over the same batch")
     ap.add_argument("--dense", action="store_true",
                     help="store phi rows unpadded (default: rows padded to 64-byte multiples)")
     ap.add_argument("--groups", type=int, default=0,
@@ -326,11 +349,20 @@ def main():
 
     shape = ctx.shape(2, n, args.waves or None, opt(args.chunk), opt(args.recompute),
                       opt(args.gamma_global), opt(args.slots), packable=pb.packable)
+    resident = args.kernel == "resident" or (args.kernel == "auto" and pb.resident_ok())
+    if resident and not pb.resident_ok():
+        raise SystemExit("no register-resident kernel for pts=%d" % n)
 
-    def launch():
+    def launch_packed():
         pb.launch(waves=shape.waves, groups=args.groups, chunk=shape.chunk,
                   recompute=shape.recompute, gamma_global=shape.gamma_global,
-                  slots=shape.slots)
+                  slots=shape.slots, resident=False)
+
+    def launch():
+        if resident:
+            pb.launch_resident(groups=args.groups)
+        else:
+            launch_packed()
 
     def one_step():
         arena.copy_(phi0d.expand(args.batch, n, n))
@@ -343,7 +375,9 @@ def main():
     if not args.no_check:
         # correctness spot check of the batch against the plain path (bit-exact)
         spot = order[0]
-        solo = engine.run_single(phi0.copy(), xx, segs[spot], _lib.ORDER_PRE, 0, 3)
+        # (another kernel: the factored one)
+        solo = engine.run_single(phi0.copy(), xx, segs[spot], _lib.ORDER_PRE, 0, 3,
+                                 mode="factored")
         one_step()
         torch.cuda.synchronize()
         assert np.array_equal(arena[spot].cpu().numpy(), solo), "batched != plain result"
@@ -365,6 +399,27 @@ def main():
     kern = [a.elapsed_time(b) for a, b in kern_ms]
     kavg = float(np.mean(kern)) / 1e3
     value = args.batch * args.steps / wall
+    headline_phi = arena.cpu().numpy() if resident and args.compare_packed else None
+    if resident:
+        flagged = int(np.sum(pb.tab.cpu().numpy().view(engine.INST_DTYPE)["reserved"] & 1))
+
+    # the packed streaming kernel over the same batch, for comparison
+    packed = None
+    if resident and args.compare_packed:
+        arena.copy_(phi0d.expand(args.batch, n, n))
+        pb.restart()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        launch_packed()
+        e1.record()
+        torch.cuda.synchronize()
+        pk = e0.elapsed_time(e1) / 1e3
+        packed = {"kernel": kernel_of(pb, packed=True), "kernel_ms": pk * 1e3,
+                  "value": args.batch / pk, "unit": "integrations/s",
+                  "roofline_frac": alg_bytes / pk / PEAK_HBM,
+                  "same_result": bool(np.array_equal(arena.cpu().numpy(), headline_phi))}
+        del headline_phi
 
     # plain path: one integration per call (same draws), host copies included
     lat = []
@@ -394,7 +449,8 @@ def main():
                     rpb.launch()
                 else:
                     rpb.launch(waves=shape.waves, chunk=shape.chunk, recompute=shape.recompute,
-                               gamma_global=shape.gamma_global, slots=shape.slots)
+                               gamma_global=shape.gamma_global, slots=shape.slots,
+                               resident=False)
             go()
             torch.cuda.synchronize()
             t1 = time.perf_counter()
@@ -403,8 +459,7 @@ def main():
             rates[mode] = B / (time.perf_counter() - t1)
             outs[mode] = ar.cpu().numpy()
         routing.append({"batch": B, "unit": "integrations/s",
-                        "auto_kernel": "k_integrate_fz<2, 1>" if rpb.latency_bound() else
-                        "k_integrate_packed<2, 1>",
+                        "auto_kernel": kernel_of(rpb),
                         "auto": rates["auto"], "packed": rates["packed"],
                         "same_result": bool(np.array_equal(outs["auto"], outs["packed"]))})
         del rpb, rjobs, ar
@@ -422,10 +477,27 @@ def main():
     workloads = model_workloads(args, np.random.default_rng(args.seed + 2))
 
     achieved = alg_bytes / kavg
-    kname = ("k_integrate_packed<2, 1>" if shape.slots >= 2 else
-             "k_integrate_nd<2, 1, %d, %d, %d>" % (shape.chunk, shape.recompute,
-                                                  shape.gamma_global))
+    if resident:
+        kname = kernel_of(pb)
+    elif shape.slots >= 2:
+        kname = "k_integrate_packed<2, 1>"
+    else:
+        kname = "k_integrate_nd<2, 1, %d, %d, %d>" % (shape.chunk, shape.recompute,
+                                                     shape.gamma_global)
     traffic = None if args.dense else measured_traffic(n, args.batch, args.seed, kname)
+    if resident:
+        launch_cfg = {"parallelism": "single GPU, one workgroup of %d waves per integration "
+                                     "at a time (persistent), phi held in registers"
+                                     % ((n + 63) // 64),
+                      "instances_per_workgroup": 1,
+                      "integrations_on_exact_division_kernel": flagged}
+    else:
+        launch_cfg = {"parallelism": "single GPU, persistent workgroups, %d integration(s) "
+                                     "each" % shape.slots,
+                      "waves_per_workgroup": shape.waves, "thomas_chunk": shape.chunk,
+                      "thomas_recompute": shape.recompute,
+                      "thomas_gamma_global": shape.gamma_global,
+                      "instances_per_workgroup": shape.slots}
     out = {
         "metric": "2D split_mig integrations/s at pts=100 FP64, batched vs plain; p99 req latency",
         "value": value,
@@ -439,21 +511,16 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded parameter draws; phi from phi_1D_to_2D(phi_1D))",
-        "config": {"workload": "2D split_mig two_pops integrations, pts=%d, batch of %d seeded "
-                               "draws per persistent launch" % (n, args.batch),
-                   "pts": n, "batch": args.batch, "time_steps_per_batch": steps_total,
-                   "parallelism": "single GPU, persistent workgroups, %d integration(s) each"
-                                  % shape.slots,
-                   "waves_per_workgroup": shape.waves, "thomas_chunk": shape.chunk,
-                   "thomas_recompute": shape.recompute,
-                   "thomas_gamma_global": shape.gamma_global,
-                   "instances_per_workgroup": shape.slots,
-                   "row_pitch": engine.pitch_of(arena[0]) or n},
+        "config": dict({"workload": "2D split_mig two_pops integrations, pts=%d, batch of %d "
+                                    "seeded draws per launch" % (n, args.batch),
+                        "pts": n, "batch": args.batch, "time_steps_per_batch": steps_total,
+                        "row_pitch": engine.pitch_of(arena[0]) or n}, **launch_cfg),
         "plain": {"value": plain_rate, "unit": "integrations/s", "calls": len(lat),
                   "time_steps": plain_steps,
                   "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3) if lat else None,
                                  "p99": float(np.percentile(lat, 99) * 1e3) if lat else None}},
         "batched_over_plain": value / plain_rate if plain_rate else None,
+        "packed_kernel": packed,
         "routing": routing,
         "serving": serving,
         "model_workloads": workloads,
