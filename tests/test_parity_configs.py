@@ -189,6 +189,64 @@ def test_dfe_two_epoch_sel_at_config_grids(dadi_amd, oracle):
         _check_spectrum(oracle, cache.spectra[ii], phis, [n], pts_l, False, "gamma %g" % g)
 
 
+def test_dfe_extrapolation_failures_match_oracle(dadi_amd, oracle, caplog):
+    """The fail_mag safeguard (dadi/Numerics.py:455-468) fires in config 4a for
+    strongly selected gammas: their high-frequency entries are rounding noise
+    of the sampling sums (1e-42 .. 1e-61, alternating in sign across grids)
+    that the quadratic extrapolation magnifies by more than 10 decades.  The
+    oracle (the reference's algorithm on the reference C's integrations)
+    fails on the same gammas, so the warning belongs to the algorithm, not to
+    this build; which noise entries fail depends on the summation order.  For
+    four strongly selected gammas and a mild one: the product warns
+    exactly when entries fail, and the failing entries and the extrapolated
+    spectrum equal the oracle's with its exactly rounded sampler; the
+    reference's summation order (numpy's BLAS dot) is only reported, as its
+    noise entries depend on the host's BLAS."""
+    import logging
+    from dadi_amd.DFE import DemogSelModels
+    pts_l, n = (200, 220, 240), 50
+    nu, T = 2.0, 0.1
+    gammas = -np.logspace(np.log10(2000), np.log10(1e-4), 500)
+    ex = dadi_amd.Numerics.make_extrap_func(DemogSelModels.two_epoch_sel)
+    fails = {}
+    for idx in (33, 56, 71, 94, 480):
+        g = float(gammas[idx])
+        res = ex((nu, T, g), (n,), pts_l, no_extrap=True)
+        xs = [r.extrap_x for r in res]
+        caplog.clear()
+        with caplog.at_level(logging.WARNING, logger="Numerics"):
+            got = dadi_amd.Numerics.extrapolate(res, xs)
+        warned = any("Extrapolation may have failed" in m for m in caplog.messages)
+        raw = dadi_amd.Numerics.extrapolate(res, xs, fail_mag=np.inf)
+        # entries 1 .. n-1: the two corners are masked in every Spectrum
+        inner = slice(1, n)
+        got_failed = 1 + np.flatnonzero(np.asarray(got)[inner] != np.asarray(raw)[inner])
+        assert warned == (len(got_failed) > 0), (g, warned, got_failed)
+        oxs = [oracle.default_grid(p)[1] for p in pts_l]
+        phis = []
+        for pts in pts_l:
+            xx = oracle.default_grid(pts)
+            phis.append(oracle.integrate_const(oracle.phi_1D(xx, gamma=g), xx, T, [nu], [[]],
+                                               [g], [0.5]))
+        exact = [oracle.from_phi_exact(f, (n,), oracle.default_grid(p))
+                 for f, p in zip(phis, pts_l)]
+        want = oracle.extrapolate(exact, oxs)
+        want_failed = 1 + np.flatnonzero(
+            (want != oracle.extrapolate(exact, oxs, fail_mag=np.inf))[inner])
+        ref = [oracle.from_phi(f, (n,), oracle.default_grid(p)) for f, p in zip(phis, pts_l)]
+        rref = oracle.extrapolate(ref, oxs)
+        ref_failed = 1 + np.flatnonzero(
+            (rref != oracle.extrapolate(ref, oxs, fail_mag=np.inf))[inner])
+        print("  gamma %.6g: failing entries %s (oracle exactly rounded %s, reference order %s)"
+              % (g, got_failed.tolist(), want_failed.tolist(), ref_failed.tolist()), flush=True)
+        assert np.array_equal(got_failed, want_failed), (g, got_failed, want_failed)
+        np.testing.assert_allclose(np.asarray(got)[inner], want[inner], rtol=1e-12, atol=0)
+        fails[idx] = len(got_failed)
+    # with exactly rounded sums gamma index 33 (-658) extrapolates within fail_mag, though
+    # in the reference's summation order its last entry fails (measured here)
+    assert all(fails[i] > 0 for i in (56, 71, 94)) and fails[480] == 0, fails
+
+
 # ---------------------------------------------------------------------------
 # configuration 4b: 2D DFE cache over gamma pairs at pts (60, 70, 80)
 # ---------------------------------------------------------------------------
