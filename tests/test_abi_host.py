@@ -8,6 +8,7 @@ import ctypes
 import os
 import re
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -181,3 +182,23 @@ def test_one_pop_x_argument_errors():
         dadi_amd.Integration.one_pop_X(phi, xx, 0.1, initial_t=0.2)
     out = dadi_amd.Integration.one_pop_X(phi, xx, 0.1, frozen=True)
     assert np.array_equal(out, phi) and out is not phi
+
+
+def test_import_raises_hw_queues_before_gpu_use():
+    """Importing the package raises GPU_MAX_HW_QUEUES (4 by default) to 16 so
+    that the serving loop's per-class streams get hardware queues of their
+    own; a larger value is kept, DADI_AMD_HW_QUEUES=0 leaves it alone and
+    other targets are capped at 32."""
+    code = "import os, dadi_amd; print(os.environ.get('GPU_MAX_HW_QUEUES'))"
+    cases = [({"GPU_MAX_HW_QUEUES": "4"}, "16"), ({}, "16"),
+             ({"GPU_MAX_HW_QUEUES": "24"}, "24"),
+             ({"GPU_MAX_HW_QUEUES": "4", "DADI_AMD_HW_QUEUES": "0"}, "4"),
+             ({"DADI_AMD_HW_QUEUES": "64"}, "32")]
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    for extra, want in cases:
+        env = {k: v for k, v in os.environ.items()
+               if k not in ("GPU_MAX_HW_QUEUES", "DADI_AMD_HW_QUEUES")}
+        env.update(extra)
+        out = subprocess.run([sys.executable, "-c", code], cwd=root, env=env,
+                             capture_output=True, text=True, check=True).stdout.split()
+        assert out[-1] == want, (extra, out)
