@@ -360,7 +360,7 @@ def main():
 
     def launch():
         if resident:
-            pb.launch_resident(groups=args.groups)
+            pb.launch_resident(groups=args.groups, latency=False)
         else:
             launch_packed()
 

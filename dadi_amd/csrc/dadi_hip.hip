@@ -1603,9 +1603,9 @@ constexpr int rr_vrows() { return NP <= 88 ? NP : 88; }
 // This lane's line: rows 0..NP-1, row j in v[j] (j < PV) or a[j - PV].
 // Indices are compile-time constants after unrolling, so get/set fold to a
 // register access (or two accumulation-register moves).
-template <int NP>
+template <int NP, int PVR = rr_vrows<NP>()>
 struct RrLine {
-    static constexpr int PV = rr_vrows<NP>();
+    static constexpr int PV = PVR;
     double v[PV];
     RrAcc a[NP > PV ? NP - PV : 1];
     __device__ __forceinline__ double get(int j) const
@@ -2191,6 +2191,355 @@ k_integrate_rr(dh_instance *__restrict__ tab, int ninst, int *__restrict__ count
             }
         }
         if (FAST && (failed || __syncthreads_or(rr_wave_out(okm, L, n)))) {
+            // phi and the progress are as taken up: the exact kernel redoes them
+            if (tid == 0) ip->reserved |= 1;
+            continue;
+        }
+        if (L < n) {
+#pragma unroll
+            for (int i = 0; i < NP; ++i)
+                if (i < n) phi[i * ld + L] = P.get(i);
+        }
+        if (tid == 0) {
+            ip->cur_seg = seg;
+            ip->cur_step = step;
+            ip->status = seg >= nseg ? 1 : 0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Kernel 1d: latency form of the register-resident 2D kernel.
+// phi is held in registers as in k_integrate_rr, but within a segment every
+// step solves the same tridiagonal systems, so at the segment's start each
+// lane forms its line's Thomas factorisation once -- a_j, bet_j,
+// y_j = rr_recip(bet_j) and gamma_{j+1}, with the operations k_integrate_rr
+// uses for them at every step -- and keeps it in a global workspace laid out
+// [axis][row][line] (consecutive lanes, consecutive addresses).  A step then
+// runs only the right-hand-side recurrences: r_j, u_j = (r_j - a_j u_{j-1}) /
+// bet_j (the shared-reciprocal quotient, as k_integrate_rr) forward and
+// u_j -= gamma_{j+1} u_{j+1} back, about a quarter of k_integrate_rr's
+// instructions per row, at the price of reading the records (40 bytes per
+// point and sweep) from the L2.  The transpose between sweeps goes through
+// one n x (n + 1) LDS tile: all lanes write, then all lanes read.  One
+// workgroup per integration, one wave per SIMD: for single integrations and
+// batches too small to fill the GPU.  The same exactness checks as
+// k_integrate_rr; an instance that leaves their range is handed to the
+// exactly dividing k_integrate_rr kernel.
+// ---------------------------------------------------------------------------
+constexpr int RF_PF = 8;       // rows of records in flight in the forward pass
+constexpr int RF_PB = 8;       // rows of gamma in flight in the back substitution
+
+// Rows of the line in vector registers (the rest in accumulation registers):
+// the record prefetch needs about 60 vector registers.
+template <int NP>
+constexpr int rf_vrows() { return NP <= 72 ? NP : 72; }
+
+template <int NP>
+constexpr size_t rf_lds_doubles()
+{
+    return (size_t)NP * (NP + 1) > 12 * (size_t)NP ? (size_t)NP * (NP + 1) : 12 * (size_t)NP;
+}
+
+// Workspace doubles of one workgroup: per axis and row, (a_j, bet_j) and
+// (y_j, gamma_{j+1}) for every line.
+template <int NP>
+constexpr long long rf_slice_doubles() { return 2LL * 2 * NP * NP * 2; }
+
+// The factorisation of line `line` for one axis, stored (if `store`) as the
+// records of rows 0..n-1 at AB[j * NP + line] / YG[j * NP + line]; returns
+// whether every bet_j of rows
+// 1..n-2 and every gamma_{j+1} formed by rr_quot is in the range where the
+// shared-reciprocal quotients give the bits of division (rr_coeffs_in_range).
+// Row 0 keeps bet_0 and gamma_1 (its quotients are IEEE divisions), the last
+// row a and bet.
+template <int NP, int ORDER>
+__device__ __forceinline__ bool rf_factor(double2 *__restrict__ AB, double2 *__restrict__ YG,
+                                          int n, int line, bool store, const double *T,
+                                          double c, double m, double inv_dt, bool bcf_on,
+                                          double bcf, bool bcl_on, double bcl)
+{
+    double pa, pc;
+    const double bet0 = rr_row0<ORDER>(T[0], T[1], T[2], T[3], T[4], c, m, inv_dt, bcf_on, bcf,
+                                       pa, pc);
+    double g = (-T[1] * pc) / bet0;
+    if (store) {
+        AB[line] = make_double2(0., bet0);
+        YG[line] = make_double2(0., g);
+    }
+    bool in = true;
+    double ca, cb, cc;
+    if (1 < n - 1)
+        rr_coef_mid<ORDER>(T[RR_TS], T[RR_TS + 1], T[RR_TS + 2], T[RR_TS + 3], T[RR_TS + 4], c,
+                           m, inv_dt, pa, pc, ca, cb, cc);
+    else
+        rr_coef_last<ORDER>(T[RR_TS + 1], inv_dt, bcl_on, bcl, pa, pc, ca, cb);
+#pragma unroll 1
+    for (int j = 1; j < n - 1; ++j) {
+        const double bet = cb - ca * g;
+        const double y = rr_recip(bet);
+        in = in & rr_in(bet, RR_DEN_LO, RR_DEN_HI);
+        double gn;
+        if (j < rr_fast_g_rows<NP>()) {
+            gn = rr_quot(cc, bet, y);
+            in = in & (rr_in(gn, RR_QUO_LO, RR_QUO_HI) |
+                       (__builtin_bit_cast(unsigned long long, cc) == 0));
+        } else {
+            gn = cc / bet;
+        }
+        if (store) {
+            AB[(long long)j * NP + line] = make_double2(ca, bet);
+            YG[(long long)j * NP + line] = make_double2(y, gn);
+        }
+        g = gn;
+        const double *R = T + (j + 1) * RR_TS;
+        if (j + 1 < n - 1)
+            rr_coef_mid<ORDER>(R[0], R[1], R[2], R[3], R[4], c, m, inv_dt, pa, pc, ca, cb, cc);
+        else
+            rr_coef_last<ORDER>(R[1], inv_dt, bcl_on, bcl, pa, pc, ca, cb);
+    }
+    if (store) AB[(long long)(n - 1) * NP + line] = make_double2(ca, cb - ca * g);
+    return in;
+}
+
+// One implicit half-step of this lane's line from its factorisation (the
+// records of line L): the operations of rr_sweep's rows (row 0 and the last
+// row by IEEE division, rows 1..n-2 by the shared reciprocal y_j), records
+// read RF_PF rows ahead.
+template <int NP, int ORDER>
+__device__ __forceinline__ void rf_sweep(RrLine<NP, rf_vrows<NP>()> &P,
+                                         const double2 *__restrict__ AB,
+                                         const double2 *__restrict__ YG, int n, int L, double dt,
+                                         double inv_dt, double y_dt, int inj_row, double inj,
+                                         unsigned long long &okm)
+{
+    constexpr int SURE = NP - RR_SPAN;   // rows 0..SURE exist; rows < SURE are not the last
+    constexpr size_t ROW = (size_t)NP * sizeof(double2);   // bytes per row of records
+    if (inj_row == 0) P.set(0, P.get(0) + inj);
+    if (inj_row == 1) P.set(1, P.get(1) + inj);
+    // uniform row bases plus this lane's 32-bit byte offset; the offset is
+    // passed through rr_after before each load, which ties the load to the
+    // recurrence (it is not hoisted) at no instruction cost
+    const char *ab0 = reinterpret_cast<const char *>(AB);
+    const char *yg0 = reinterpret_cast<const char *>(YG);
+    int voff = L * (int)sizeof(double2);
+    double2 fab[RF_PF];
+    double fy[RF_PF];
+#pragma unroll
+    for (int q = 0; q < RF_PF; ++q) {
+        fab[q] = *reinterpret_cast<const double2 *>(ab0 + q * ROW + (unsigned)voff);
+        fy[q] = *reinterpret_cast<const double *>(yg0 + q * ROW + (unsigned)voff);
+    }
+    double u;
+    {
+        const double p0 = P.get(0);
+        const double r = (ORDER == DH_ORDER_PRE) ? inv_dt * p0 : p0 / dt;
+        u = r / fab[0].y;
+        P.set(0, u);
+    }
+#pragma unroll
+    for (int j = 1; j < NP; ++j) {
+        const double2 ab = fab[j % RF_PF];
+        const double y = fy[j % RF_PF];
+        if (j - 1 + RF_PF < NP) {
+            // row j - 1's slot takes row j - 1 + RF_PF, issued once u_{j-1} is known
+            voff = rr_after(voff, u);
+            const size_t row = (size_t)(j - 1 + RF_PF) * ROW;
+            fab[(j - 1) % RF_PF] = *reinterpret_cast<const double2 *>(ab0 + row + (unsigned)voff);
+            fy[(j - 1) % RF_PF] = *reinterpret_cast<const double *>(yg0 + row + (unsigned)voff);
+        }
+        if (j < SURE || j < n) {
+            const double pj = P.get(j);
+            if (j < SURE || j < n - 1) {
+                double r;
+                if constexpr (ORDER == DH_ORDER_PRE) {
+                    r = inv_dt * pj;
+                } else {
+                    r = rr_quot(pj, dt, y_dt);
+                    rr_keep_exact(okm, r, pj);
+                }
+                const double x = r - ab.x * u;
+                u = rr_quot(x, ab.y, y);
+                rr_keep_exact(okm, u, x);
+            } else {
+                const double r = (ORDER == DH_ORDER_PRE) ? inv_dt * pj : pj / dt;
+                u = (r - ab.x * u) / ab.y;
+            }
+            P.set(j, u);
+        }
+    }
+    // back substitution: gamma_{j+1} is the second half of row j's (y, gamma)
+    // record
+    const char *g0 = yg0 + sizeof(double);
+    double gb[RF_PB];
+#pragma unroll
+    for (int q = 0; q < RF_PB; ++q) {
+        const int j = NP - 2 - q;
+        gb[q] = j >= 0 ? *reinterpret_cast<const double *>(g0 + (size_t)j * ROW + (unsigned)voff)
+                       : 0.;
+    }
+#pragma unroll
+    for (int j = NP - 2; j >= 0; --j) {
+        const int s = (NP - 2 - j) % RF_PB;
+        const double g = gb[s];
+        if (j - RF_PB >= 0) {
+            voff = rr_after(voff, u);
+            gb[s] = *reinterpret_cast<const double *>(g0 + (size_t)(j - RF_PB) * ROW +
+                                                      (unsigned)voff);
+        }
+        if (j < SURE || j <= n - 2) {
+            const double v = P.get(j) - g * u;
+            P.set(j, v);
+            u = v;
+        }
+    }
+}
+
+// Transposes the n x n block whose line L lane L holds (row k of the line =
+// M[k][L] -> M[L][k]) through the LDS tile of pitch NP + 1: every lane
+// writes its line, then every lane reads its new one.
+template <int NP>
+__device__ __forceinline__ void rf_transpose(RrLine<NP, rf_vrows<NP>()> &P, int n, int L,
+                                             double *__restrict__ tile)
+{
+    constexpr int LDW = NP + 1;
+    if (L < n) {
+#pragma unroll
+        for (int k = 0; k < NP; ++k)
+            if (k < n) tile[k * LDW + L] = P.get(k);
+    }
+    __syncthreads();
+    if (L < n) {
+#pragma unroll
+        for (int k = 0; k < NP; ++k)
+            if (k < n) P.set(k, tile[L * LDW + k]);
+    }
+    __syncthreads();
+}
+
+template <int NP, int ORDER>
+__global__ void __launch_bounds__(64 * rr_waves<NP>()) __attribute__((amdgpu_waves_per_eu(1, 1)))
+k_integrate_rf(dh_instance *__restrict__ tab, int ninst, int *__restrict__ counter,
+               long long max_steps, long long max_ticks, double *__restrict__ ws_all,
+               long long slice)
+{
+    extern __shared__ double lds[];
+    __shared__ int s_next, s_stop;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int L = tid;
+    // the tile; at a segment's start the coefficient tables (the same
+    // interleaved layout as k_integrate_rr) in its place
+    double *tile = lds;
+    double *sx = lds, *sdx = sx + NP;
+    double *blk0 = sdx + NP, *blk1 = blk0 + 5 * NP;
+    double2 *ws = reinterpret_cast<double2 *>(ws_all + (long long)blockIdx.x * slice);
+    for (;;) {
+        __syncthreads();
+        if (tid == 0) s_next = atomicAdd(counter, 1);
+        __syncthreads();
+        const int id = __builtin_amdgcn_readfirstlane(s_next);
+        if (id >= ninst) break;
+        dh_instance *ip = tab + id;
+        const int n = __builtin_amdgcn_readfirstlane(ip->n);
+        if (ip->D != 2 || n <= NP - RR_SPAN || n > NP) {
+            if (tid == 0) ip->status = -1;
+            continue;
+        }
+        __builtin_assume(n > NP - RR_SPAN && n <= NP);
+        // instances marked for exact division are left to k_integrate_rr
+        if ((__builtin_amdgcn_readfirstlane(ip->reserved) & 1) != 0) continue;
+        long long seg = ip->cur_seg, step = ip->cur_step;
+        const long long nseg = ip->nseg;
+        long long budget = max_steps > 0 ? max_steps : LLONG_MAX;
+        const int frozen = ip->frozen, injm = ip->inject;
+        int first_axis = 0;
+        while (first_axis < 2 && ((frozen >> first_axis) & 1)) ++first_axis;
+        double *phi = gbl(ip->phi_dev);
+        const long long ld = ip->ld > 0 ? ip->ld : n;
+        RrLine<NP, rf_vrows<NP>()> P;
+        if (L < n) {
+#pragma unroll
+            for (int i = 0; i < NP; ++i)
+                if (i < n) P.set(i, phi[i * ld + L]);
+        }
+        asm volatile("" ::: "memory");
+        const long long t0 = max_ticks > 0 ? (long long)wall_clock64() : 0;
+        bool out_of_time = false, failed = false;
+        unsigned long long okm = __builtin_amdgcn_ballot_w64(true);
+        if (first_axis == 2) seg = nseg;
+        while (seg < nseg && budget > 0 && !out_of_time && !failed) {
+            const dh_segment *sp = gbl(ip->seg_dev) + seg;
+            // the tile is free between steps: the tables go there
+            __syncthreads();
+            rr_grid_tables(gbl(ip->xx_dev), n, sx, sdx, blk0, tid, nthr);
+            __syncthreads();
+            for (int i = tid; i < n; i += nthr) {
+                blk1[i * RR_TS] = blk0[i * RR_TS];
+                blk1[i * RR_TS + 1] = blk0[i * RR_TS + 1];
+            }
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                if ((frozen >> a) & 1) continue;
+                rr_axis_tables(sp, a, n, sx, sdx, a ? blk1 : blk0, tid, nthr);
+            }
+            __syncthreads();
+            const double dt = sp->dt, inv_dt = 1. / dt, inj = sp->inject;
+            const double y_dt = rr_recip(dt);
+            const int line = L < n ? L : 0;
+            bool in = ORDER == DH_ORDER_PRE || rr_in(dt, RR_DEN_LO, RR_DEN_HI);
+            int inj_row[2] = {-1, -1};
+#pragma unroll 1
+            for (int a = 0; a < 2; ++a) {
+                if ((frozen >> a) & 1) continue;
+                LineSetup<2, ORDER> Ls;
+                const double mg[1] = {sp->mig[a][0]};
+                line_setup<2, ORDER>(Ls, line, n, a, sx, mg, sp->nu[a], sp->gamma[a], sp->h[a],
+                                     sdx[0], sdx[n - 2], first_axis == a, injm, sp->chrom);
+                inj_row[a] = Ls.inj_row;
+                double2 *AB = ws + (long long)a * 2 * NP * NP, *YG = AB + (long long)NP * NP;
+                // lanes past the last line form line 0's values and store nothing
+                in = in & rf_factor<NP, ORDER>(AB, YG, n, line, L < n, a ? blk1 : blk0,
+                                              Ls.c[0], mg[0], inv_dt, Ls.bcf_on, Ls.bcf,
+                                              Ls.bcl_on, Ls.bcl);
+            }
+            okm &= __builtin_amdgcn_ballot_w64(in || L >= n);
+            // the tables are read: the transposes may use the tile again
+            __syncthreads();
+            long long nst = sp->nsteps - step;
+            if (nst > budget) nst = budget;
+            for (long long k = 0; k < nst; ++k) {
+#pragma unroll 1
+                for (int a = 0; a < 2; ++a) {
+                    if (!((frozen >> a) & 1)) {
+                        const double2 *AB = ws + (long long)a * 2 * NP * NP;
+                        rf_sweep<NP, ORDER>(P, AB, AB + (long long)NP * NP, n, line, dt, inv_dt,
+                                            y_dt, inj_row[a], inj, okm);
+                    }
+                    rf_transpose<NP>(P, n, L, tile);
+                }
+                if ((k & 31) == 31 && __syncthreads_or(rr_wave_out(okm, L, n))) {
+                    nst = k + 1;
+                    failed = true;
+                    break;
+                }
+                if (max_ticks > 0) {
+                    if (tid == 0) s_stop = (long long)wall_clock64() - t0 >= max_ticks;
+                    __syncthreads();
+                    if (s_stop) {
+                        nst = k + 1;
+                        out_of_time = true;
+                        break;
+                    }
+                }
+            }
+            budget -= nst;
+            step += nst;
+            if (step >= sp->nsteps) {
+                ++seg;
+                step = 0;
+            }
+        }
+        if (failed || __syncthreads_or(rr_wave_out(okm, L, n))) {
             // phi and the progress are as taken up: the exact kernel redoes them
             if (tid == 0) ip->reserved |= 1;
             continue;
@@ -3266,6 +3615,34 @@ const void *pick_rr(int n, int order, bool fast, size_t *lds, int *waves)
     return nullptr;
 }
 
+// latency form of the register-resident kernel (k_integrate_rf): kernel, LDS
+// bytes, waves and workspace doubles per workgroup for n rows (null if none)
+template <int NP>
+const void *rf_kernel(int order)
+{
+    switch (order) {
+    case DH_ORDER_TV:
+        return reinterpret_cast<const void *>(&k_integrate_rf<NP, DH_ORDER_TV>);
+    case DH_ORDER_PRE:
+        return reinterpret_cast<const void *>(&k_integrate_rf<NP, DH_ORDER_PRE>);
+    }
+    return nullptr;
+}
+
+const void *pick_rf(int n, int order, size_t *lds, int *waves, long long *slice)
+{
+#define DH_RF(np)                                                                   \
+    if (n > (np) - RR_SPAN && n <= (np)) {                                          \
+        *lds = sizeof(double) * rf_lds_doubles<np>();                               \
+        *waves = rr_waves<np>();                                                    \
+        *slice = rf_slice_doubles<np>();                                            \
+        return rf_kernel<np>(order);                                                \
+    }
+    DH_RF(40) DH_RF(48) DH_RF(56) DH_RF(64) DH_RF(72) DH_RF(80) DH_RF(88) DH_RF(96) DH_RF(104)
+#undef DH_RF
+    return nullptr;
+}
+
 const void *pick_fz(int D, int order)
 {
     switch (D * 4 + order) {
@@ -3771,6 +4148,58 @@ int dh_integrate_resident(dh_instance *table_dev, int ninst, int order, int n,
         hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, counter_dev, 0);
         DH_HIP(hipLaunchKernel(fn, dim3((unsigned)groups), dim3(64 * waves), args, lds, st));
     }
+    return DH_OK;
+}
+
+long long dh_resident_latency_slice_doubles(int n)
+{
+    size_t lds = 0;
+    int waves = 0;
+    long long slice = 0;
+    return pick_rf(n, DH_ORDER_PRE, &lds, &waves, &slice) ? slice : 0;
+}
+
+int dh_integrate_resident_latency(dh_instance *table_dev, int ninst, int order, int n,
+                                  int *counter_dev, const dh_launch *cfg, void *stream)
+{
+    if (int e = check_init()) return e;
+    if (order != DH_ORDER_TV && order != DH_ORDER_PRE)
+        return fail(DH_EINVAL, "order must be TV or PRE (2 populations)");
+    if (ninst <= 0) return DH_OK;
+    if (!table_dev || !counter_dev || !cfg) return fail(DH_EINVAL, "null pointer");
+    size_t lds = 0, rr_lds = 0;
+    int waves = 0, rr_w = 0;
+    long long slice = 0;
+    const void *fast = pick_rf(n, order, &lds, &waves, &slice);
+    const void *exact = pick_rr(n, order, false, &rr_lds, &rr_w);
+    if (!fast || !exact)
+        return fail(DH_ERANGE, "no register-resident kernel for n = " + std::to_string(n));
+    if (!cfg->gamma_dev)
+        return fail(DH_EINVAL, "gamma_dev must name the factorisation workspace "
+                               "(dh_resident_latency_slice_doubles(n) doubles per workgroup)");
+    if (int e = set_lds(fast, lds)) return e;
+    if (int e = set_lds(exact, rr_lds)) return e;
+    // one wave per SIMD; the LDS tile may allow fewer workgroups per compute unit
+    const int per_cu = std::max(1, std::min(4 / waves, (int)(g_max_lds / lds)));
+    long long groups = cfg->groups > 0 ? cfg->groups
+                                       : std::min<long long>(ninst, (long long)per_cu * g_num_cu);
+    if (cfg->gamma_doubles < groups * slice)
+        return fail(DH_ERANGE, "factorisation workspace too small: need " +
+                                   std::to_string(groups * slice) + " doubles");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    long long ms = cfg->max_steps, ticks = budget_ticks(cfg);
+    double *ws = cfg->gamma_dev;
+    void *args[] = {&table_dev, &ninst, &counter_dev, &ms, &ticks, &ws, &slice};
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, counter_dev, 0);
+    DH_HIP(hipLaunchKernel(fast, dim3((unsigned)groups), dim3(64 * waves), args, lds, st));
+    // the instances the fast kernel handed over (or found marked) run on the
+    // exactly dividing register-resident kernel
+    const long long rr_groups = cfg->groups > 0 ? cfg->groups
+                                                : std::min<long long>(ninst,
+                                                                      (long long)(4 / rr_w) * g_num_cu);
+    void *rargs[] = {&table_dev, &ninst, &counter_dev, &ms, &ticks};
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, counter_dev, 0);
+    DH_HIP(hipLaunchKernel(exact, dim3((unsigned)rr_groups), dim3(64 * rr_w), rargs, rr_lds, st));
     return DH_OK;
 }
 

@@ -64,6 +64,11 @@ class Context:
     #: 2D batches whose grid size has a register-resident kernel use it when
     #: no other launch shape is asked for; False keeps them on the others
     resident_auto = True
+    #: register-resident batches of at most this many integrations take the
+    #: kernel's latency form (dh_integrate_resident_latency: the Thomas
+    #: factorisation kept per segment, a step runs only the right-hand-side
+    #: recurrences), larger ones its throughput form
+    resident_latency_max_inst = 64
 
     def one_d_wave(self, ninst, nmax):
         """Whether a 1D batch runs one wave per integration (LDS-resident,
@@ -95,6 +100,37 @@ class Context:
         cfg.max_us = int(max_us)
         check(self.lib.dh_integrate_factored(tab_ptr, ninst, D, order, nmax, counter_ptr, cfg,
                                              stream), "dh_integrate_factored")
+
+    def resident_launch(self, tab_ptr, ninst, order, nmax, counter_ptr, stream, workspace,
+                        groups=0, max_steps=0, max_us=0, latency=None):
+        """One register-resident 2D launch: the latency form
+        (dh_integrate_resident_latency, one workgroup per compute unit, its
+        factorisation workspace kept in ``workspace``, a dict owned by the
+        caller, one per stream) when ``latency`` is True or, for None, the
+        batch has at most resident_latency_max_inst integrations; else
+        dh_integrate_resident."""
+        if latency is None:
+            latency = ninst <= self.resident_latency_max_inst
+        if not latency:
+            cfg = DhLaunch(1, groups, max_steps, 0, 0)
+            cfg.max_us = int(max_us)
+            check(self.lib.dh_integrate_resident(tab_ptr, ninst, order, nmax, counter_ptr, cfg,
+                                                 stream), "dh_integrate_resident")
+            return
+        slice_d = int(self.lib.dh_resident_latency_slice_doubles(int(nmax)))
+        g = groups if groups > 0 else min(ninst, self.num_cu)
+        need = g * slice_d
+        ws = workspace.get("resident")
+        if ws is None or ws.numel() < need:
+            ws = torch().empty(need, dtype=torch().float64, device=self.device)
+            workspace["resident"] = ws
+        cfg = DhLaunch(1, g, max_steps, 0, 0)
+        cfg.gamma_dev = ws.data_ptr()
+        cfg.gamma_doubles = ws.numel()
+        cfg.max_us = int(max_us)
+        check(self.lib.dh_integrate_resident_latency(tab_ptr, ninst, order, nmax, counter_ptr,
+                                                     cfg, stream),
+              "dh_integrate_resident_latency")
 
     def factored_groups(self, ninst, slice_doubles):
         """Workgroups of a factored launch: one per compute unit, two for
@@ -460,15 +496,16 @@ def run_single(phi, xx, seg, order, frozen, inject, ctx=None, mode="auto", waves
         check(ctx.lib.dh_integrate_nd(tab.data_ptr(), 1, D, order, n, counter.data_ptr(),
                                       cfg, stream), "dh_integrate_nd")
         keep = (tab, counter, segd)
-    elif mode == "resident":
+    elif mode in ("resident", "resident_throughput"):
+        # "resident": the latency form (per-segment factorisation)
         rec = _inst_record(phid.data_ptr(), xxd.data_ptr(), segd.data_ptr(), len(seg), D, n,
                            order, frozen, inject, ld)
         tab = ctx.to_dev(rec.view(np.uint8))
         counter = t.zeros(1, dtype=t.int32, device=ctx.device)
-        cfg = DhLaunch(1, 1, 0, 0, 0)
-        check(ctx.lib.dh_integrate_resident(tab.data_ptr(), 1, order, n, counter.data_ptr(),
-                                            cfg, stream), "dh_integrate_resident")
-        keep = (tab, counter, segd)
+        ws = {}
+        ctx.resident_launch(tab.data_ptr(), 1, order, n, counter.data_ptr(), stream, ws,
+                            groups=1, latency=(mode == "resident"))
+        keep = (tab, counter, segd, ws)
     elif mode == "factored":
         rec = _inst_record(phid.data_ptr(), xxd.data_ptr(), segd.data_ptr(), len(seg), D, n,
                            order, frozen, inject, ld)
@@ -585,30 +622,30 @@ class PackedBatch:
         rows = self.ctx.lib.dh_resident_rows(self.nmax)
         return rows > 0 and all(self.ctx.lib.dh_resident_rows(j.n) == rows for j in self.jobs)
 
-    def launch_resident(self, groups=0, max_steps=0, max_us=0, stream=None):
-        """One dh_integrate_resident launch over the table."""
+    def launch_resident(self, groups=0, max_steps=0, max_us=0, stream=None, latency=None):
+        """One register-resident launch over the table (Context.resident_launch:
+        latency True / False picks the form, None by batch size)."""
         ctx = self.ctx
         stream = ctx.stream() if stream is None else stream
-        cfg = DhLaunch(1, groups, max_steps, 0, 0)
-        cfg.max_us = int(max_us)
-        check(ctx.lib.dh_integrate_resident(self.tab.data_ptr(), len(self.jobs), self.order,
-                                            self.nmax, self.counter.data_ptr(), cfg, stream),
-              "dh_integrate_resident")
+        ctx.resident_launch(self.tab.data_ptr(), len(self.jobs), self.order, self.nmax,
+                            self.counter.data_ptr(), stream, self.workspace, groups, max_steps,
+                            max_us, latency)
 
     def launch(self, waves=None, groups=0, max_steps=0, stream=None, chunk=None,
-               recompute=None, gamma_global=None, slots=None, resident=None):
+               recompute=None, gamma_global=None, slots=None, resident=None, latency=None):
         """One persistent launch over the table (max_steps > 0: at most that
         many steps per instance).  Shape arguments left None are autotuned /
         chosen by rule of thumb; slots=1 forbids packing.  resident: True runs
         the register-resident 2D kernel, False never does; None uses it when
-        it can run the batch and no other launch shape is asked for."""
+        it can run the batch and no other launch shape is asked for; latency
+        picks its form (launch_resident)."""
         ctx = self.ctx
         stream = ctx.stream() if stream is None else stream
         if resident or (resident is None and ctx.resident_auto and self.resident_ok() and
                         all(v is None for v in (waves, chunk, recompute, gamma_global, slots))):
             if not self.resident_ok():
                 raise ValueError("the register-resident kernel cannot run this batch")
-            return self.launch_resident(groups, max_steps, 0, stream)
+            return self.launch_resident(groups, max_steps, 0, stream, latency)
         if self.D == 1:
             if ctx.one_d_wave(len(self.jobs), self.nmax):
                 cfg = DhLaunch(1, groups, max_steps, 0, 0)

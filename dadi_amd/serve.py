@@ -421,11 +421,11 @@ class Server:
                                                    counter.data_ptr(), scr.data_ptr(), scr.numel(),
                                                    cfg, h), "dh_integrate_1d")
             elif len(key) == 3:
-                cfg = _lib.DhLaunch(1, 0, 0, 0, 0)   # a time budget, not a step count
-                cfg.max_us = budget_us
-                _lib.check(ctx.lib.dh_integrate_resident(tabd.data_ptr(), len(group), int(order),
-                                                         nmax, counter.data_ptr(), cfg, h),
-                           "dh_integrate_resident")
+                # a time budget, not a step count; the kernel's latency form
+                # while the class holds few integrations
+                ws = self.scratch.setdefault(key, {})
+                ctx.resident_launch(tabd.data_ptr(), len(group), int(order), nmax,
+                                    counter.data_ptr(), h, ws, max_us=budget_us)
             elif self.factored and ctx.factorable([r.seg for r in group]):
                 ws = self.scratch.setdefault(key, {})
                 ctx.factored_launch(tabd.data_ptr(), len(group), int(D), int(order), nmax,

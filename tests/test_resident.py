@@ -1,7 +1,8 @@
 """
-The register-resident 2D kernel (dh_integrate_resident): phi held in
-registers for a whole integration, lines transposed through LDS between the
-sweeps.  Its results must be the bits of the other kernels and of the oracle:
+The register-resident 2D kernel (dh_integrate_resident) and its latency form
+(dh_integrate_resident_latency, the Thomas factorisation kept per segment):
+phi held in registers for a whole integration, lines transposed through LDS
+between the sweeps.  Every test runs both forms.  Its results must be the bits of the other kernels and of the oracle:
 split_mig draws on the grid sizes it is built for, both coefficient orders
 (constant parameters and time-varying ones), frozen populations, integrations
 run in quanta of a few steps, and an instance of another shape left alone.
@@ -63,15 +64,16 @@ def _jobs(eng, oracle, rng, n, k, tv=False, frozen_every=0, tmax=0.02):
     return jobs, refs, phi0
 
 
+@pytest.mark.parametrize("latency", [False, True])
 @pytest.mark.parametrize("n", [33, 40, 57, 64, 65, 80, 90, 97, 100, 104])
-def test_resident_equals_packed_and_oracle(eng, oracle, n):
+def test_resident_equals_packed_and_oracle(eng, oracle, n, latency):
     ctx = eng.context()
     assert ctx.lib.dh_resident_rows(n) == (n + 7) // 8 * 8
     rng = np.random.default_rng(n)
     jobs, refs, phi0 = _jobs(eng, oracle, rng, n, 24, frozen_every=5)
     pb = eng.PackedBatch(ctx, jobs)
     assert pb.resident_ok()
-    pb.launch(resident=True)
+    pb.launch(resident=True, latency=latency)
     got = [j.phi.cpu().numpy() for j in jobs]
     for g, r in zip(got, refs):
         assert np.array_equal(g, r)
@@ -84,14 +86,15 @@ def test_resident_equals_packed_and_oracle(eng, oracle, n):
         assert np.array_equal(j.phi.cpu().numpy(), g)
 
 
-def test_resident_time_varying_and_quanta(eng, oracle):
+@pytest.mark.parametrize("latency", [False, True])
+def test_resident_time_varying_and_quanta(eng, oracle, latency):
     ctx = eng.context()
     rng = np.random.default_rng(3)
     jobs, refs, _ = _jobs(eng, oracle, rng, 100, 6, tv=True, tmax=0.004)
     pb = eng.PackedBatch(ctx, jobs)
     # a few steps per launch, resuming from the recorded progress
     for _ in range(10000):
-        pb.launch(resident=True, max_steps=7)
+        pb.launch(resident=True, max_steps=7, latency=latency)
         st = pb.tab.cpu().numpy().view(eng.INST_DTYPE)["status"]
         if np.all(st == 1):
             break
@@ -111,7 +114,8 @@ def test_resident_sizes():
     assert {ctx.lib.dh_resident_rows(n) for n in have} == set(range(40, 105, 8))
 
 
-def test_resident_mixed_sizes_in_one_kernel(eng, oracle):
+@pytest.mark.parametrize("latency", [False, True])
+def test_resident_mixed_sizes_in_one_kernel(eng, oracle, latency):
     """Instances on different grid sizes served by one kernel (97 to 104
     points) run in one launch, each to the oracle's bits."""
     ctx = eng.context()
@@ -123,12 +127,13 @@ def test_resident_mixed_sizes_in_one_kernel(eng, oracle):
         refs += r
     pb = eng.PackedBatch(ctx, jobs)
     assert pb.resident_ok()
-    pb.launch(resident=True)
+    pb.launch(resident=True, latency=latency)
     for j, r in zip(jobs, refs):
         assert np.array_equal(j.phi.cpu().numpy(), r)
 
 
-def test_resident_rejects_other_shapes(eng, oracle):
+@pytest.mark.parametrize("latency", [False, True])
+def test_resident_rejects_other_shapes(eng, oracle, latency):
     """An instance whose n the kernel is not built for is left untouched
     with status -1; the others complete."""
     from dadi_amd import _lib
@@ -142,7 +147,7 @@ def test_resident_rejects_other_shapes(eng, oracle):
     ot = eng.PackedBatch(ctx, other).tab.cpu().numpy().view(eng.INST_DTYPE)
     tab[1] = ot[0]
     pb.tab.copy_(ctx.to_dev(tab.view(np.uint8)))
-    pb.launch_resident()
+    pb.launch_resident(latency=latency)
     st = pb.tab.cpu().numpy().view(eng.INST_DTYPE)["status"]
     assert list(st) == [1, -1, 1]
     assert np.array_equal(other[0].phi.cpu().numpy(), phi_o)
@@ -158,8 +163,9 @@ def _mark_exact(eng, ctx, pb, which):
     pb.tab.copy_(ctx.to_dev(tab.view(np.uint8)))
 
 
+@pytest.mark.parametrize("latency", [False, True])
 @pytest.mark.parametrize("tv", [False, True])
-def test_resident_exact_kernel_equals_fast(eng, oracle, tv):
+def test_resident_exact_kernel_equals_fast(eng, oracle, tv, latency):
     """Instances marked for exact division (run by the second kernel only)
     give the bits of the fast kernel and of the oracle."""
     ctx = eng.context()
@@ -167,7 +173,7 @@ def test_resident_exact_kernel_equals_fast(eng, oracle, tv):
     jobs, refs, phi0 = _jobs(eng, oracle, rng, 90, 8, tv=tv, frozen_every=3, tmax=0.005)
     pb = eng.PackedBatch(ctx, jobs)
     _mark_exact(eng, ctx, pb, range(0, 8, 2))
-    pb.launch(resident=True)
+    pb.launch(resident=True, latency=latency)
     tab = pb.tab.cpu().numpy().view(eng.INST_DTYPE)
     assert list(tab["status"]) == [1] * 8
     assert list(tab["reserved"] & 1) == [1, 0] * 4
@@ -176,8 +182,9 @@ def test_resident_exact_kernel_equals_fast(eng, oracle, tv):
         assert np.array_equal(np.signbit(j.phi.cpu().numpy()), np.signbit(r))
 
 
+@pytest.mark.parametrize("latency", [False, True])
 @pytest.mark.parametrize("tv", [False, True])
-def test_resident_tiny_values_go_to_exact_kernel(eng, oracle, tv):
+def test_resident_tiny_values_go_to_exact_kernel(eng, oracle, tv, latency):
     """phi with entries far below 2^-900, or negative zeros, makes quotients
     leave the fast kernel's exact range: those integrations are marked and
     redone by the exactly dividing kernel.  Positive zeros (phi after a
@@ -226,7 +233,7 @@ def test_resident_tiny_values_go_to_exact_kernel(eng, oracle, tv):
             refs.append(ref)
         pb = eng.PackedBatch(ctx, jobs)
         for _ in range(10000):
-            pb.launch(resident=True, max_steps=quanta)
+            pb.launch(resident=True, max_steps=quanta, latency=latency)
             tab = pb.tab.cpu().numpy().view(eng.INST_DTYPE)
             if np.all(tab["status"] == 1):
                 break
