@@ -1688,14 +1688,18 @@ __device__ __forceinline__ void rr_take(double &xi, double &df, double &w0, doub
 // denormal or non-finite operand, b beyond 2^1022, x below 2^-969, x / b
 // denormal or above 2^767); elsewhere b' = b, x' = x and the result is
 // fma(e, y, q).  rr_recip and rr_quot are that sequence with b' = b and
-// x' = x, so y is shared by both quotients of a row.  rr_in records whether
-// the operands were inside the range where this gives the bits of x / b:
-// with |b| in [2^-60, 2^100] and the quotient in [2^-900, 2^600] (the
-// quotient is accurate to an ulp or two, so these margins cover its error)
-// every condition above holds.  So does x = +0 with such a b: div_fixup
-// returns the zero of sign(b), and so does fma(e, y, q) (q = +0 * y, e = +0);
-// for x = -0 and b > 0 it would give +0, not -0.  A lane that leaves the
-// range marks its integration for the exactly dividing kernel.
+// x' = x, so y is shared by both quotients of a row.  With b in [2^-60,
+// 2^100] and x in [2^-800, 2^500] every condition above holds, so the result
+// is the bits of x / b.  rr_quot0 forms the residual as b q - x and
+// subtracts it, fma(-(fma(b, q, -x)), y, q): for nonzero x the same value
+// (round to nearest is symmetric, and a zero residual adds nothing to q), and
+// for a zero x with b > 0 the zero of x's sign, as x / b gives (rr_quot gives
+// +0 for x = -0).  The recurrences' numerators are checked by their binary
+// exponent (rr_track: v_frexp_exp and one unsigned compare), which leaves
+// zeros alone; what else the argument needs -- the coefficients and dt in
+// range and every value finite -- is checked once per segment
+// (rr_coeffs_in_range) and when an instance is taken up (rr_phi_in_range).
+// A lane that fails marks its integration for the exactly dividing kernel.
 __device__ __forceinline__ double rr_recip(double b)
 {
     double y = __builtin_amdgcn_rcp(b);
@@ -1712,24 +1716,43 @@ __device__ __forceinline__ double rr_quot(double x, double b, double y)
     return __builtin_fma(e, y, q);
 }
 
+__device__ __forceinline__ double rr_quot0(double x, double b, double y)
+{
+    const double q = x * y;
+    const double e = __builtin_fma(b, q, -x);
+    return __builtin_fma(-e, y, q);
+}
+
 __device__ __forceinline__ bool rr_in(double x, double lo, double hi)
 {
     return (__builtin_fabs(x) >= lo) & (__builtin_fabs(x) <= hi);
 }
 
 constexpr double RR_DEN_LO = 0x1p-60, RR_DEN_HI = 0x1p100;
-constexpr double RR_QUO_LO = 0x1p-900, RR_QUO_HI = 0x1p600;
+constexpr double RR_QUO_LO = 0x1p-900;
+constexpr double RR_COEF_HI = 0x1p100;   // |a_j|, |gamma_j|, 1/dt
+constexpr double RR_PHI_HI = 0x1p600;    // phi when an instance is taken up
+// numerators x of the recurrences' quotients: 0, or 2^(e-1) <= |x| < 2^e
+// with e = frexp_exp(x) in [RR_EXP_LO, RR_EXP_HI], i.e. |x| in [2^-800, 2^500)
+constexpr int RR_EXP_LO = -799, RR_EXP_HI = 500;
 
-// okm &= the lanes where the quotient q of x by an in-range b is exact: q in
-// the quotient range, or x = +0.  A ballot of each compare, folded into okm
-// at once (an empty asm holds okm in scalar registers, so the masks are not
-// kept for a later fold).
-__device__ __forceinline__ void rr_keep_exact(unsigned long long &okm, double q, double x)
+// Positive and in [2^-60, 2^100] (false for NaN).
+__device__ __forceinline__ bool rr_pos(double b)
 {
-    okm &= (__builtin_amdgcn_ballot_w64(__builtin_fabs(q) >= RR_QUO_LO) &
-            __builtin_amdgcn_ballot_w64(__builtin_fabs(q) <= RR_QUO_HI)) |
-           __builtin_amdgcn_ballot_w64(__builtin_amdgcn_class(x, 64));   // class 64: +0
-    asm volatile("" : "+s"(okm));
+    return (b >= RR_DEN_LO) & (b <= RR_DEN_HI);
+}
+
+// bad |= the lanes whose numerator x has its binary exponent outside
+// [RR_EXP_LO, RR_EXP_HI] (one unsigned compare: an exponent below RR_EXP_LO
+// wraps to a large value), folded at once into the wave mask in scalar
+// registers (the empty asm).  frexp_exp gives 0 for zeros (accepted) and
+// for infinities and NaN, which the segment and take-up checks keep out of
+// the recurrences.
+__device__ __forceinline__ void rr_track(unsigned long long &bad, double x)
+{
+    const unsigned e = (unsigned)(__builtin_amdgcn_frexp_exp(x) - RR_EXP_LO);
+    bad |= __builtin_amdgcn_ballot_w64(e > (unsigned)(RR_EXP_HI - RR_EXP_LO));
+    asm volatile("" : "+s"(bad));
 }
 
 // Loads row j's tables (after `dep`: see rr_after).
@@ -1814,12 +1837,12 @@ __device__ __forceinline__ void rr_coef_last(double df, double inv_dt, bool bcl_
 // into g.  FAST: the quotients by rr_recip / rr_quot (and pj / dt from
 // y_dt, the reciprocal of dt), gamma_{j+1} only if fast_g (else by
 // division: the last rows, where c_j = -0 at x = 1 when the advection term
-// is zero); u_j's exactness goes into okm (bet_j and gamma_{j+1} are the same
-// at every step of a segment: rr_coeffs_in_range checks them once).
+// is zero); the numerators' exponents go into bad (bet_j and gamma_{j+1} are
+// the same at every step of a segment: rr_coeffs_in_range checks them once).
 template <int ORDER, bool FAST>
 __device__ __forceinline__ void rr_solve_mid(double pj, double a, double b, double cj,
                                              double dt, double inv_dt, double y_dt, bool fast_g,
-                                             double &g, double &u, unsigned long long &okm)
+                                             double &g, double &u, unsigned long long &bad)
 {
     const double bet = b - a * g;
     if constexpr (FAST) {
@@ -1828,12 +1851,12 @@ __device__ __forceinline__ void rr_solve_mid(double pj, double a, double b, doub
         if constexpr (ORDER == DH_ORDER_PRE) {
             r = inv_dt * pj;
         } else {
-            r = rr_quot(pj, dt, y_dt);
-            rr_keep_exact(okm, r, pj);
+            rr_track(bad, pj);
+            r = rr_quot0(pj, dt, y_dt);
         }
         const double x = r - a * u;
-        u = rr_quot(x, bet, y);
-        rr_keep_exact(okm, u, x);
+        rr_track(bad, x);
+        u = rr_quot0(x, bet, y);
         g = fast_g ? rr_quot(cj, bet, y) : cj / bet;
     } else {
         const double r = (ORDER == DH_ORDER_PRE) ? inv_dt * pj : pj / dt;
@@ -1852,36 +1875,54 @@ __device__ __forceinline__ void rr_solve_last(double pj, double a, double b, dou
     u = (r - a * u) / bet;
 }
 
-// Whether every bet_j of rows 1..n-2 of this lane's line (the axis' tables
-// at T) is inside rr_recip / rr_quot's exact range, and every gamma_{j+1}
-// of rows 1..fast_rows-1 (those rr_sweep computes by rr_quot) is exact, by
-// the operations of rr_sweep's rows: within a segment these values are the
-// same at every step.
+// Whether a segment's coefficients of this lane's line (the axis' tables at
+// T), formed with the operations of rr_sweep's rows, keep its quotients
+// exact and its values finite: bet_j of every row positive and in [2^-60,
+// 2^100], |a_j| and |gamma_{j+1}| at most 2^100, and every gamma_{j+1} that
+// rr_sweep forms by rr_quot (rows 1..fast_rows-1) at least 2^-900 or from
+// c_j = +0.  Within a segment these values are the same at every step.
 template <int ORDER>
 __device__ __forceinline__ bool rr_coeffs_in_range(int n, int fast_rows, const double *T,
                                                    double c, double m, double inv_dt,
-                                                   bool bcf_on, double bcf)
+                                                   bool bcf_on, double bcf, bool bcl_on,
+                                                   double bcl)
 {
     double pa, pc;
     const double bet0 = rr_row0<ORDER>(T[0], T[1], T[2], T[3], T[4], c, m, inv_dt, bcf_on,
                                        bcf, pa, pc);
     double g = (-T[1] * pc) / bet0;
-    bool in = true;
+    bool in = rr_pos(bet0) & (__builtin_fabs(g) <= RR_COEF_HI);
 #pragma unroll 1
     for (int j = 1; j < n - 1; ++j) {
         const double *R = T + j * RR_TS;
         double a, b, cj;
         rr_coef_mid<ORDER>(R[0], R[1], R[2], R[3], R[4], c, m, inv_dt, pa, pc, a, b, cj);
         const double bet = b - a * g;
-        in = in & rr_in(bet, RR_DEN_LO, RR_DEN_HI);
+        in = in & rr_pos(bet) & (__builtin_fabs(a) <= RR_COEF_HI);
         if (j < fast_rows) {
             g = rr_quot(cj, bet, rr_recip(bet));
-            in = in & (rr_in(g, RR_QUO_LO, RR_QUO_HI) |
+            in = in & (rr_in(g, RR_QUO_LO, RR_COEF_HI) |
                        (__builtin_bit_cast(unsigned long long, cj) == 0));
         } else {
             g = cj / bet;
+            in = in & (__builtin_fabs(g) <= RR_COEF_HI);
         }
     }
+    double a, b;
+    rr_coef_last<ORDER>(T[(n - 1) * RR_TS + 1], inv_dt, bcl_on, bcl, pa, pc, a, b);
+    return in & rr_pos(b - a * g) & (__builtin_fabs(a) <= RR_COEF_HI);
+}
+
+// Whether this lane's line of phi (rows 0..n-1) is finite with |phi| at most
+// 2^600, which with the segment checks keeps every value of the recurrences
+// finite until a numerator leaves its range.
+template <int NP, class Line>
+__device__ __forceinline__ bool rr_phi_in_range(const Line &P, int n)
+{
+    bool in = true;
+#pragma unroll
+    for (int i = 0; i < NP; ++i)
+        if (i < n) in = in & (__builtin_fabs(P.get(i)) <= RR_PHI_HI);
     return in;
 }
 
@@ -1906,7 +1947,7 @@ __device__ __forceinline__ void rr_sweep(RrLine<NP> &P, RrAcc (&G)[NP], int n,
                                          const double *lds, int off, double c, double m,
                                          double dt, double inv_dt, double y_dt, bool bcf_on,
                                          double bcf, bool bcl_on, double bcl, int inj_row,
-                                         double inj, unsigned long long &okm)
+                                         double inj, unsigned long long &bad)
 {
     constexpr int SURE = NP - RR_SPAN;   // rows 0..SURE exist; rows < SURE are not the last
     if (inj_row == 0) P.set(0, P.get(0) + inj);
@@ -1940,7 +1981,7 @@ __device__ __forceinline__ void rr_sweep(RrLine<NP> &P, RrAcc (&G)[NP], int n,
         rr_take(xi, df, w0, w1, sl, nxi, ndf, nw0, nw1, nsl);
         if (j + 2 < NP) rr_fetch<NP>(lds, off, u, j + 2, nxi, ndf, nw0, nw1, nsl);
         rr_solve_mid<ORDER, FAST>(P.get(j), ca, cb, cc, dt, inv_dt, y_dt,
-                                  j < rr_fast_g_rows<NP>(), g, u, okm);
+                                  j < rr_fast_g_rows<NP>(), g, u, bad);
         P.set(j, u);
         G[j + 1] = rr_acc_put(g);
         if (j + 1 < SURE || j + 1 < n - 1)
@@ -1959,7 +2000,7 @@ __device__ __forceinline__ void rr_sweep(RrLine<NP> &P, RrAcc (&G)[NP], int n,
         if (j == SURE || j < n) {
             if (j < n - 1) {
                 rr_solve_mid<ORDER, FAST>(P.get(j), ca, cb, cc, dt, inv_dt, y_dt, false, g, u,
-                                          okm);
+                                          bad);
                 P.set(j, u);
                 if (j + 1 < NP) G[j + 1] = rr_acc_put(g);
                 if (j + 1 < n - 1)
@@ -2048,9 +2089,10 @@ __device__ __forceinline__ void rr_transpose(RrLine<NP> &P, RrAcc (&X)[NP > 64 ?
 }
 
 // Whether a lane of this wave that holds a line left the exact range.
-__device__ __forceinline__ int rr_wave_out(unsigned long long okm, int L, int n)
+__device__ __forceinline__ int rr_wave_out(unsigned long long okm, unsigned long long bad,
+                                           int L, int n)
 {
-    return (~okm & __builtin_amdgcn_ballot_w64(L < n)) != 0;
+    return ((~okm | bad) & __builtin_amdgcn_ballot_w64(L < n)) != 0;
 }
 
 // FAST: the quotients of rows 1..n-2 by rr_recip / rr_quot.  A workgroup
@@ -2115,9 +2157,12 @@ k_integrate_rr(dh_instance *__restrict__ tab, int ninst, int *__restrict__ count
         asm volatile("" ::: "memory");
         const long long t0 = max_ticks > 0 ? (long long)wall_clock64() : 0;
         bool out_of_time = false, failed = false;
-        // FAST: the lanes whose quotients so far were in range (lanes past
-        // the last line hold no line and count as in range)
-        unsigned long long okm = __builtin_amdgcn_ballot_w64(true);
+        // FAST: the lanes whose phi and segments so far were in range (lanes
+        // past the last line hold no line and count as in range), and those
+        // with a numerator out of range so far (rr_track)
+        unsigned long long okm = __builtin_amdgcn_ballot_w64(
+            !FAST || L >= n || rr_phi_in_range<NP>(P, n));
+        unsigned long long bad = 0;
         if (first_axis == 2) seg = nseg;   // every axis frozen: the identity
         while (seg < nseg && budget > 0 && !out_of_time && !failed) {
             const dh_segment *sp = gbl(ip->seg_dev) + seg;
@@ -2132,7 +2177,7 @@ k_integrate_rr(dh_instance *__restrict__ tab, int ninst, int *__restrict__ count
             const double y_dt = rr_recip(dt);
             const int line = L < n ? L : 0;
             if constexpr (FAST) {
-                bool in = ORDER == DH_ORDER_PRE || rr_in(dt, RR_DEN_LO, RR_DEN_HI);
+                bool in = rr_pos(dt) & (__builtin_fabs(inj) <= RR_PHI_HI);
 #pragma unroll 1
                 for (int a = 0; a < 2; ++a) {
                     if ((frozen >> a) & 1) continue;
@@ -2143,7 +2188,8 @@ k_integrate_rr(dh_instance *__restrict__ tab, int ninst, int *__restrict__ count
                                          sp->chrom);
                     in = in & rr_coeffs_in_range<ORDER>(n, rr_fast_g_rows<NP>(),
                                                         a ? blk1 : blk0, Ls.c[0], mg[0],
-                                                        inv_dt, Ls.bcf_on, Ls.bcf);
+                                                        inv_dt, Ls.bcf_on, Ls.bcf, Ls.bcl_on,
+                                                        Ls.bcl);
                 }
                 okm &= __builtin_amdgcn_ballot_w64(in || L >= n);
             }
@@ -2164,11 +2210,11 @@ k_integrate_rr(dh_instance *__restrict__ tab, int ninst, int *__restrict__ count
                         const int off = (int)((const char *)(a ? blk1 : blk0) - (const char *)lds);
                         rr_sweep<NP, ORDER, FAST>(P, G, n, lds, off, Ls.c[0], mg[0], dt, inv_dt,
                                                   y_dt, Ls.bcf_on, Ls.bcf, Ls.bcl_on, Ls.bcl,
-                                                  Ls.inj_row, inj, okm);
+                                                  Ls.inj_row, inj, bad);
                     }
                     rr_transpose<NP>(P, X, n, L, wave, tile);
                 }
-                if (FAST && (k & 31) == 31 && __syncthreads_or(rr_wave_out(okm, L, n))) {
+                if (FAST && (k & 31) == 31 && __syncthreads_or(rr_wave_out(okm, bad, L, n))) {
                     nst = k + 1;
                     failed = true;
                     break;
@@ -2190,7 +2236,7 @@ k_integrate_rr(dh_instance *__restrict__ tab, int ninst, int *__restrict__ count
                 step = 0;
             }
         }
-        if (FAST && (failed || __syncthreads_or(rr_wave_out(okm, L, n)))) {
+        if (FAST && (failed || __syncthreads_or(rr_wave_out(okm, bad, L, n)))) {
             // phi and the progress are as taken up: the exact kernel redoes them
             if (tid == 0) ip->reserved |= 1;
             continue;
@@ -2248,7 +2294,7 @@ constexpr long long rf_slice_doubles() { return 2LL * 2 * NP * NP * 2; }
 
 // The factorisation of line `line` for one axis, stored (if `store`) as the
 // records of rows 0..n-1 at AB[j * NP + line] / YG[j * NP + line]; returns
-// whether every bet_j of rows
+// the checks of rr_coeffs_in_range: whether every bet_j of rows
 // 1..n-2 and every gamma_{j+1} formed by rr_quot is in the range where the
 // shared-reciprocal quotients give the bits of division (rr_coeffs_in_range).
 // Row 0 keeps bet_0 and gamma_1 (its quotients are IEEE divisions), the last
@@ -2267,7 +2313,7 @@ __device__ __forceinline__ bool rf_factor(double2 *__restrict__ AB, double2 *__r
         AB[line] = make_double2(0., bet0);
         YG[line] = make_double2(0., g);
     }
-    bool in = true;
+    bool in = rr_pos(bet0) & (__builtin_fabs(g) <= RR_COEF_HI);
     double ca, cb, cc;
     if (1 < n - 1)
         rr_coef_mid<ORDER>(T[RR_TS], T[RR_TS + 1], T[RR_TS + 2], T[RR_TS + 3], T[RR_TS + 4], c,
@@ -2278,14 +2324,15 @@ __device__ __forceinline__ bool rf_factor(double2 *__restrict__ AB, double2 *__r
     for (int j = 1; j < n - 1; ++j) {
         const double bet = cb - ca * g;
         const double y = rr_recip(bet);
-        in = in & rr_in(bet, RR_DEN_LO, RR_DEN_HI);
+        in = in & rr_pos(bet) & (__builtin_fabs(ca) <= RR_COEF_HI);
         double gn;
         if (j < rr_fast_g_rows<NP>()) {
             gn = rr_quot(cc, bet, y);
-            in = in & (rr_in(gn, RR_QUO_LO, RR_QUO_HI) |
+            in = in & (rr_in(gn, RR_QUO_LO, RR_COEF_HI) |
                        (__builtin_bit_cast(unsigned long long, cc) == 0));
         } else {
             gn = cc / bet;
+            in = in & (__builtin_fabs(gn) <= RR_COEF_HI);
         }
         if (store) {
             AB[(long long)j * NP + line] = make_double2(ca, bet);
@@ -2298,8 +2345,9 @@ __device__ __forceinline__ bool rf_factor(double2 *__restrict__ AB, double2 *__r
         else
             rr_coef_last<ORDER>(R[1], inv_dt, bcl_on, bcl, pa, pc, ca, cb);
     }
-    if (store) AB[(long long)(n - 1) * NP + line] = make_double2(ca, cb - ca * g);
-    return in;
+    const double betl = cb - ca * g;
+    if (store) AB[(long long)(n - 1) * NP + line] = make_double2(ca, betl);
+    return in & rr_pos(betl) & (__builtin_fabs(ca) <= RR_COEF_HI);
 }
 
 // One implicit half-step of this lane's line from its factorisation (the
@@ -2311,7 +2359,7 @@ __device__ __forceinline__ void rf_sweep(RrLine<NP, rf_vrows<NP>()> &P,
                                          const double2 *__restrict__ AB,
                                          const double2 *__restrict__ YG, int n, int L, double dt,
                                          double inv_dt, double y_dt, int inj_row, double inj,
-                                         unsigned long long &okm)
+                                         unsigned long long &bad)
 {
     constexpr int SURE = NP - RR_SPAN;   // rows 0..SURE exist; rows < SURE are not the last
     constexpr size_t ROW = (size_t)NP * sizeof(double2);   // bytes per row of records
@@ -2355,12 +2403,12 @@ __device__ __forceinline__ void rf_sweep(RrLine<NP, rf_vrows<NP>()> &P,
                 if constexpr (ORDER == DH_ORDER_PRE) {
                     r = inv_dt * pj;
                 } else {
-                    r = rr_quot(pj, dt, y_dt);
-                    rr_keep_exact(okm, r, pj);
+                    rr_track(bad, pj);
+                    r = rr_quot0(pj, dt, y_dt);
                 }
                 const double x = r - ab.x * u;
-                u = rr_quot(x, ab.y, y);
-                rr_keep_exact(okm, u, x);
+                rr_track(bad, x);
+                u = rr_quot0(x, ab.y, y);
             } else {
                 const double r = (ORDER == DH_ORDER_PRE) ? inv_dt * pj : pj / dt;
                 u = (r - ab.x * u) / ab.y;
@@ -2465,7 +2513,9 @@ k_integrate_rf(dh_instance *__restrict__ tab, int ninst, int *__restrict__ count
         asm volatile("" ::: "memory");
         const long long t0 = max_ticks > 0 ? (long long)wall_clock64() : 0;
         bool out_of_time = false, failed = false;
-        unsigned long long okm = __builtin_amdgcn_ballot_w64(true);
+        unsigned long long okm =
+            __builtin_amdgcn_ballot_w64(L >= n || rr_phi_in_range<NP>(P, n));
+        unsigned long long bad = 0;
         if (first_axis == 2) seg = nseg;
         while (seg < nseg && budget > 0 && !out_of_time && !failed) {
             const dh_segment *sp = gbl(ip->seg_dev) + seg;
@@ -2486,7 +2536,7 @@ k_integrate_rf(dh_instance *__restrict__ tab, int ninst, int *__restrict__ count
             const double dt = sp->dt, inv_dt = 1. / dt, inj = sp->inject;
             const double y_dt = rr_recip(dt);
             const int line = L < n ? L : 0;
-            bool in = ORDER == DH_ORDER_PRE || rr_in(dt, RR_DEN_LO, RR_DEN_HI);
+            bool in = rr_pos(dt) & (__builtin_fabs(inj) <= RR_PHI_HI);
             int inj_row[2] = {-1, -1};
 #pragma unroll 1
             for (int a = 0; a < 2; ++a) {
@@ -2513,11 +2563,11 @@ k_integrate_rf(dh_instance *__restrict__ tab, int ninst, int *__restrict__ count
                     if (!((frozen >> a) & 1)) {
                         const double2 *AB = ws + (long long)a * 2 * NP * NP;
                         rf_sweep<NP, ORDER>(P, AB, AB + (long long)NP * NP, n, line, dt, inv_dt,
-                                            y_dt, inj_row[a], inj, okm);
+                                            y_dt, inj_row[a], inj, bad);
                     }
                     rf_transpose<NP>(P, n, L, tile);
                 }
-                if ((k & 31) == 31 && __syncthreads_or(rr_wave_out(okm, L, n))) {
+                if ((k & 31) == 31 && __syncthreads_or(rr_wave_out(okm, bad, L, n))) {
                     nst = k + 1;
                     failed = true;
                     break;
@@ -2539,7 +2589,7 @@ k_integrate_rf(dh_instance *__restrict__ tab, int ninst, int *__restrict__ count
                 step = 0;
             }
         }
-        if (failed || __syncthreads_or(rr_wave_out(okm, L, n))) {
+        if (failed || __syncthreads_or(rr_wave_out(okm, bad, L, n))) {
             // phi and the progress are as taken up: the exact kernel redoes them
             if (tid == 0) ip->reserved |= 1;
             continue;

@@ -185,12 +185,13 @@ def test_resident_exact_kernel_equals_fast(eng, oracle, tv, latency):
 @pytest.mark.parametrize("latency", [False, True])
 @pytest.mark.parametrize("tv", [False, True])
 def test_resident_tiny_values_go_to_exact_kernel(eng, oracle, tv, latency):
-    """phi with entries far below 2^-900, or negative zeros, makes quotients
+    """phi with entries far below 2^-800 makes the recurrences' numerators
     leave the fast kernel's exact range: those integrations are marked and
-    redone by the exactly dividing kernel.  Positive zeros (phi after a
+    redone by the exactly dividing kernel.  Zeros of either sign (phi after a
     split is zero off the diagonal) and ordinary values stay on the fast
-    kernel.  Every result is the oracle's, also when run in quanta of a few
-    steps."""
+    kernel, whose quotient gives a zero numerator's sign as division does.
+    Every result is the oracle's, zeros' signs included, also when run in
+    quanta of a few steps."""
     from dadi_amd import schedule as S, _lib
     ctx = eng.context()
     rng = np.random.default_rng(23 + tv)
@@ -240,8 +241,38 @@ def test_resident_tiny_values_go_to_exact_kernel(eng, oracle, tv, latency):
         assert np.all(tab["status"] == 1)
         marked = [int(v) for v in tab["reserved"] & 1]
         assert marked[0::4] == [0, 0] and marked[1::4] == [1, 1], marked
-        assert marked[2::4] == [0, 0] and marked[3::4] == [1, 1], marked
+        assert marked[2::4] == [0, 0] and marked[3::4] == [0, 0], marked
         for j, r in zip(jobs, refs):
             got = j.phi.cpu().numpy()
             assert np.array_equal(got, r)
             assert np.array_equal(np.signbit(got), np.signbit(r))   # zeros' signs too
+
+
+@pytest.mark.parametrize("latency", [False, True])
+def test_resident_out_of_range_phi_goes_to_exact_kernel(eng, oracle, latency):
+    """An instance whose phi holds a value beyond 2^600 when it is taken up
+    runs on the exactly dividing kernel (the fast kernel's checks assume
+    finite recurrences); its neighbours stay on the fast kernel; every result
+    is the oracle's."""
+    ctx = eng.context()
+    rng = np.random.default_rng(31)
+    jobs, refs, phi0 = _jobs(eng, oracle, rng, 80, 4)
+    from dadi_amd import schedule as S
+    big = phi0.copy()
+    big[5, 7] = 1e200
+    j = jobs[2]
+    nu1, nu2 = 1.5, 0.7
+    dt = min(S.compute_dt(nu1, [0.3], 0, 0.5), S.compute_dt(nu2, [0.3], 0, 0.5))
+    xx = oracle.default_grid(80)
+    j.seg = S.const_segments(2, xx, 0.0, 0.002, dt, [nu1, nu2], [[0.3], [0.3]], [0, 0],
+                             [0.5, 0.5], 1.0)
+    j.phi.copy_(ctx.to_dev(big))
+    refs[2] = oracle.integrate_const(big, xx, 0.002, [nu1, nu2], [[0.3], [0.3]], [0, 0],
+                                     [0.5, 0.5])
+    pb = eng.PackedBatch(ctx, jobs)
+    pb.launch(resident=True, latency=latency)
+    tab = pb.tab.cpu().numpy().view(eng.INST_DTYPE)
+    assert list(tab["status"]) == [1] * 4
+    assert list(tab["reserved"] & 1) == [0, 0, 1, 0]
+    for jb, r in zip(jobs, refs):
+        assert np.array_equal(jb.phi.cpu().numpy(), r)
