@@ -107,6 +107,7 @@ _SIGS = {
     "dh_integrate_resident_latency": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp,
                                        ctypes.POINTER(DhLaunch), _vp], ctypes.c_int),
     "dh_resident_latency_slice_doubles": ([ctypes.c_int], ctypes.c_longlong),
+    "dh_resident_latency_rows": ([ctypes.c_int], ctypes.c_int),
     "dh_sweep_axis": ([ctypes.POINTER(DhInstance), ctypes.c_int, ctypes.c_int, _vp,
                        ctypes.c_longlong, _vp, ctypes.c_longlong, ctypes.c_int, _vp],
                       ctypes.c_int),

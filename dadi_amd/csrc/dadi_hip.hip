@@ -2271,7 +2271,10 @@ k_integrate_rr(dh_instance *__restrict__ tab, int ninst, int *__restrict__ count
 // workgroup per integration, one wave per SIMD: for single integrations and
 // batches too small to fill the GPU.  The same exactness checks as
 // k_integrate_rr; an instance that leaves their range is handed to the
-// exactly dividing k_integrate_rr kernel.
+// exactly dividing form (FAST false: IEEE divisions in the recurrences and
+// in every gamma, the bits of k_integrate_rr's exact twin), which runs only
+// the instances so marked.  NP up to 128 (two waves; the tile's LDS allows
+// one workgroup per compute unit from NP = 96 on).
 // ---------------------------------------------------------------------------
 constexpr int RF_PF = 8;       // rows of records in flight in the forward pass
 constexpr int RF_PB = 8;       // rows of gamma in flight in the back substitution
@@ -2299,7 +2302,7 @@ constexpr long long rf_slice_doubles() { return 2LL * 2 * NP * NP * 2; }
 // shared-reciprocal quotients give the bits of division (rr_coeffs_in_range).
 // Row 0 keeps bet_0 and gamma_1 (its quotients are IEEE divisions), the last
 // row a and bet.
-template <int NP, int ORDER>
+template <int NP, int ORDER, bool FAST>
 __device__ __forceinline__ bool rf_factor(double2 *__restrict__ AB, double2 *__restrict__ YG,
                                           int n, int line, bool store, const double *T,
                                           double c, double m, double inv_dt, bool bcf_on,
@@ -2323,10 +2326,10 @@ __device__ __forceinline__ bool rf_factor(double2 *__restrict__ AB, double2 *__r
 #pragma unroll 1
     for (int j = 1; j < n - 1; ++j) {
         const double bet = cb - ca * g;
-        const double y = rr_recip(bet);
+        const double y = FAST ? rr_recip(bet) : 0.;
         in = in & rr_pos(bet) & (__builtin_fabs(ca) <= RR_COEF_HI);
         double gn;
-        if (j < rr_fast_g_rows<NP>()) {
+        if (FAST && j < rr_fast_g_rows<NP>()) {
             gn = rr_quot(cc, bet, y);
             in = in & (rr_in(gn, RR_QUO_LO, RR_COEF_HI) |
                        (__builtin_bit_cast(unsigned long long, cc) == 0));
@@ -2354,7 +2357,7 @@ __device__ __forceinline__ bool rf_factor(double2 *__restrict__ AB, double2 *__r
 // records of line L): the operations of rr_sweep's rows (row 0 and the last
 // row by IEEE division, rows 1..n-2 by the shared reciprocal y_j), records
 // read RF_PF rows ahead.
-template <int NP, int ORDER>
+template <int NP, int ORDER, bool FAST>
 __device__ __forceinline__ void rf_sweep(RrLine<NP, rf_vrows<NP>()> &P,
                                          const double2 *__restrict__ AB,
                                          const double2 *__restrict__ YG, int n, int L, double dt,
@@ -2398,7 +2401,7 @@ __device__ __forceinline__ void rf_sweep(RrLine<NP, rf_vrows<NP>()> &P,
         }
         if (j < SURE || j < n) {
             const double pj = P.get(j);
-            if (j < SURE || j < n - 1) {
+            if (FAST && (j < SURE || j < n - 1)) {
                 double r;
                 if constexpr (ORDER == DH_ORDER_PRE) {
                     r = inv_dt * pj;
@@ -2465,7 +2468,7 @@ __device__ __forceinline__ void rf_transpose(RrLine<NP, rf_vrows<NP>()> &P, int 
     __syncthreads();
 }
 
-template <int NP, int ORDER>
+template <int NP, int ORDER, bool FAST>
 __global__ void __launch_bounds__(64 * rr_waves<NP>()) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_integrate_rf(dh_instance *__restrict__ tab, int ninst, int *__restrict__ counter,
                long long max_steps, long long max_ticks, double *__restrict__ ws_all,
@@ -2494,8 +2497,9 @@ k_integrate_rf(dh_instance *__restrict__ tab, int ninst, int *__restrict__ count
             continue;
         }
         __builtin_assume(n > NP - RR_SPAN && n <= NP);
-        // instances marked for exact division are left to k_integrate_rr
-        if ((__builtin_amdgcn_readfirstlane(ip->reserved) & 1) != 0) continue;
+        // the fast form leaves instances marked for exact division to the
+        // exact form, which runs only those
+        if (FAST == ((__builtin_amdgcn_readfirstlane(ip->reserved) & 1) != 0)) continue;
         long long seg = ip->cur_seg, step = ip->cur_step;
         const long long nseg = ip->nseg;
         long long budget = max_steps > 0 ? max_steps : LLONG_MAX;
@@ -2514,7 +2518,7 @@ k_integrate_rf(dh_instance *__restrict__ tab, int ninst, int *__restrict__ count
         const long long t0 = max_ticks > 0 ? (long long)wall_clock64() : 0;
         bool out_of_time = false, failed = false;
         unsigned long long okm =
-            __builtin_amdgcn_ballot_w64(L >= n || rr_phi_in_range<NP>(P, n));
+            __builtin_amdgcn_ballot_w64(!FAST || L >= n || rr_phi_in_range<NP>(P, n));
         unsigned long long bad = 0;
         if (first_axis == 2) seg = nseg;
         while (seg < nseg && budget > 0 && !out_of_time && !failed) {
@@ -2548,11 +2552,11 @@ k_integrate_rf(dh_instance *__restrict__ tab, int ninst, int *__restrict__ count
                 inj_row[a] = Ls.inj_row;
                 double2 *AB = ws + (long long)a * 2 * NP * NP, *YG = AB + (long long)NP * NP;
                 // lanes past the last line form line 0's values and store nothing
-                in = in & rf_factor<NP, ORDER>(AB, YG, n, line, L < n, a ? blk1 : blk0,
+                in = in & rf_factor<NP, ORDER, FAST>(AB, YG, n, line, L < n, a ? blk1 : blk0,
                                               Ls.c[0], mg[0], inv_dt, Ls.bcf_on, Ls.bcf,
                                               Ls.bcl_on, Ls.bcl);
             }
-            okm &= __builtin_amdgcn_ballot_w64(in || L >= n);
+            okm &= __builtin_amdgcn_ballot_w64(!FAST || in || L >= n);
             // the tables are read: the transposes may use the tile again
             __syncthreads();
             long long nst = sp->nsteps - step;
@@ -2562,12 +2566,13 @@ k_integrate_rf(dh_instance *__restrict__ tab, int ninst, int *__restrict__ count
                 for (int a = 0; a < 2; ++a) {
                     if (!((frozen >> a) & 1)) {
                         const double2 *AB = ws + (long long)a * 2 * NP * NP;
-                        rf_sweep<NP, ORDER>(P, AB, AB + (long long)NP * NP, n, line, dt, inv_dt,
+                        rf_sweep<NP, ORDER, FAST>(P, AB, AB + (long long)NP * NP, n, line, dt,
+                                                  inv_dt,
                                             y_dt, inj_row[a], inj, bad);
                     }
                     rf_transpose<NP>(P, n, L, tile);
                 }
-                if ((k & 31) == 31 && __syncthreads_or(rr_wave_out(okm, bad, L, n))) {
+                if (FAST && (k & 31) == 31 && __syncthreads_or(rr_wave_out(okm, bad, L, n))) {
                     nst = k + 1;
                     failed = true;
                     break;
@@ -2589,7 +2594,7 @@ k_integrate_rf(dh_instance *__restrict__ tab, int ninst, int *__restrict__ count
                 step = 0;
             }
         }
-        if (failed || __syncthreads_or(rr_wave_out(okm, bad, L, n))) {
+        if (FAST && (failed || __syncthreads_or(rr_wave_out(okm, bad, L, n)))) {
             // phi and the progress are as taken up: the exact kernel redoes them
             if (tid == 0) ip->reserved |= 1;
             continue;
@@ -3667,28 +3672,31 @@ const void *pick_rr(int n, int order, bool fast, size_t *lds, int *waves)
 
 // latency form of the register-resident kernel (k_integrate_rf): kernel, LDS
 // bytes, waves and workspace doubles per workgroup for n rows (null if none)
-template <int NP>
+template <int NP, bool FAST>
 const void *rf_kernel(int order)
 {
     switch (order) {
     case DH_ORDER_TV:
-        return reinterpret_cast<const void *>(&k_integrate_rf<NP, DH_ORDER_TV>);
+        return reinterpret_cast<const void *>(&k_integrate_rf<NP, DH_ORDER_TV, FAST>);
     case DH_ORDER_PRE:
-        return reinterpret_cast<const void *>(&k_integrate_rf<NP, DH_ORDER_PRE>);
+        return reinterpret_cast<const void *>(&k_integrate_rf<NP, DH_ORDER_PRE, FAST>);
     }
     return nullptr;
 }
 
-const void *pick_rf(int n, int order, size_t *lds, int *waves, long long *slice)
+const void *pick_rf(int n, int order, bool fast, size_t *lds, int *waves, long long *slice,
+                    int *rows = nullptr)
 {
 #define DH_RF(np)                                                                   \
     if (n > (np) - RR_SPAN && n <= (np)) {                                          \
         *lds = sizeof(double) * rf_lds_doubles<np>();                               \
         *waves = rr_waves<np>();                                                    \
         *slice = rf_slice_doubles<np>();                                            \
-        return rf_kernel<np>(order);                                                \
+        if (rows) *rows = np;                                                       \
+        return fast ? rf_kernel<np, true>(order) : rf_kernel<np, false>(order);     \
     }
     DH_RF(40) DH_RF(48) DH_RF(56) DH_RF(64) DH_RF(72) DH_RF(80) DH_RF(88) DH_RF(96) DH_RF(104)
+    DH_RF(112) DH_RF(120) DH_RF(128)
 #undef DH_RF
     return nullptr;
 }
@@ -4206,7 +4214,15 @@ long long dh_resident_latency_slice_doubles(int n)
     size_t lds = 0;
     int waves = 0;
     long long slice = 0;
-    return pick_rf(n, DH_ORDER_PRE, &lds, &waves, &slice) ? slice : 0;
+    return pick_rf(n, DH_ORDER_PRE, true, &lds, &waves, &slice) ? slice : 0;
+}
+
+int dh_resident_latency_rows(int n)
+{
+    size_t lds = 0;
+    int waves = 0, rows = 0;
+    long long slice = 0;
+    return pick_rf(n, DH_ORDER_PRE, true, &lds, &waves, &slice, &rows) ? rows : 0;
 }
 
 int dh_integrate_resident_latency(dh_instance *table_dev, int ninst, int order, int n,
@@ -4217,18 +4233,18 @@ int dh_integrate_resident_latency(dh_instance *table_dev, int ninst, int order, 
         return fail(DH_EINVAL, "order must be TV or PRE (2 populations)");
     if (ninst <= 0) return DH_OK;
     if (!table_dev || !counter_dev || !cfg) return fail(DH_EINVAL, "null pointer");
-    size_t lds = 0, rr_lds = 0;
-    int waves = 0, rr_w = 0;
+    size_t lds = 0;
+    int waves = 0;
     long long slice = 0;
-    const void *fast = pick_rf(n, order, &lds, &waves, &slice);
-    const void *exact = pick_rr(n, order, false, &rr_lds, &rr_w);
+    const void *fast = pick_rf(n, order, true, &lds, &waves, &slice);
+    const void *exact = pick_rf(n, order, false, &lds, &waves, &slice);
     if (!fast || !exact)
         return fail(DH_ERANGE, "no register-resident kernel for n = " + std::to_string(n));
     if (!cfg->gamma_dev)
         return fail(DH_EINVAL, "gamma_dev must name the factorisation workspace "
                                "(dh_resident_latency_slice_doubles(n) doubles per workgroup)");
     if (int e = set_lds(fast, lds)) return e;
-    if (int e = set_lds(exact, rr_lds)) return e;
+    if (int e = set_lds(exact, lds)) return e;
     // one wave per SIMD; the LDS tile may allow fewer workgroups per compute unit
     const int per_cu = std::max(1, std::min(4 / waves, (int)(g_max_lds / lds)));
     long long groups = cfg->groups > 0 ? cfg->groups
@@ -4240,16 +4256,12 @@ int dh_integrate_resident_latency(dh_instance *table_dev, int ninst, int order, 
     long long ms = cfg->max_steps, ticks = budget_ticks(cfg);
     double *ws = cfg->gamma_dev;
     void *args[] = {&table_dev, &ninst, &counter_dev, &ms, &ticks, &ws, &slice};
-    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, counter_dev, 0);
-    DH_HIP(hipLaunchKernel(fast, dim3((unsigned)groups), dim3(64 * waves), args, lds, st));
-    // the instances the fast kernel handed over (or found marked) run on the
-    // exactly dividing register-resident kernel
-    const long long rr_groups = cfg->groups > 0 ? cfg->groups
-                                                : std::min<long long>(ninst,
-                                                                      (long long)(4 / rr_w) * g_num_cu);
-    void *rargs[] = {&table_dev, &ninst, &counter_dev, &ms, &ticks};
-    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, counter_dev, 0);
-    DH_HIP(hipLaunchKernel(exact, dim3((unsigned)rr_groups), dim3(64 * rr_w), rargs, rr_lds, st));
+    // the fast form, then the exact form over the instances it handed over
+    // (or found marked by an earlier call)
+    for (const void *fn : {fast, exact}) {
+        hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, st, counter_dev, 0);
+        DH_HIP(hipLaunchKernel(fn, dim3((unsigned)groups), dim3(64 * waves), args, lds, st));
+    }
     return DH_OK;
 }
 

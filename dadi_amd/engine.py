@@ -111,6 +111,8 @@ class Context:
         dh_integrate_resident."""
         if latency is None:
             latency = ninst <= self.resident_latency_max_inst
+        if not self.lib.dh_resident_rows(int(nmax)):
+            latency = True    # above 104 points only the latency form exists
         if not latency:
             cfg = DhLaunch(1, groups, max_steps, 0, 0)
             cfg.max_us = int(max_us)
@@ -479,7 +481,7 @@ def run_single(phi, xx, seg, order, frozen, inject, ctx=None, mode="auto", waves
                 ctx.factorable([seg]):
             mode = "factored"
         if D == 2 and not delj and ctx.resident_auto and \
-                order in (_lib.ORDER_TV, _lib.ORDER_PRE) and ctx.lib.dh_resident_rows(n):
+                order in (_lib.ORDER_TV, _lib.ORDER_PRE) and ctx.lib.dh_resident_latency_rows(n):
             mode = "resident"
     stream = ctx.stream()
     if mode == "wg":
@@ -614,13 +616,15 @@ class PackedBatch:
         self.tab.copy_(self.tab0)
 
     def resident_ok(self):
-        """Whether the register-resident 2D kernel (dh_integrate_resident) can
-        run this batch: two populations, grid sizes within one of its kernels'
-        ranges, the TV or PRE coefficient order, no delta trick."""
+        """Whether the register-resident 2D kernels can run this batch: two
+        populations, grid sizes within one kernel's range (33..104 points for
+        dh_integrate_resident, up to 128 for its latency form), the TV or PRE
+        coefficient order, no delta trick."""
         if self.D != 2 or self.delj or self.order not in (_lib.ORDER_TV, _lib.ORDER_PRE):
             return False
-        rows = self.ctx.lib.dh_resident_rows(self.nmax)
-        return rows > 0 and all(self.ctx.lib.dh_resident_rows(j.n) == rows for j in self.jobs)
+        rows = self.ctx.lib.dh_resident_latency_rows(self.nmax)
+        return rows > 0 and all(self.ctx.lib.dh_resident_latency_rows(j.n) == rows
+                                for j in self.jobs)
 
     def launch_resident(self, groups=0, max_steps=0, max_us=0, stream=None, latency=None):
         """One register-resident launch over the table (Context.resident_launch:

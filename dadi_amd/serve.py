@@ -377,7 +377,7 @@ class Server:
         kernel)."""
         D, order = int(r.rec["D"][0]), int(r.rec["order"][0])
         if D == 2 and self.resident and order in (_lib.ORDER_TV, _lib.ORDER_PRE):
-            rows = self.ctx.lib.dh_resident_rows(int(r.rec["n"][0]))
+            rows = self.ctx.lib.dh_resident_latency_rows(int(r.rec["n"][0]))
             if rows:
                 return (D, order, rows)
         return (D, order)

@@ -276,26 +276,32 @@ int dh_integrate_resident(dh_instance *table_dev, int ninst, int order, int n,
  * 33..104). */
 int dh_resident_rows(int n);
 /*
- * Latency form of dh_integrate_resident, same arguments, shapes and results
- * (the bits of dh_integrate_nd): phi resident in registers, and at the start
+ * Latency form of dh_integrate_resident, same arguments and results (the bits
+ * of dh_integrate_nd), for 33 to 128 points (R = dh_resident_latency_rows(n);
+ * every instance between R - 7 and R points): phi resident in registers, and at the start
  * of each constant-parameter segment each lane forms its line's Thomas
  * factorisation once (a_j, bet_j, its reciprocal, gamma_{j+1}) into the
  * workspace cfg->gamma_dev, so that a time step runs only the right-hand-side
- * recurrences: about a third of dh_integrate_resident's step time for one
- * integration, more memory traffic per step (40 bytes per point and sweep,
- * from the L2), so it is meant for single integrations and batches too small
- * to fill the GPU.  cfg->gamma_dev must hold
+ * recurrences: three quarters of dh_integrate_resident's step time for one
+ * integration (20 against 27 us at 100 points), more memory traffic per step
+ * (32 bytes of records per point and sweep, from the L2), so it is meant for
+ * single integrations, batches too small to fill the GPU and grids above 104
+ * points, where dh_integrate_resident has no kernel.  cfg->gamma_dev must hold
  * groups * dh_resident_latency_slice_doubles(n) doubles (groups = cfg->groups,
  * or 0: min(ninst, workgroups that fit)) and may not be shared by launches
  * that overlap.  Instances whose checks fail, or marked by an earlier call,
- * run on dh_integrate_resident's exactly dividing kernel in the same call.
+ * run on the exactly dividing form of the same kernel (IEEE divisions, the
+ * same factorisation) in the same call.
  * Replaces what dh_integrate_resident replaces.
  */
 int dh_integrate_resident_latency(dh_instance *table_dev, int ninst, int order, int n,
                                   int *counter_dev, const dh_launch *cfg, void *stream);
 /* Workspace doubles per workgroup of dh_integrate_resident_latency for n
- * points, or 0 when there is no kernel (n outside 33..104). */
+ * points, or 0 when there is no kernel (n outside 33..128). */
 long long dh_resident_latency_slice_doubles(int n);
+/* Rows R of the dh_integrate_resident_latency kernel for n points (it
+ * serves n = R-7 .. R; R is a multiple of 8), or 0 (n outside 33..128). */
+int dh_resident_latency_rows(int n);
 /* LDS per workgroup of dh_integrate_1d_wave, bytes. */
 size_t dh_1d_wave_lds_bytes(int nmax);
 

@@ -86,6 +86,29 @@ def test_resident_equals_packed_and_oracle(eng, oracle, n, latency):
         assert np.array_equal(j.phi.cpu().numpy(), g)
 
 
+@pytest.mark.parametrize("n", [105, 112, 119, 128])
+def test_latency_form_above_104_points(eng, oracle, n):
+    """The latency form serves 105..128 points (the throughput form stops at
+    104): each integration gets the oracle's bits and the packed kernel's."""
+    ctx = eng.context()
+    assert ctx.lib.dh_resident_rows(n) == 0
+    assert ctx.lib.dh_resident_latency_rows(n) == (n + 7) // 8 * 8
+    rng = np.random.default_rng(n)
+    jobs, refs, phi0 = _jobs(eng, oracle, rng, n, 6, frozen_every=3)
+    pb = eng.PackedBatch(ctx, jobs)
+    assert pb.resident_ok()
+    pb.launch(resident=True)
+    got = [j.phi.cpu().numpy() for j in jobs]
+    for g, r in zip(got, refs):
+        assert np.array_equal(g, r)
+    for j in jobs:
+        j.phi.copy_(ctx.to_dev(phi0))
+    pb.restart()
+    pb.launch(resident=False)
+    for j, g in zip(jobs, got):
+        assert np.array_equal(j.phi.cpu().numpy(), g)
+
+
 @pytest.mark.parametrize("latency", [False, True])
 def test_resident_time_varying_and_quanta(eng, oracle, latency):
     ctx = eng.context()
@@ -104,7 +127,8 @@ def test_resident_time_varying_and_quanta(eng, oracle, latency):
 
 
 def test_resident_sizes():
-    """The grid sizes with a kernel: 33 to 104 points."""
+    """The grid sizes with a kernel: 33 to 104 points for the throughput
+    form, 33 to 128 for the latency form, in the same row classes."""
     import dadi_amd
     assert dadi_amd.cuda_enabled(True)
     from dadi_amd import engine
@@ -112,6 +136,9 @@ def test_resident_sizes():
     have = [n for n in range(3, 200) if ctx.lib.dh_resident_rows(n)]
     assert have == list(range(33, 105))
     assert {ctx.lib.dh_resident_rows(n) for n in have} == set(range(40, 105, 8))
+    lat = [n for n in range(3, 200) if ctx.lib.dh_resident_latency_rows(n)]
+    assert lat == list(range(33, 129))
+    assert all(ctx.lib.dh_resident_latency_rows(n) == ctx.lib.dh_resident_rows(n) for n in have)
 
 
 @pytest.mark.parametrize("latency", [False, True])
