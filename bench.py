@@ -240,7 +240,11 @@ def kernel_of(pb, packed=False):
     register-resident kernel where it applies, else the factored kernel for
     few integrations, else the packed one; packed=True: the packed kernel."""
     if not packed and pb.resident_ok():
-        return "k_integrate_rr<%d, 1, true>" % pb.ctx.lib.dh_resident_rows(pb.nmax)
+        ctx = pb.ctx
+        rows = ctx.lib.dh_resident_rows(pb.nmax)
+        if len(pb.jobs) <= ctx.resident_latency_max(pb.nmax) or not rows:
+            return "k_integrate_rf<%d, 1, true>" % ctx.lib.dh_resident_latency_rows(pb.nmax)
+        return "k_integrate_rr<%d, 1, true>" % rows
     if not packed and pb.latency_bound():
         return "k_integrate_fz<2, 1>"
     return "k_integrate_packed<2, 1>"
@@ -360,7 +364,7 @@ def main():
 
     def launch():
         if resident:
-            pb.launch_resident(groups=args.groups, latency=False)
+            pb.launch_resident(groups=args.groups)
         else:
             launch_packed()
 

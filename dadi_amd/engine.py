@@ -64,11 +64,28 @@ class Context:
     #: 2D batches whose grid size has a register-resident kernel use it when
     #: no other launch shape is asked for; False keeps them on the others
     resident_auto = True
-    #: register-resident batches of at most this many integrations take the
-    #: kernel's latency form (dh_integrate_resident_latency: the Thomas
-    #: factorisation kept per segment, a step runs only the right-hand-side
-    #: recurrences), larger ones its throughput form
-    resident_latency_max_inst = 64
+    #: register-resident batches take the kernel's latency form
+    #: (dh_integrate_resident_latency: the Thomas factorisation kept per
+    #: segment, a step runs only the right-hand-side recurrences) up to this
+    #: many integrations per workgroup the latency form keeps resident, the
+    #: throughput form above.  Measured on MI355X with split_mig draws
+    #: (latency / throughput form, integrations/s): n = 100, 32..512
+    #: integrations 1.34..1.08x, 1024 0.85x; n = 60, 32..1024 1.35..1.06x.
+    resident_latency_per_group = 2
+
+    def resident_latency_max(self, nmax):
+        """Largest batch on the latency form: resident_latency_per_group
+        integrations per workgroup it keeps resident (one per compute unit
+        when its n x (n+1) LDS tile exceeds half the LDS, else two)."""
+        return self.resident_latency_per_group * self.resident_latency_groups(nmax)
+
+    def resident_latency_groups(self, nmax):
+        """Workgroups of the latency form resident at once: one or two per
+        compute unit (its n x (n+1) LDS tile), 0 without a kernel."""
+        rows = int(self.lib.dh_resident_latency_rows(int(nmax)))
+        if rows == 0:
+            return 0
+        return max(1, min(2, self.max_lds // (8 * rows * (rows + 1)))) * self.num_cu
 
     def one_d_wave(self, ninst, nmax):
         """Whether a 1D batch runs one wave per integration (LDS-resident,
@@ -104,13 +121,13 @@ class Context:
     def resident_launch(self, tab_ptr, ninst, order, nmax, counter_ptr, stream, workspace,
                         groups=0, max_steps=0, max_us=0, latency=None):
         """One register-resident 2D launch: the latency form
-        (dh_integrate_resident_latency, one workgroup per compute unit, its
-        factorisation workspace kept in ``workspace``, a dict owned by the
+        (dh_integrate_resident_latency, one or two workgroups per compute
+        unit, its factorisation workspace kept in ``workspace``, a dict owned by the
         caller, one per stream) when ``latency`` is True or, for None, the
-        batch has at most resident_latency_max_inst integrations; else
+        batch has at most resident_latency_max(nmax) integrations; else
         dh_integrate_resident."""
         if latency is None:
-            latency = ninst <= self.resident_latency_max_inst
+            latency = ninst <= self.resident_latency_max(nmax)
         if not self.lib.dh_resident_rows(int(nmax)):
             latency = True    # above 104 points only the latency form exists
         if not latency:
@@ -120,7 +137,7 @@ class Context:
                                                  stream), "dh_integrate_resident")
             return
         slice_d = int(self.lib.dh_resident_latency_slice_doubles(int(nmax)))
-        g = groups if groups > 0 else min(ninst, self.num_cu)
+        g = groups if groups > 0 else min(ninst, self.resident_latency_groups(nmax))
         need = g * slice_d
         ws = workspace.get("resident")
         if ws is None or ws.numel() < need:
