@@ -150,6 +150,14 @@ struct DeljTabs {
 #define DH_WPE 3
 #endif
 constexpr int PF = DH_PF;
+// Rows per block of the multi-workgroup sweeps (k_sweep_axis, k_sweep_batch):
+// one wave per SIMD there, so nothing hides a row's latency but the rows of
+// a block interleaving (3-population sweeps on 40-64 points, MI355X: 8 rows
+// 11-14% faster per step than 4)
+#ifndef DH_SWEEP_PF
+#define DH_SWEEP_PF 8
+#endif
+constexpr int SWEEP_PF = DH_SWEEP_PF;
 constexpr int STAGE_C = DH_STAGE_C;
 
 // Coefficients a_j, b_j and c_{j-1} of row j >= 1 (compute_abc_nobc,
@@ -297,7 +305,7 @@ __device__ __forceinline__ void regamma(int cb, int n, double bt,
 // One line: inject (optional), build a/b/c on the fly, Thomas forward with u'
 // over phi in place and gamma_j in `gam`, then back substitution.
 //   ts: stride of the 1D tables; gs: stride of the gamma column.
-template <int NO, int ORDER, int DJ = 0>
+template <int NO, int ORDER, int DJ = 0, int BP = PF>
 __device__ __forceinline__ void solve_line(
     double *__restrict__ p, long long s, int n,
     const double *__restrict__ xInt, const double *__restrict__ df,
@@ -311,10 +319,10 @@ __device__ __forceinline__ void solve_line(
 {
     if (inj_row >= 0) p[inj_row * s] = p[inj_row * s] + inj;
 
-    // rows 1..PF in flight while row 0 is formed
-    double nxt[PF];
+    // rows 1..BP in flight while row 0 is formed
+    double nxt[BP];
 #pragma unroll
-    for (int q = 0; q < PF; ++q) {
+    for (int q = 0; q < BP; ++q) {
         const int jj = 1 + q;
         nxt[q] = p[(jj < n ? jj : n - 1) * s];
     }
@@ -325,56 +333,77 @@ __device__ __forceinline__ void solve_line(
                         pc, dj);
     p[0] = u;
 
-    // forward elimination, PF rows per block, the next block's rows loading
-    for (int j0 = 1; j0 < n; j0 += PF) {
-        double cur[PF];
+    // forward elimination, BP rows per block, the next block's rows loading.
+    // A whole block has no per-row branch, so one basic block holds its rows
+    // and the compiler interleaves row j's u division with row j+1's
+    // coefficients and gamma division (waves issue in order: with a branch
+    // per row each row's two divisions ran back to back)
+    for (int j0 = 1; j0 < n; j0 += BP) {
+        double cur[BP];
 #pragma unroll
-        for (int q = 0; q < PF; ++q) cur[q] = nxt[q];
+        for (int q = 0; q < BP; ++q) cur[q] = nxt[q];
 #pragma unroll
-        for (int q = 0; q < PF; ++q) {
-            const int jj = j0 + PF + q;
+        for (int q = 0; q < BP; ++q) {
+            const int jj = j0 + BP + q;
             nxt[q] = p[(jj < n ? jj : n - 1) * s];
         }
+        if (j0 + BP <= n) {
 #pragma unroll
-        for (int q = 0; q < PF; ++q) {
-            const int j = j0 + q;
-            if (j < n)
-                fwd_row<NO, ORDER, DJ>(j, n, cur[q], xInt, df, W0, W1, S, ts, c, m, dt, inv_dt,
-                                       bcl_on, bcl, bet, u, pa, pc, p, s, gam, gs, dj);
+            for (int q = 0; q < BP; ++q)
+                fwd_row<NO, ORDER, DJ>(j0 + q, n, cur[q], xInt, df, W0, W1, S, ts, c, m, dt,
+                                       inv_dt, bcl_on, bcl, bet, u, pa, pc, p, s, gam, gs, dj);
+        } else {
+#pragma unroll
+            for (int q = 0; q < BP; ++q) {
+                const int j = j0 + q;
+                if (j < n)
+                    fwd_row<NO, ORDER, DJ>(j, n, cur[q], xInt, df, W0, W1, S, ts, c, m, dt,
+                                           inv_dt, bcl_on, bcl, bet, u, pa, pc, p, s, gam, gs,
+                                           dj);
+            }
         }
     }
 
     // back substitution (dadi/tridiag.c:31-33), rows and gammas loaded a
     // block ahead
-    double bp[PF], bg[PF];
+    double bp[BP], bg[BP];
 #pragma unroll
-    for (int q = 0; q < PF; ++q) {
+    for (int q = 0; q < BP; ++q) {
         const int jj = n - 2 - q;
         const int jc = jj >= 0 ? jj : 0;
         bp[q] = p[jc * s];
         bg[q] = gam[(jc + 1) * gs];
     }
-    for (int j0 = n - 2; j0 >= 0; j0 -= PF) {
-        double cp[PF], cg[PF];
+    for (int j0 = n - 2; j0 >= 0; j0 -= BP) {
+        double cp[BP], cg[BP];
 #pragma unroll
-        for (int q = 0; q < PF; ++q) {
+        for (int q = 0; q < BP; ++q) {
             cp[q] = bp[q];
             cg[q] = bg[q];
         }
 #pragma unroll
-        for (int q = 0; q < PF; ++q) {
-            const int jj = j0 - PF - q;
+        for (int q = 0; q < BP; ++q) {
+            const int jj = j0 - BP - q;
             const int jc = jj >= 0 ? jj : 0;
             bp[q] = p[jc * s];
             bg[q] = gam[(jc + 1) * gs];
         }
+        if (j0 + 1 >= BP) {
 #pragma unroll
-        for (int q = 0; q < PF; ++q) {
-            const int j = j0 - q;
-            if (j >= 0) {
+            for (int q = 0; q < BP; ++q) {
                 const double v = cp[q] - cg[q] * u;
-                p[j * s] = v;
+                p[(j0 - q) * s] = v;
                 u = v;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < BP; ++q) {
+                const int j = j0 - q;
+                if (j >= 0) {
+                    const double v = cp[q] - cg[q] * u;
+                    p[j * s] = v;
+                    u = v;
+                }
             }
         }
     }
@@ -1117,7 +1146,7 @@ __device__ __forceinline__ void line_addr(int n, long long ld, int a, long long 
 
 // All lines of axis `a`, tiles of 64 lines per wave.  GG: gamcol is this
 // lane's gamma column in the global workspace (rows 64 doubles apart).
-template <int D, int ORDER, int CK, int RU, int GG, int DJ = 0>
+template <int D, int ORDER, int CK, int RU, int GG, int DJ = 0, int BP = PF>
 __device__ __forceinline__ void sweep_tiles(double *__restrict__ phi, int n, long long ld, int a,
                                             bool first_axis,
                                             int injm, const dh_segment *sp,
@@ -1175,7 +1204,7 @@ __device__ __forceinline__ void sweep_tiles(double *__restrict__ phi, int n, lon
             line_addr<D>(n, ld, a, line, off, s);
             double *p = phi + off;
             if constexpr (CK == 0)
-                solve_line<NO, ORDER, DJ>(p, s, n, sxi, sdf, W0, W1, S, 1, L.c, mig, dt, inv_dt,
+                solve_line<NO, ORDER, DJ, BP>(p, s, n, sxi, sdf, W0, W1, S, 1, L.c, mig, dt, inv_dt,
                                           L.bcf_on, L.bcf, L.bcl_on, L.bcl, L.inj_row, inj,
                                           gamcol, 64, dj);
             else if constexpr (RU)
@@ -3220,7 +3249,7 @@ __device__ __forceinline__ double fz_back_rows(const double (&cu)[FZ_PF],
 template <int D, int ORDER, bool MK>
 __device__ void fz_sweep_axis(double *__restrict__ phi, int n, long long ld, long long L, int a,
                               bool first_axis, int injm, const dh_segment *sp, const double *sx,
-                              const double *sdx, double dt, double inv_dt,
+                              double dx0, double dxl, double dt, double inv_dt,
                               const double *__restrict__ Fa, double *__restrict__ Uw,
                               long long tile0, long long tstep, int lane)
 {
@@ -3236,8 +3265,8 @@ __device__ void fz_sweep_axis(double *__restrict__ phi, int n, long long ld, lon
         const double *G = Fa + 3 * N + line;
         double *U = Uw + line;
         LineSetup<D, ORDER> Ls;
-        line_setup<D, ORDER>(Ls, line, n, a, sx, mig, sp->nu[a], sp->gamma[a], sp->h[a], sdx[0],
-                             sdx[n - 2], first_axis, injm, sp->chrom);
+        line_setup<D, ORDER>(Ls, line, n, a, sx, mig, sp->nu[a], sp->gamma[a], sp->h[a], dx0,
+                             dxl, first_axis, injm, sp->chrom);
         long long off, s;
         line_addr<D>(n, ld, a, line, off, s);
         double *p = phi + off;
@@ -3357,13 +3386,13 @@ __global__ void __launch_bounds__(512) k_integrate_fz(dh_instance *__restrict__ 
                 for (int a = 0; a < D; ++a) {
                     if ((frozen >> a) & 1) continue;
                     if (mk)
-                        fz_sweep_axis<D, ORDER, true>(phi, n, ld, L, a, first, injm, sp, sx, sdx,
-                                                      dt, inv_dt, F + 4LL * a * N, U, wave, nw,
-                                                      lane);
+                        fz_sweep_axis<D, ORDER, true>(phi, n, ld, L, a, first, injm, sp, sx,
+                                                      sdx[0], sdx[n - 2], dt, inv_dt,
+                                                      F + 4LL * a * N, U, wave, nw, lane);
                     else
                         fz_sweep_axis<D, ORDER, false>(phi, n, ld, L, a, first, injm, sp, sx,
-                                                       sdx, dt, inv_dt, F + 4LL * a * N, U, wave,
-                                                       nw, lane);
+                                                       sdx[0], sdx[n - 2], dt, inv_dt,
+                                                       F + 4LL * a * N, U, wave, nw, lane);
                     __syncthreads();
                     first = false;
                 }
@@ -3417,10 +3446,11 @@ __global__ void __launch_bounds__(256) k_sweep_axis(dh_instance inst, int a, int
     build_axis_tables<D>(sp, a, n, sx, sxi, sdx, W0, W1, S, tid, nthr);
     __syncthreads();
     const double dt = sp->dt, inv_dt = 1. / dt;
-    sweep_tiles<D, ORDER, 0, 0, 0>(gbl(inst.phi_dev), n, inst.ld > 0 ? inst.ld : n, a, first_axis != 0, inst.inject, sp, sx, sxi, sdf,
-                          sdx, W0, W1, S, dt, inv_dt, gamcol, nullptr,
-                          (long long)blockIdx.x * nw + wave,
-                          (long long)gridDim.x * nw, lane);
+    sweep_tiles<D, ORDER, 0, 0, 0, 0, SWEEP_PF>(gbl(inst.phi_dev), n, inst.ld > 0 ? inst.ld : n, a,
+                                   first_axis != 0, inst.inject, sp, sx, sxi, sdf, sdx, W0, W1,
+                                   S, dt, inv_dt, gamcol, nullptr,
+                                   (long long)blockIdx.x * nw + wave,
+                                   (long long)gridDim.x * nw, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -3448,14 +3478,14 @@ __global__ void __launch_bounds__(256) k_sweep_batch(const dh_sweep_item *__rest
     const dh_segment *sp = gbl(I.seg_dev) + gbl(it.seg_of_step_dev)[st];
     double *sx = lds, *sxi = sx + n, *sdf = sxi + n, *sdx = sdf + n;
     double *W0 = sdx + n, *W1 = W0 + n, *S = W1 + n;
-    double *gamcol = S + n + (long long)wave * 64 * col + lane;
-    double *stage = S + n + (long long)nw * 64 * col + (long long)wave * 64 * (STAGE_C + 1);
     build_grid_tables(gbl(I.xx_dev), n, sx, sxi, sdf, sdx, tid, nthr);
     __syncthreads();
     build_axis_tables<D>(sp, a, n, sx, sxi, sdx, W0, W1, S, tid, nthr);
     __syncthreads();
     const double dt = sp->dt, inv_dt = 1. / dt;
-    sweep_tiles<D, ORDER, CK, 0, 0>(gbl(I.phi_dev), n, I.ld > 0 ? I.ld : n, a,
+    double *gamcol = S + n + (long long)wave * 64 * col + lane;
+    double *stage = S + n + (long long)nw * 64 * col + (long long)wave * 64 * (STAGE_C + 1);
+    sweep_tiles<D, ORDER, CK, 0, 0, 0, SWEEP_PF>(gbl(I.phi_dev), n, I.ld > 0 ? I.ld : n, a,
                                     a == it.first_axis, I.inject, sp, sx, sxi, sdf, sdx, W0, W1,
                                     S, dt, inv_dt, gamcol, stage,
                                     (long long)(blockIdx.x - it.wg_begin) * nw + wave,
@@ -3947,6 +3977,55 @@ int static_scratch(double **out)
         g_scratch_dev[dev] = static_cast<double *>(p);
     }
     *out = g_scratch_dev[dev];
+    return DH_OK;
+}
+
+}  // namespace
+
+namespace {
+
+// Steps first_step .. first_step+nsteps-1 (capped at max_total_steps) of a
+// batch: nsteps_per_graph steps of sweep launches (launch(k, axis, stream))
+// plus the step counter's advance, captured once per key into a cached
+// hipGraph and replayed.
+template <class Launch>
+int batch_steps(const std::string &key, int D, int nsteps_per_graph, long long *step_base_dev,
+                long long max_total_steps, long long first_step, long long nsteps, void *stream,
+                Launch launch)
+{
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    GraphRef entry = graph_lookup(key);
+    if (!entry) {
+        hipGraphExec_t exec = nullptr;
+        hipStream_t cap;
+        DH_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+        DH_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+        hipError_t el = hipSuccess;
+        for (int k = 0; k < nsteps_per_graph && el == hipSuccess; ++k)
+            for (int a = 0; a < D && el == hipSuccess; ++a) el = launch(k, a, cap);
+        if (el == hipSuccess)
+            hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, cap, step_base_dev,
+                               (long long)nsteps_per_graph);
+        hipGraph_t graph = nullptr;
+        hipError_t ec = hipStreamEndCapture(cap, &graph);
+        (void)hipStreamDestroy(cap);
+        if (el != hipSuccess) ec = el;
+        if (ec != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return fail(DH_EHIP, std::string("capture: ") + hipGetErrorString(ec));
+        }
+        hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess)
+            return fail(DH_EHIP, std::string("instantiate: ") + hipGetErrorString(ei));
+        entry = graph_insert(key, exec);
+    }
+    hipLaunchKernelGGL(k_set_ll, dim3(1), dim3(1), 0, st, step_base_dev, first_step);
+    const long long last =
+        first_step + nsteps < max_total_steps ? first_step + nsteps : max_total_steps;
+    const long long reps = (last - first_step + nsteps_per_graph - 1) / nsteps_per_graph;
+    for (long long r = 0; r < reps; ++r) DH_HIP(hipGraphLaunch(entry->exec, st));
+    DH_HIP(entry->note_launch(st));
     return DH_OK;
 }
 
@@ -4466,54 +4545,21 @@ int dh_integrate_batch(const dh_sweep_item *items_dev, const int *wg_item_dev, i
     const size_t lds = ck ? sizeof(double) * (tables + (size_t)waves * 64 * (col + STAGE_C + 1))
                           : lds0;
     if (int e = set_lds(fn, lds)) return e;
-    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     std::string key("batch");
     key.append((const char *)&items_dev, sizeof(items_dev));
     key.append((const char *)&wg_item_dev, sizeof(wg_item_dev));
     key.append((const char *)&step_base_dev, sizeof(step_base_dev));
     const int ints[] = {ninst, total_wgs, D, order, nmax, waves, nsteps_per_graph};
     key.append((const char *)ints, sizeof(ints));
-    GraphRef entry = graph_lookup(key);
-    if (!entry) {
-        hipGraphExec_t exec = nullptr;
-        hipStream_t cap;
-        DH_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
-        DH_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-        hipError_t el = hipSuccess;
-        for (int k = 0; k < nsteps_per_graph && el == hipSuccess; ++k) {
-            for (int a = 0; a < D && el == hipSuccess; ++a) {
-                long long off = k;
-                int ax = a;
-                void *args[] = {(void *)&items_dev, (void *)&wg_item_dev, &ax,
-                                (void *)&step_base_dev, &off};
-                el = hipLaunchKernel(fn, dim3((unsigned)total_wgs), dim3(64 * waves), args, lds,
-                                     cap);
-            }
-        }
-        if (el == hipSuccess)
-            hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, cap, step_base_dev,
-                               (long long)nsteps_per_graph);
-        hipGraph_t graph = nullptr;
-        hipError_t ec = hipStreamEndCapture(cap, &graph);
-        (void)hipStreamDestroy(cap);
-        if (el != hipSuccess) ec = el;
-        if (ec != hipSuccess) {
-            if (graph) (void)hipGraphDestroy(graph);
-            return fail(DH_EHIP, std::string("capture: ") + hipGetErrorString(ec));
-        }
-        hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        if (ei != hipSuccess)
-            return fail(DH_EHIP, std::string("instantiate: ") + hipGetErrorString(ei));
-        entry = graph_insert(key, exec);
-    }
-    hipLaunchKernelGGL(k_set_ll, dim3(1), dim3(1), 0, st, step_base_dev, first_step);
-    const long long last =
-        first_step + nsteps < max_total_steps ? first_step + nsteps : max_total_steps;
-    const long long reps = (last - first_step + nsteps_per_graph - 1) / nsteps_per_graph;
-    for (long long r = 0; r < reps; ++r) DH_HIP(hipGraphLaunch(entry->exec, st));
-    DH_HIP(entry->note_launch(st));
-    return DH_OK;
+    return batch_steps(key, D, nsteps_per_graph, step_base_dev, max_total_steps, first_step,
+                       nsteps, stream, [&](int k, int a, hipStream_t cap) {
+                           long long off = k;
+                           int ax = a;
+                           void *args[] = {(void *)&items_dev, (void *)&wg_item_dev, &ax,
+                                           (void *)&step_base_dev, &off};
+                           return hipLaunchKernel(fn, dim3((unsigned)total_wgs),
+                                                  dim3(64 * waves), args, lds, cap);
+                       });
 }
 
 int dh_sample_axis(const double *in_dev, long long P, int n, const double *xx_dev, int m,

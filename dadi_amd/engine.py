@@ -801,7 +801,17 @@ def run_batch(jobs, ctx=None, waves=None, groups=0, sort=True, chunk=None, recom
     explicit = any(v is not None for v in (waves, chunk, recompute, gamma_global, slots))
     for key in sorted(groups_by):
         js = groups_by[key]
-        D, _, delj = key
+        D, order, delj = key
+        rows = ctx.lib.dh_resident_latency_rows
+        if (D == 2 and not explicit and not delj and ctx.resident_auto and
+                order in (_lib.ORDER_TV, _lib.ORDER_PRE) and all(rows(j.n) for j in js)):
+            # the register-resident kernels take mixed grids of one row
+            # bucket in a launch: one part per bucket, side by side
+            by = {}
+            for j in js:
+                by.setdefault(rows(j.n), []).append(j)
+            parts.extend(by[k] for k in sorted(by))
+            continue
         # the latency paths take mixed grids in one launch: one launch, not
         # one per grid size on streams that may share a hardware queue
         one_launch = not explicit and not delj and (
