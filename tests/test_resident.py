@@ -86,15 +86,18 @@ def test_resident_equals_packed_and_oracle(eng, oracle, n, latency):
         assert np.array_equal(j.phi.cpu().numpy(), g)
 
 
-@pytest.mark.parametrize("n", [105, 112, 119, 128])
-def test_latency_form_above_104_points(eng, oracle, n):
+@pytest.mark.parametrize("n,tv", [(105, False), (112, False), (119, False), (128, False),
+                                  (112, True), (128, True)])
+def test_latency_form_above_104_points(eng, oracle, n, tv):
     """The latency form serves 105..128 points (the throughput form stops at
-    104): each integration gets the oracle's bits and the packed kernel's."""
+    104), both coefficient orders: each integration gets the oracle's bits
+    and the packed kernel's."""
     ctx = eng.context()
     assert ctx.lib.dh_resident_rows(n) == 0
     assert ctx.lib.dh_resident_latency_rows(n) == (n + 7) // 8 * 8
-    rng = np.random.default_rng(n)
-    jobs, refs, phi0 = _jobs(eng, oracle, rng, n, 6, frozen_every=3)
+    rng = np.random.default_rng(n + tv)
+    jobs, refs, phi0 = _jobs(eng, oracle, rng, n, 6, tv=tv, frozen_every=3,
+                             tmax=0.004 if tv else 0.02)
     pb = eng.PackedBatch(ctx, jobs)
     assert pb.resident_ok()
     pb.launch(resident=True)
@@ -279,8 +282,9 @@ def test_resident_tiny_values_go_to_exact_kernel(eng, oracle, tv, latency):
 def test_resident_out_of_range_phi_goes_to_exact_kernel(eng, oracle, latency):
     """An instance whose phi holds a value beyond 2^600 when it is taken up
     runs on the exactly dividing kernel (the fast kernel's checks assume
-    finite recurrences); its neighbours stay on the fast kernel; every result
-    is the oracle's."""
+    finite recurrences), and so does one whose value near 2^531 drives a
+    numerator above 2^500 at its first step; their neighbours stay on the
+    fast kernel; every result is the oracle's."""
     ctx = eng.context()
     rng = np.random.default_rng(31)
     jobs, refs, phi0 = _jobs(eng, oracle, rng, 80, 4)
@@ -296,10 +300,17 @@ def test_resident_out_of_range_phi_goes_to_exact_kernel(eng, oracle, latency):
     j.phi.copy_(ctx.to_dev(big))
     refs[2] = oracle.integrate_const(big, xx, 0.002, [nu1, nu2], [[0.3], [0.3]], [0, 0],
                                      [0.5, 0.5])
+    large = phi0.copy()
+    large[40, 41] = 1e160
+    assert 2.0 ** 500 < 1e160 < 2.0 ** 600
+    jobs[3].seg = j.seg
+    jobs[3].phi.copy_(ctx.to_dev(large))
+    refs[3] = oracle.integrate_const(large, xx, 0.002, [nu1, nu2], [[0.3], [0.3]], [0, 0],
+                                     [0.5, 0.5])
     pb = eng.PackedBatch(ctx, jobs)
     pb.launch(resident=True, latency=latency)
     tab = pb.tab.cpu().numpy().view(eng.INST_DTYPE)
     assert list(tab["status"]) == [1] * 4
-    assert list(tab["reserved"] & 1) == [0, 0, 1, 0]
+    assert list(tab["reserved"] & 1) == [0, 0, 1, 1]
     for jb, r in zip(jobs, refs):
         assert np.array_equal(jb.phi.cpu().numpy(), r)
