@@ -2320,9 +2320,12 @@ constexpr size_t rf_lds_doubles()
 }
 
 // Workspace doubles of one workgroup: per axis and row, (a_j, bet_j) and
-// (y_j, gamma_{j+1}) for every line.
+// (y_j, gamma_{j+1}) for every line, then a checkpoint of phi (NP x NP, row
+// j of line L at j * NP + L).
 template <int NP>
-constexpr long long rf_slice_doubles() { return 2LL * 2 * NP * NP * 2; }
+constexpr long long rf_records_doubles() { return 2LL * 2 * NP * NP * 2; }
+template <int NP>
+constexpr long long rf_slice_doubles() { return rf_records_doubles<NP>() + (long long)NP * NP; }
 
 // The factorisation of line `line` for one axis, stored (if `store`) as the
 // records of rows 0..n-1 at AB[j * NP + line] / YG[j * NP + line]; returns
@@ -2497,6 +2500,153 @@ __device__ __forceinline__ void rf_transpose(RrLine<NP, rf_vrows<NP>()> &P, int 
     __syncthreads();
 }
 
+// Where rf_run starts: phi rows at src[i * src_ld + L] (the instance's phi,
+// or the checkpoint), progress (seg, step), at most `budget` steps.
+struct RfStart {
+    const double *src;
+    long long src_ld, seg, step, budget;
+};
+
+// One instance from `at` to the end of its schedule or of the budget; phi
+// and the progress are stored at the end.  FAST: every 32 steps whose checks
+// pass, phi goes to the checkpoint (ckp) and `at` records it; when a check
+// fails, false is returned with nothing stored (the steps since the last
+// checkpoint are discarded).  The exact form always completes.
+template <int NP, int ORDER, bool FAST>
+__device__ __forceinline__ bool rf_run(dh_instance *ip, int n, RfStart &at, long long max_ticks,
+                                       double2 *ws, double *ckp, double *lds, int *s_stop)
+{
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int L = tid;
+    // the tile; at a segment's start the coefficient tables (the same
+    // interleaved layout as k_integrate_rr) in its place
+    double *tile = lds;
+    double *sx = lds, *sdx = sx + NP;
+    double *blk0 = sdx + NP, *blk1 = blk0 + 5 * NP;
+    long long seg = at.seg, step = at.step;
+    const long long nseg = ip->nseg;
+    long long budget = at.budget;
+    const int frozen = ip->frozen, injm = ip->inject;
+    int first_axis = 0;
+    while (first_axis < 2 && ((frozen >> first_axis) & 1)) ++first_axis;
+    double *phi = gbl(ip->phi_dev);
+    const long long ld = ip->ld > 0 ? ip->ld : n;
+    RrLine<NP, rf_vrows<NP>()> P;
+    if (L < n) {
+        const double *src = at.src;
+#pragma unroll
+        for (int i = 0; i < NP; ++i)
+            if (i < n) P.set(i, src[i * at.src_ld + L]);
+    }
+    asm volatile("" ::: "memory");
+    const long long t0 = max_ticks > 0 ? (long long)wall_clock64() : 0;
+    bool out_of_time = false, failed = false;
+    unsigned long long okm =
+        __builtin_amdgcn_ballot_w64(!FAST || L >= n || rr_phi_in_range<NP>(P, n));
+    unsigned long long bad = 0;
+    if (first_axis == 2) seg = nseg;
+    while (seg < nseg && budget > 0 && !out_of_time && !failed) {
+        const dh_segment *sp = gbl(ip->seg_dev) + seg;
+        // the tile is free between steps: the tables go there
+        __syncthreads();
+        rr_grid_tables(gbl(ip->xx_dev), n, sx, sdx, blk0, tid, nthr);
+        __syncthreads();
+        for (int i = tid; i < n; i += nthr) {
+            blk1[i * RR_TS] = blk0[i * RR_TS];
+            blk1[i * RR_TS + 1] = blk0[i * RR_TS + 1];
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            if ((frozen >> a) & 1) continue;
+            rr_axis_tables(sp, a, n, sx, sdx, a ? blk1 : blk0, tid, nthr);
+        }
+        __syncthreads();
+        const double dt = sp->dt, inv_dt = 1. / dt, inj = sp->inject;
+        const double y_dt = rr_recip(dt);
+        const int line = L < n ? L : 0;
+        bool in = rr_pos(dt) & (__builtin_fabs(inj) <= RR_PHI_HI);
+        int inj_row[2] = {-1, -1};
+#pragma unroll 1
+        for (int a = 0; a < 2; ++a) {
+            if ((frozen >> a) & 1) continue;
+            LineSetup<2, ORDER> Ls;
+            const double mg[1] = {sp->mig[a][0]};
+            line_setup<2, ORDER>(Ls, line, n, a, sx, mg, sp->nu[a], sp->gamma[a], sp->h[a],
+                                 sdx[0], sdx[n - 2], first_axis == a, injm, sp->chrom);
+            inj_row[a] = Ls.inj_row;
+            double2 *AB = ws + (long long)a * 2 * NP * NP, *YG = AB + (long long)NP * NP;
+            // lanes past the last line form line 0's values and store nothing
+            in = in & rf_factor<NP, ORDER, FAST>(AB, YG, n, line, L < n, a ? blk1 : blk0,
+                                                 Ls.c[0], mg[0], inv_dt, Ls.bcf_on, Ls.bcf,
+                                                 Ls.bcl_on, Ls.bcl);
+        }
+        okm &= __builtin_amdgcn_ballot_w64(!FAST || in || L >= n);
+        // the tables are read: the transposes may use the tile again
+        __syncthreads();
+        long long nst = sp->nsteps - step;
+        if (nst > budget) nst = budget;
+        for (long long k = 0; k < nst; ++k) {
+#pragma unroll 1
+            for (int a = 0; a < 2; ++a) {
+                if (!((frozen >> a) & 1)) {
+                    const double2 *AB = ws + (long long)a * 2 * NP * NP;
+                    rf_sweep<NP, ORDER, FAST>(P, AB, AB + (long long)NP * NP, n, line, dt,
+                                              inv_dt, y_dt, inj_row[a], inj, bad);
+                }
+                rf_transpose<NP>(P, n, L, tile);
+            }
+            if (FAST && (k & 31) == 31) {
+                if (__syncthreads_or(rr_wave_out(okm, bad, L, n))) {
+                    nst = k + 1;
+                    failed = true;
+                    break;
+                }
+                // steps checked: the checkpoint moves here
+                if (L < n) {
+#pragma unroll
+                    for (int i = 0; i < NP; ++i)
+                        if (i < n) ckp[i * NP + L] = P.get(i);
+                }
+                at = RfStart{ckp, NP, seg, step + k + 1, budget - (k + 1)};
+            }
+            if (max_ticks > 0) {
+                if (tid == 0) *s_stop = (long long)wall_clock64() - t0 >= max_ticks;
+                __syncthreads();
+                if (*s_stop) {
+                    nst = k + 1;
+                    out_of_time = true;
+                    break;
+                }
+            }
+        }
+        budget -= nst;
+        step += nst;
+        if (step >= sp->nsteps) {
+            ++seg;
+            step = 0;
+        }
+    }
+    if (FAST && (failed || __syncthreads_or(rr_wave_out(okm, bad, L, n)))) return false;
+    if (L < n) {
+#pragma unroll
+        for (int i = 0; i < NP; ++i)
+            if (i < n) phi[i * ld + L] = P.get(i);
+    }
+    if (tid == 0) {
+        ip->cur_seg = seg;
+        ip->cur_step = step;
+        ip->status = seg >= nseg ? 1 : 0;
+    }
+    return true;
+}
+
+// FAST: an instance whose checks fail is continued at once by the same
+// workgroup with exact divisions (rf_run<.., false>: the factorisation's
+// gamma and the quotients by IEEE division) from its last checkpoint (every
+// 32 checked steps; else from where it was taken up), so a batch with a few
+// such instances neither waits for a second launch nor redoes the steps
+// already checked.  Not FAST: only the instances marked for exact
+// division (by the throughput form's fast kernel).
 template <int NP, int ORDER, bool FAST>
 __global__ void __launch_bounds__(64 * rr_waves<NP>()) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_integrate_rf(dh_instance *__restrict__ tab, int ninst, int *__restrict__ counter,
@@ -2505,14 +2655,9 @@ k_integrate_rf(dh_instance *__restrict__ tab, int ninst, int *__restrict__ count
 {
     extern __shared__ double lds[];
     __shared__ int s_next, s_stop;
-    const int tid = threadIdx.x, nthr = blockDim.x;
-    const int L = tid;
-    // the tile; at a segment's start the coefficient tables (the same
-    // interleaved layout as k_integrate_rr) in its place
-    double *tile = lds;
-    double *sx = lds, *sdx = sx + NP;
-    double *blk0 = sdx + NP, *blk1 = blk0 + 5 * NP;
+    const int tid = threadIdx.x;
     double2 *ws = reinterpret_cast<double2 *>(ws_all + (long long)blockIdx.x * slice);
+    double *ckp = ws_all + (long long)blockIdx.x * slice + rf_records_doubles<NP>();
     for (;;) {
         __syncthreads();
         if (tid == 0) s_next = atomicAdd(counter, 1);
@@ -2526,117 +2671,13 @@ k_integrate_rf(dh_instance *__restrict__ tab, int ninst, int *__restrict__ count
             continue;
         }
         __builtin_assume(n > NP - RR_SPAN && n <= NP);
-        // the fast form leaves instances marked for exact division to the
-        // exact form, which runs only those
         if (FAST == ((__builtin_amdgcn_readfirstlane(ip->reserved) & 1) != 0)) continue;
-        long long seg = ip->cur_seg, step = ip->cur_step;
-        const long long nseg = ip->nseg;
-        long long budget = max_steps > 0 ? max_steps : LLONG_MAX;
-        const int frozen = ip->frozen, injm = ip->inject;
-        int first_axis = 0;
-        while (first_axis < 2 && ((frozen >> first_axis) & 1)) ++first_axis;
-        double *phi = gbl(ip->phi_dev);
         const long long ld = ip->ld > 0 ? ip->ld : n;
-        RrLine<NP, rf_vrows<NP>()> P;
-        if (L < n) {
-#pragma unroll
-            for (int i = 0; i < NP; ++i)
-                if (i < n) P.set(i, phi[i * ld + L]);
-        }
-        asm volatile("" ::: "memory");
-        const long long t0 = max_ticks > 0 ? (long long)wall_clock64() : 0;
-        bool out_of_time = false, failed = false;
-        unsigned long long okm =
-            __builtin_amdgcn_ballot_w64(!FAST || L >= n || rr_phi_in_range<NP>(P, n));
-        unsigned long long bad = 0;
-        if (first_axis == 2) seg = nseg;
-        while (seg < nseg && budget > 0 && !out_of_time && !failed) {
-            const dh_segment *sp = gbl(ip->seg_dev) + seg;
-            // the tile is free between steps: the tables go there
-            __syncthreads();
-            rr_grid_tables(gbl(ip->xx_dev), n, sx, sdx, blk0, tid, nthr);
-            __syncthreads();
-            for (int i = tid; i < n; i += nthr) {
-                blk1[i * RR_TS] = blk0[i * RR_TS];
-                blk1[i * RR_TS + 1] = blk0[i * RR_TS + 1];
-            }
-#pragma unroll
-            for (int a = 0; a < 2; ++a) {
-                if ((frozen >> a) & 1) continue;
-                rr_axis_tables(sp, a, n, sx, sdx, a ? blk1 : blk0, tid, nthr);
-            }
-            __syncthreads();
-            const double dt = sp->dt, inv_dt = 1. / dt, inj = sp->inject;
-            const double y_dt = rr_recip(dt);
-            const int line = L < n ? L : 0;
-            bool in = rr_pos(dt) & (__builtin_fabs(inj) <= RR_PHI_HI);
-            int inj_row[2] = {-1, -1};
-#pragma unroll 1
-            for (int a = 0; a < 2; ++a) {
-                if ((frozen >> a) & 1) continue;
-                LineSetup<2, ORDER> Ls;
-                const double mg[1] = {sp->mig[a][0]};
-                line_setup<2, ORDER>(Ls, line, n, a, sx, mg, sp->nu[a], sp->gamma[a], sp->h[a],
-                                     sdx[0], sdx[n - 2], first_axis == a, injm, sp->chrom);
-                inj_row[a] = Ls.inj_row;
-                double2 *AB = ws + (long long)a * 2 * NP * NP, *YG = AB + (long long)NP * NP;
-                // lanes past the last line form line 0's values and store nothing
-                in = in & rf_factor<NP, ORDER, FAST>(AB, YG, n, line, L < n, a ? blk1 : blk0,
-                                              Ls.c[0], mg[0], inv_dt, Ls.bcf_on, Ls.bcf,
-                                              Ls.bcl_on, Ls.bcl);
-            }
-            okm &= __builtin_amdgcn_ballot_w64(!FAST || in || L >= n);
-            // the tables are read: the transposes may use the tile again
-            __syncthreads();
-            long long nst = sp->nsteps - step;
-            if (nst > budget) nst = budget;
-            for (long long k = 0; k < nst; ++k) {
-#pragma unroll 1
-                for (int a = 0; a < 2; ++a) {
-                    if (!((frozen >> a) & 1)) {
-                        const double2 *AB = ws + (long long)a * 2 * NP * NP;
-                        rf_sweep<NP, ORDER, FAST>(P, AB, AB + (long long)NP * NP, n, line, dt,
-                                                  inv_dt,
-                                            y_dt, inj_row[a], inj, bad);
-                    }
-                    rf_transpose<NP>(P, n, L, tile);
-                }
-                if (FAST && (k & 31) == 31 && __syncthreads_or(rr_wave_out(okm, bad, L, n))) {
-                    nst = k + 1;
-                    failed = true;
-                    break;
-                }
-                if (max_ticks > 0) {
-                    if (tid == 0) s_stop = (long long)wall_clock64() - t0 >= max_ticks;
-                    __syncthreads();
-                    if (s_stop) {
-                        nst = k + 1;
-                        out_of_time = true;
-                        break;
-                    }
-                }
-            }
-            budget -= nst;
-            step += nst;
-            if (step >= sp->nsteps) {
-                ++seg;
-                step = 0;
-            }
-        }
-        if (FAST && (failed || __syncthreads_or(rr_wave_out(okm, bad, L, n)))) {
-            // phi and the progress are as taken up: the exact kernel redoes them
-            if (tid == 0) ip->reserved |= 1;
-            continue;
-        }
-        if (L < n) {
-#pragma unroll
-            for (int i = 0; i < NP; ++i)
-                if (i < n) phi[i * ld + L] = P.get(i);
-        }
-        if (tid == 0) {
-            ip->cur_seg = seg;
-            ip->cur_step = step;
-            ip->status = seg >= nseg ? 1 : 0;
+        RfStart at{gbl(ip->phi_dev), ld, ip->cur_seg, ip->cur_step,
+                   max_steps > 0 ? max_steps : LLONG_MAX};
+        if (!rf_run<NP, ORDER, FAST>(ip, n, at, max_ticks, ws, ckp, lds, &s_stop)) {
+            __syncthreads();   // the checkpoint's stores before its loads
+            rf_run<NP, ORDER, false>(ip, n, at, max_ticks, ws, ckp, lds, &s_stop);
         }
     }
 }

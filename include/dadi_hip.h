@@ -289,9 +289,11 @@ int dh_resident_rows(int n);
  * points, where dh_integrate_resident has no kernel.  cfg->gamma_dev must hold
  * groups * dh_resident_latency_slice_doubles(n) doubles (groups = cfg->groups,
  * or 0: min(ninst, workgroups that fit)) and may not be shared by launches
- * that overlap.  Instances whose checks fail, or marked by an earlier call,
- * run on the exactly dividing form of the same kernel (IEEE divisions, the
- * same factorisation) in the same call.
+ * that overlap.  An instance whose checks fail is redone at once, from the
+ * progress it was taken up with, by the same workgroup in the exactly
+ * dividing form (IEEE divisions, the same factorisation), and is not marked;
+ * instances marked by an earlier call (bit 0 of `reserved`, e.g. by
+ * dh_integrate_resident) run on that form in the same call.
  * Replaces what dh_integrate_resident replaces.
  */
 int dh_integrate_resident_latency(dh_instance *table_dev, int ninst, int order, int n,
