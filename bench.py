@@ -143,7 +143,9 @@ def model_workloads(args, rng):
     (engine.map_batched coalesces their integrations into shared launches):
     split_mig and the 3-population chain extrapolated over three grids, and
     the DFE caches (Cache1D of two_epoch_sel; optionally Cache2D of
-    split_mig_sel)."""
+    split_mig_sel); optionally one 3- or 4-population integration's step time
+    on the multi-workgroup sweeps."""
+    import torch
     from dadi_amd import engine, Numerics, Demographics2D, Demographics3D
     from dadi_amd.DFE import DemogSelModels, Cache1D_mod, Cache2D_mod
     out = {}
@@ -204,6 +206,32 @@ def model_workloads(args, rng):
         out["dfe_cache1d"] = {"value": el, "unit": "s", "spectra": len(c1.gammas) + 1,
                               "model": "two_epoch_sel nu=2 T=0.1, 500 gammas, pts (200, 220, "
                                        "240), ns 50"}
+    if args.sweep:
+        # one 3- or 4-population integration on the multi-workgroup path
+        # (dh_integrate_batch), constant parameters, the serving loop's and
+        # the chain evaluations' epochs at one size: microseconds per step
+        from dadi_amd import schedule as S, _lib
+        D, pts, nst = (int(v) for v in args.sweep.split(","))
+        ctx = engine.context()
+        xx = Numerics.default_grid(pts)
+        seg = S.const_segments(D, xx, 0.0, 0.05, 0.05 / nst, [1.3] * D, [[0.7] * (D - 1)] * D,
+                               [0.0] * D, [0.5] * D, 1.0)
+        order = _lib.ORDER_PRE if D <= 3 else _lib.ORDER_TV
+        init = np.random.default_rng(0).uniform(0.5, 1.0, (pts,) * D)
+        phi = ctx.empty_phi(init.shape)
+        best = None
+        for _ in range(2):
+            phi.copy_(torch.from_numpy(init))
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            engine.PackedBatch(ctx, [engine.Job(phi, xx, seg, order, 0, 1)]).launch_spread(
+                ctx.stream())
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t0
+            best = el if best is None else min(best, el)
+        out["sweep_steps"] = {"value": 1e6 * best / nst, "unit": "us/step", "steps": nst,
+                              "model": "one %d-population integration, pts %d, constant "
+                                       "parameters, multi-workgroup sweeps" % (D, pts)}
     if args.dfe2d:
         nu1, nu2, T, m = draws(rng, 1)[0]
         c2, el = timed(lambda: Cache2D_mod.Cache2D((nu1, nu2, T, m), (20, 20),
@@ -290,6 +318,9 @@ def main():
                     help="BFGS iterations of the optimiser comparison (0 = skip)")
     ap.add_argument("--dfe2d", type=int, default=1,
                     help="time a Cache2D build (10x10 gammas; long: up to ~5e5*T steps)")
+    ap.add_argument("--sweep", default="",
+                    help="D,pts,steps: time one D-population integration (D = 3, 4) on the "
+                         "multi-workgroup sweeps; '' = skip")
     ap.add_argument("--routing", default="512,2048",
                     help="batch sizes (comma-separated, from the same draws) timed with the "
                          "automatic kernel choice and with the packed kernel; '' = skip")
