@@ -302,10 +302,61 @@ __device__ __forceinline__ void regamma(int cb, int n, double bt,
     }
 }
 
+// BP interior rows j0 .. j0+BP-1 (all below the last row) of the forward
+// elimination: fwd_row's operations, with the block's table entries loaded
+// before any of its stores (the gamma column shares the LDS with the tables,
+// so loads after a store could not move above it: each row waited for its
+// own table reads).  Used by the multi-workgroup sweeps (one wave per SIMD);
+// the persistent kernels, register-bound, keep fwd_row.
+template <int NO, int ORDER, int BP>
+__device__ __forceinline__ void fwd_block_interior(
+    int j0, const double (&cur)[BP], const double *__restrict__ xInt,
+    const double *__restrict__ df, const double *__restrict__ W0,
+    const double *__restrict__ W1, const double *__restrict__ S, long long ts,
+    const double *c, const double *m, double dt, double inv_dt, double &bet, double &u,
+    double &pa, double &pc, double *__restrict__ p, long long s, double *__restrict__ gam,
+    long long gs)
+{
+    double tdf[BP + 1], txi[BP], tsl[BP], tw0[BP], tw1[BP];
+    tdf[0] = df[(j0 - 1) * ts];
+#pragma unroll
+    for (int q = 0; q < BP; ++q) {
+        const long long o = (long long)(j0 + q) * ts;
+        tdf[q + 1] = df[o];
+        txi[q] = xInt[o];
+        tsl[q] = S[o];
+        tw0[q] = W0[o];
+        tw1[q] = W1[o];
+    }
+#pragma unroll
+    for (int q = 0; q < BP; ++q) {
+        // row_coeffs (j < n - 1, no delta trick) and fwd_core
+        const double dfj = tdf[q + 1];
+        const double a = -dfj * pa;
+        const double cm = -tdf[q] * pc;
+        const double Mj = d_M<NO>(c, m, txi[q], tsl[q]);
+        const double na = Mj * 0.5 + tw0[q];
+        const double nc = -Mj * (1 - 0.5) + tw1[q];
+        double b;
+        if constexpr (ORDER == DH_ORDER_TV)
+            b = (inv_dt + dfj * pc) + dfj * na;
+        else
+            b = (dfj * na + dfj * pc) + inv_dt;
+        pa = na;
+        pc = nc;
+        const double g = cm / bet;
+        bet = b - a * g;
+        const double r = (ORDER == DH_ORDER_PRE) ? inv_dt * cur[q] : cur[q] / dt;
+        u = (r - a * u) / bet;
+        gam[(j0 + q) * gs] = g;
+        p[(j0 + q) * s] = u;
+    }
+}
+
 // One line: inject (optional), build a/b/c on the fly, Thomas forward with u'
 // over phi in place and gamma_j in `gam`, then back substitution.
 //   ts: stride of the 1D tables; gs: stride of the gamma column.
-template <int NO, int ORDER, int DJ = 0, int BP = PF>
+template <int NO, int ORDER, int DJ = 0, int BP = PF, bool IB = false>
 __device__ __forceinline__ void solve_line(
     double *__restrict__ p, long long s, int n,
     const double *__restrict__ xInt, const double *__restrict__ df,
@@ -347,7 +398,10 @@ __device__ __forceinline__ void solve_line(
             const int jj = j0 + BP + q;
             nxt[q] = p[(jj < n ? jj : n - 1) * s];
         }
-        if (j0 + BP <= n) {
+        if (IB && !DJ && j0 + BP <= n - 1) {
+            fwd_block_interior<NO, ORDER, BP>(j0, cur, xInt, df, W0, W1, S, ts, c, m, dt,
+                                              inv_dt, bet, u, pa, pc, p, s, gam, gs);
+        } else if (j0 + BP <= n) {
 #pragma unroll
             for (int q = 0; q < BP; ++q)
                 fwd_row<NO, ORDER, DJ>(j0 + q, n, cur[q], xInt, df, W0, W1, S, ts, c, m, dt,
@@ -1146,7 +1200,7 @@ __device__ __forceinline__ void line_addr(int n, long long ld, int a, long long 
 
 // All lines of axis `a`, tiles of 64 lines per wave.  GG: gamcol is this
 // lane's gamma column in the global workspace (rows 64 doubles apart).
-template <int D, int ORDER, int CK, int RU, int GG, int DJ = 0, int BP = PF>
+template <int D, int ORDER, int CK, int RU, int GG, int DJ = 0, int BP = PF, bool IB = false>
 __device__ __forceinline__ void sweep_tiles(double *__restrict__ phi, int n, long long ld, int a,
                                             bool first_axis,
                                             int injm, const dh_segment *sp,
@@ -1204,7 +1258,7 @@ __device__ __forceinline__ void sweep_tiles(double *__restrict__ phi, int n, lon
             line_addr<D>(n, ld, a, line, off, s);
             double *p = phi + off;
             if constexpr (CK == 0)
-                solve_line<NO, ORDER, DJ, BP>(p, s, n, sxi, sdf, W0, W1, S, 1, L.c, mig, dt, inv_dt,
+                solve_line<NO, ORDER, DJ, BP, IB>(p, s, n, sxi, sdf, W0, W1, S, 1, L.c, mig, dt, inv_dt,
                                           L.bcf_on, L.bcf, L.bcl_on, L.bcl, L.inj_row, inj,
                                           gamcol, 64, dj);
             else if constexpr (RU)
@@ -3487,7 +3541,7 @@ __global__ void __launch_bounds__(256) k_sweep_axis(dh_instance inst, int a, int
     build_axis_tables<D>(sp, a, n, sx, sxi, sdx, W0, W1, S, tid, nthr);
     __syncthreads();
     const double dt = sp->dt, inv_dt = 1. / dt;
-    sweep_tiles<D, ORDER, 0, 0, 0, 0, SWEEP_PF>(gbl(inst.phi_dev), n, inst.ld > 0 ? inst.ld : n, a,
+    sweep_tiles<D, ORDER, 0, 0, 0, 0, SWEEP_PF, true>(gbl(inst.phi_dev), n, inst.ld > 0 ? inst.ld : n, a,
                                    first_axis != 0, inst.inject, sp, sx, sxi, sdf, sdx, W0, W1,
                                    S, dt, inv_dt, gamcol, nullptr,
                                    (long long)blockIdx.x * nw + wave,
@@ -3526,7 +3580,7 @@ __global__ void __launch_bounds__(256) k_sweep_batch(const dh_sweep_item *__rest
     const double dt = sp->dt, inv_dt = 1. / dt;
     double *gamcol = S + n + (long long)wave * 64 * col + lane;
     double *stage = S + n + (long long)nw * 64 * col + (long long)wave * 64 * (STAGE_C + 1);
-    sweep_tiles<D, ORDER, CK, 0, 0, 0, SWEEP_PF>(gbl(I.phi_dev), n, I.ld > 0 ? I.ld : n, a,
+    sweep_tiles<D, ORDER, CK, 0, 0, 0, SWEEP_PF, true>(gbl(I.phi_dev), n, I.ld > 0 ? I.ld : n, a,
                                     a == it.first_axis, I.inject, sp, sx, sxi, sdf, sdx, W0, W1,
                                     S, dt, inv_dt, gamcol, stage,
                                     (long long)(blockIdx.x - it.wg_begin) * nw + wave,
