@@ -1617,7 +1617,6 @@ k_integrate_packed(dh_instance *__restrict__ tab, int ninst, int *__restrict__ c
 //      tables are repeated per axis so that one address reaches a row's five
 //      values) | transpose tile 64 x (NP + 1).
 // ---------------------------------------------------------------------------
-constexpr int RR_MAX_NP = 112;
 constexpr int RR_SPAN = 8;   // n values one kernel covers
 
 template <int NP>
@@ -1629,13 +1628,12 @@ constexpr size_t rr_lds_doubles() { return 12 * (size_t)NP + 64 * (size_t)(NP + 
 // Returns `off` unchanged, through an empty asm that also consumes `dep`: no
 // instruction is emitted, but whatever is addressed through the result
 // cannot be loaded before `dep` is computed.  Row j's coefficient tables are
-// addressed through the u of row j - RR_AHEAD, so their loads (and the
+// addressed through the u of row j - 2, so their loads (and the
 // coefficient arithmetic on them) start a fixed distance ahead of the
 // recurrence instead of all at once; hoisted, every row's coefficients would
 // be live together, and within a segment they are even invariant across
 // steps.  `off` is the byte offset of the axis' table block in LDS, so each
 // table read is one LDS load with an immediate offset.
-constexpr int RR_AHEAD = 2;
 
 // A zero the compiler cannot see through, produced anew at each use: values
 // formed from it inside the step loop (the line setup of a sweep) stay inside
