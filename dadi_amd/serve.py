@@ -164,8 +164,8 @@ class Server:
     quantum: time steps each in-flight integration advances per launch.
     wide_streams: streams the multi-workgroup (3-5 population) epochs are
     spread over, by request; measured on MI355X with the seeded 200-request
-    mix, p99 0.44-0.45 s with 4 against 0.47-0.52 s with 2 and 0.54-0.58 s
-    with 6 or 8.
+    mix, p99 0.43-0.56 s (median of five runs 0.44 s) with 4 against
+    0.47-0.52 s with 2 and 0.54-0.58 s with 6 or 8.
     """
 
     def __init__(self, budget_bytes=8 << 30, quantum=200, ctx=None, quantum_ms=8.0,
