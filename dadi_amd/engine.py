@@ -61,6 +61,8 @@ class Context:
     #: each integration's sweeps over several workgroups; 0 disables it
     spread_max_inst = 512
     spread_waves = 0          # 0: chosen per batch
+    #: spread batches run in phases that drop finished integrations
+    spread_phases = True
     #: 2D batches whose grid size has a register-resident kernel use it when
     #: no other launch shape is asked for; False keeps them on the others
     resident_auto = True
@@ -717,53 +719,73 @@ class PackedBatch:
         return len(self.jobs) <= ctx.spread_max_inst
 
     def launch_spread(self, stream, waves=None, steps_per_graph=8, max_wgs=4096):
-        """Run every job to completion with dh_integrate_batch."""
+        """Run every job to completion with dh_integrate_batch, in phases:
+        each phase launches only the integrations still running, and ends
+        (on a multiple of steps_per_graph) when half of them have finished,
+        so the long tail of a batch of unequal integrations is not swept by
+        the workgroups of finished ones (Context.spread_phases False: one
+        phase).  Each phase picks its own launch shape."""
         ctx = self.ctx
         t = torch()
-        items = np.zeros(len(self.jobs), dtype=ITEM_DTYPE)
         raw = self.tab.cpu().numpy().view(INST_DTYPE)
         keep = []
-        tiles = [-(-(j.n ** (j.D - 1)) // 64) for j in self.jobs]
-        total_tiles = sum(tiles)
-        if waves is None:
-            # two waves per workgroup (each lane's whole gamma column in LDS)
-            # while that leaves at most one workgroup per compute unit, else
-            # four (more workgroups resident per compute unit)
-            waves = ctx.spread_waves or (2 if total_tiles <= 2 * ctx.num_cu else 4)
-        # workgroups per integration: one wave per 64-line tile while the
-        # launch stays within max_wgs workgroups
-        scale = min(1.0, max_wgs * waves / max(total_tiles, 1))
-        wg = 0
-        wg_item = []
-        longest = 0
-        for i, (j, nt) in enumerate(zip(self.jobs, tiles)):
-            items[i]["inst"] = raw[i]
+        totals, sos_ptr, first_axis = [], [], []
+        for j in self.jobs:
             act = [k for k in range(j.D) if not (j.frozen >> k) & 1]
             total = int(np.sum(j.seg["nsteps"])) if act else 0
             sos = ctx.to_dev(np.repeat(np.arange(len(j.seg), dtype=np.int32), j.seg["nsteps"])
                              if total else np.zeros(1, dtype=np.int32))
             keep.append(sos)
-            items[i]["seg_of_step"] = sos.data_ptr()
-            items[i]["total_steps"] = total
-            items[i]["first_axis"] = act[0] if act else 0
-            w = max(1, int(-(-nt * scale // waves)))
-            items[i]["wg_begin"] = wg
-            items[i]["wgs"] = w
-            wg_item.extend([i] * w)
-            wg += w
-            longest = max(longest, total)
+            totals.append(total)
+            sos_ptr.append(sos.data_ptr())
+            first_axis.append(act[0] if act else 0)
+        longest = max(totals) if totals else 0
         if longest == 0:
             return
-        itd = ctx.to_dev(items.view(np.uint8))
-        wgd = ctx.to_dev(np.asarray(wg_item, dtype=np.int32))
-        base = t.zeros(1, dtype=t.int64, device=ctx.device)
-        nmax = max(j.n for j in self.jobs)
-        check(ctx.lib.dh_integrate_batch(itd.data_ptr(), wgd.data_ptr(), len(self.jobs), wg,
-                                         self.D, self.order, nmax, waves, base.data_ptr(),
-                                         longest, 0, longest, steps_per_graph, stream),
-              "dh_integrate_batch")
+        spg = steps_per_graph
+        start = 0
+        while start < longest:
+            live = [i for i, tot in enumerate(totals) if tot > start]
+            end = longest
+            if ctx.spread_phases and len(live) > 1:
+                half = sorted(totals[i] for i in live)[len(live) // 2 - 1]
+                end = min(longest, start + max(1, -(-(half - start) // spg)) * spg)
+            tiles = [-(-(self.jobs[i].n ** (self.D - 1)) // 64) for i in live]
+            total_tiles = sum(tiles)
+            w_per = waves
+            if w_per is None:
+                # two waves per workgroup (each lane's whole gamma column in
+                # LDS) while that leaves at most one workgroup per compute
+                # unit, else four (more workgroups resident per compute unit)
+                w_per = ctx.spread_waves or (2 if total_tiles <= 2 * ctx.num_cu else 4)
+            # workgroups per integration: one wave per 64-line tile while the
+            # launch stays within max_wgs workgroups
+            scale = min(1.0, max_wgs * w_per / max(total_tiles, 1))
+            items = np.zeros(len(live), dtype=ITEM_DTYPE)
+            wg = 0
+            wg_item = []
+            for k, (i, nt) in enumerate(zip(live, tiles)):
+                items[k]["inst"] = raw[i]
+                items[k]["seg_of_step"] = sos_ptr[i]
+                items[k]["total_steps"] = totals[i]
+                items[k]["first_axis"] = first_axis[i]
+                w = max(1, int(-(-nt * scale // w_per)))
+                items[k]["wg_begin"] = wg
+                items[k]["wgs"] = w
+                wg_item.extend([k] * w)
+                wg += w
+            itd = ctx.to_dev(items.view(np.uint8))
+            wgd = ctx.to_dev(np.asarray(wg_item, dtype=np.int32))
+            base = t.zeros(1, dtype=t.int64, device=ctx.device)
+            nmax = max(self.jobs[i].n for i in live)
+            check(ctx.lib.dh_integrate_batch(itd.data_ptr(), wgd.data_ptr(), len(live), wg,
+                                             self.D, self.order, nmax, w_per, base.data_ptr(),
+                                             longest, start, end - start, spg, stream),
+                  "dh_integrate_batch")
+            keep.append((itd, wgd, base))
+            start = end
         # the buffers must outlive the queued work
-        self.workspace["spread_keep"] = (itd, wgd, base, keep)
+        self.workspace["spread_keep"] = keep
 
     def latency_bound(self, waves=None, chunk=None, recompute=None, gamma_global=None,
                       slots=None):
