@@ -743,13 +743,7 @@ class PackedBatch:
         if longest == 0:
             return
         spg = steps_per_graph
-        start = 0
-        while start < longest:
-            live = [i for i, tot in enumerate(totals) if tot > start]
-            end = longest
-            if ctx.spread_phases and len(live) > 1:
-                half = sorted(totals[i] for i in live)[len(live) // 2 - 1]
-                end = min(longest, start + max(1, -(-(half - start) // spg)) * spg)
+        for start, end, live in spread_phases(totals, spg, ctx.spread_phases):
             tiles = [-(-(self.jobs[i].n ** (self.D - 1)) // 64) for i in live]
             total_tiles = sum(tiles)
             w_per = waves
@@ -783,7 +777,6 @@ class PackedBatch:
                                              longest, start, end - start, spg, stream),
                   "dh_integrate_batch")
             keep.append((itd, wgd, base))
-            start = end
         # the buffers must outlive the queued work
         self.workspace["spread_keep"] = keep
 
@@ -805,6 +798,26 @@ class PackedBatch:
     def progress(self):
         raw = self.tab.cpu().numpy().view(INST_DTYPE)
         return raw
+
+
+def spread_phases(totals, spg, phased=True):
+    """Phases (start, end, live) of a spread batch whose integrations take
+    ``totals`` steps: steps start .. end-1 run the integrations in ``live``
+    (those with more than ``start`` steps), each phase ending on a multiple
+    of ``spg`` steps after its start (or at the longest schedule) once half
+    of its integrations have finished; phased False: one phase."""
+    longest = max(totals) if len(totals) else 0
+    out = []
+    start = 0
+    while start < longest:
+        live = [i for i, tot in enumerate(totals) if tot > start]
+        end = longest
+        if phased and len(live) > 1:
+            half = sorted(totals[i] for i in live)[len(live) // 2 - 1]
+            end = min(longest, start + max(1, -(-(half - start) // spg)) * spg)
+        out.append((start, end, live))
+        start = end
+    return out
 
 
 def run_batch(jobs, ctx=None, waves=None, groups=0, sort=True, chunk=None, recompute=None,

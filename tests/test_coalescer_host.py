@@ -1,10 +1,13 @@
 """
 CPU tests of the coalescer that merges the integrations of concurrent model
 evaluations into shared launches (the launch itself is replaced by a
-recorder: no GPU).
+recorder: no GPU), and of the phase plan of spread batches.
 """
 import random
 import time
+
+import numpy as np
+import pytest
 
 from dadi_amd import engine
 
@@ -115,3 +118,30 @@ def test_failed_launch_raises_in_every_worker_of_it(monkeypatch):
 
     out = engine.map_batched(model, [(k,) for k in range(16)], workers=16, ctx=object())
     assert all(isinstance(r, RuntimeError) for r in out)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_spread_phases_cover_every_step_once(seed):
+    """The phases of a spread batch cover steps 0 .. longest-1 once, in
+    order; each runs exactly the integrations that still have steps then;
+    all but the last end a multiple of the graph's steps after they start;
+    each drops at least half of its integrations before the next; without
+    phasing there is one phase with everything."""
+    from dadi_amd.engine import spread_phases
+    rng = np.random.default_rng(seed)
+    totals = list(rng.integers(0, 5000, rng.integers(1, 300)))
+    totals[0] = max(totals[0], 1)
+    spg = int(rng.choice([1, 4, 8, 16]))
+    ph = spread_phases(totals, spg)
+    longest = max(totals)
+    assert ph[0][0] == 0 and ph[-1][1] == longest
+    for (s0, e0, live), nxt in zip(ph, ph[1:] + [None]):
+        assert s0 < e0
+        assert live == [i for i, t in enumerate(totals) if t > s0]
+        if nxt is not None:
+            assert nxt[0] == e0
+            assert (e0 - s0) % spg == 0
+            assert len(nxt[2]) <= len(live) - len(live) // 2
+    one = spread_phases(totals, spg, phased=False)
+    assert one == [(0, longest, [i for i, t in enumerate(totals) if t > 0])]
+    assert spread_phases([0, 0], spg) == []
