@@ -21,18 +21,26 @@ except AttributeError:  # numpy < 2
     _trapz = np.trapz
 
 
-def _evaluate_grid(demo_sel_func, param_sets, ns, pts, workers):
+def _evaluate_grid(demo_sel_func, param_sets, ns, pts, workers, cost=None):
     """Extrapolated spectra of demo_sel_func for every parameter set: each
-    (set, grid size) is one worker task; their integrations share launches."""
-    tasks = [(tuple(p), tuple(ns), int(pt)) for p in param_sets for pt in pts]
+    (set, grid size) is one worker task; their integrations share launches.
+    cost: a number per set; sets are evaluated most costly first, so that the
+    launch rounds (at most `workers` tasks each, each bounded by its longest
+    integration) hold integrations of similar length.  Results come back in
+    the order of param_sets either way."""
+    order = list(range(len(param_sets)))
+    if cost is not None:
+        order.sort(key=lambda i: -cost[i])
+    tasks = [(tuple(param_sets[i]), tuple(ns), int(pt)) for i in order for pt in pts]
     raw = engine.map_batched(lambda p, n, pt: demo_sel_func(p, n, pt), tasks, workers=workers)
     for r in raw:
         if isinstance(r, Exception):
             raise r
-    out = []
     k = len(pts)
+    where = {i: r for r, i in enumerate(order)}
+    out = []
     for i in range(len(param_sets)):
-        res = raw[i * k:(i + 1) * k]
+        res = raw[where[i] * k:(where[i] + 1) * k]
         if k == 1:
             out.append(res[0])
         else:

@@ -28,7 +28,11 @@ class Cache2D:
         pairs = [(i, j) for i in range(ng) for j in range(ng)]
         mine = pairs[this_job_id::split_jobs]
         sets = [tuple(params) + (self.gammas[i], self.gammas[j]) for i, j in mine]
-        specs = _evaluate_grid(demo_sel_func, sets, ns, pts, workers)
+        # the more selective population of a pair sets its time step, so the
+        # number of steps grows with max |gamma|: pairs go out most selective
+        # first (launch rounds of similar integrations)
+        cost = [max(abs(self.gammas[i]), abs(self.gammas[j])) for i, j in mine]
+        specs = _evaluate_grid(demo_sel_func, sets, ns, pts, workers, cost)
         self.spectra = [[None] * ng for _ in range(ng)]
         for (i, j), s in zip(mine, specs):
             self.spectra[i][j] = s.data if hasattr(s, 'data') else s

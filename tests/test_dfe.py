@@ -170,3 +170,26 @@ def test_cache2d_point_pos_quadrant_weights():
     np.testing.assert_allclose(np.asarray(got)[keep], want[keep], rtol=1e-12)
     with pytest.raises(IndexError):
         c.integrate_point_pos(dens + [p1, 4.0, p2, 3.0], None, PDFs.biv_lognormal, 2.0)
+
+
+def test_cache2d_pairs_go_most_selective_first_and_come_back_in_order():
+    """Cache2D hands its gamma pairs to the workers most selective first (the
+    larger |gamma| of a pair sets its time step, so launch rounds then hold
+    integrations of similar length) and stores every spectrum at its own
+    (i, j), as the unsorted loop does (host only: the model is a recorder)."""
+    from dadi_amd.DFE import Cache2D
+    seen = []
+
+    def model(p, ns, pts):
+        seen.append(tuple(p[-2:]))
+        return np.full(tuple(n + 1 for n in ns), 1000 * p[-2] + p[-1])
+
+    c = Cache2D((1.0,), (2, 2), model, [10], gamma_bounds=(1e-2, 100), gamma_pts=4,
+                workers=1)
+    g = c.gammas
+    key = [max(abs(a), abs(b)) for a, b in seen]
+    assert key == sorted(key, reverse=True)
+    assert sorted(seen) == sorted((a, b) for a in g for b in g)
+    for i in range(len(g)):
+        for j in range(len(g)):
+            assert np.all(c.spectra[i][j] == 1000 * g[i] + g[j])
