@@ -2437,6 +2437,36 @@ __device__ __forceinline__ bool rf_factor(double2 *__restrict__ AB, double2 *__r
     return in & rr_pos(betl) & (__builtin_fabs(ca) <= RR_COEF_HI);
 }
 
+// A pointer that is the same in every lane, moved into scalar registers
+// (readfirstlane of both halves), as a global-address-space pointer.
+__device__ __forceinline__ const __attribute__((address_space(1))) char *rf_sbase(const char *p)
+{
+    const unsigned long long v = reinterpret_cast<unsigned long long>(p);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return reinterpret_cast<const __attribute__((address_space(1))) char *>(
+        ((unsigned long long)hi << 32) | lo);
+}
+
+// A record of a row: the row's base (uniform) plus this lane's 32-bit byte
+// offset, so the load takes its base from scalar registers and its offset
+// from one vector register, with no vector address arithmetic (64-bit
+// addresses formed per row were a third of the forward pass's VALU
+// instructions).
+__device__ __forceinline__ double2 rf_rec2(const char *row, int voff)
+{
+    typedef double v2d __attribute__((ext_vector_type(2)));
+    const v2d v = *reinterpret_cast<const __attribute__((address_space(1))) v2d *>(
+        rf_sbase(row) + (unsigned)voff);
+    return make_double2(v.x, v.y);
+}
+
+__device__ __forceinline__ double rf_rec1(const char *row, int voff)
+{
+    return *reinterpret_cast<const __attribute__((address_space(1))) double *>(
+        rf_sbase(row) + (unsigned)voff);
+}
+
 // One implicit half-step of this lane's line from its factorisation (the
 // records of line L): the operations of rr_sweep's rows (row 0 and the last
 // row by IEEE division, rows 1..n-2 by the shared reciprocal y_j), records
@@ -2452,9 +2482,10 @@ __device__ __forceinline__ void rf_sweep(RrLine<NP, rf_vrows<NP>()> &P,
     constexpr size_t ROW = (size_t)NP * sizeof(double2);   // bytes per row of records
     if (inj_row == 0) P.set(0, P.get(0) + inj);
     if (inj_row == 1) P.set(1, P.get(1) + inj);
-    // uniform row bases plus this lane's 32-bit byte offset; the offset is
-    // passed through rr_after before each load, which ties the load to the
-    // recurrence (it is not hoisted) at no instruction cost
+    // uniform row bases (scalar registers) plus this lane's 32-bit byte
+    // offset; the offset is passed through rr_after before each load, which
+    // ties the load to the recurrence (it is not hoisted) at no instruction
+    // cost
     const char *ab0 = reinterpret_cast<const char *>(AB);
     const char *yg0 = reinterpret_cast<const char *>(YG);
     int voff = L * (int)sizeof(double2);
@@ -2462,8 +2493,8 @@ __device__ __forceinline__ void rf_sweep(RrLine<NP, rf_vrows<NP>()> &P,
     double fy[RF_PF];
 #pragma unroll
     for (int q = 0; q < RF_PF; ++q) {
-        fab[q] = *reinterpret_cast<const double2 *>(ab0 + q * ROW + (unsigned)voff);
-        fy[q] = *reinterpret_cast<const double *>(yg0 + q * ROW + (unsigned)voff);
+        fab[q] = rf_rec2(ab0 + q * ROW, voff);
+        fy[q] = rf_rec1(yg0 + q * ROW, voff);
     }
     double u;
     {
@@ -2480,8 +2511,8 @@ __device__ __forceinline__ void rf_sweep(RrLine<NP, rf_vrows<NP>()> &P,
             // row j - 1's slot takes row j - 1 + RF_PF, issued once u_{j-1} is known
             voff = rr_after(voff, u);
             const size_t row = (size_t)(j - 1 + RF_PF) * ROW;
-            fab[(j - 1) % RF_PF] = *reinterpret_cast<const double2 *>(ab0 + row + (unsigned)voff);
-            fy[(j - 1) % RF_PF] = *reinterpret_cast<const double *>(yg0 + row + (unsigned)voff);
+            fab[(j - 1) % RF_PF] = rf_rec2(ab0 + row, voff);
+            fy[(j - 1) % RF_PF] = rf_rec1(yg0 + row, voff);
         }
         if (j < SURE || j < n) {
             const double pj = P.get(j);
@@ -2510,8 +2541,7 @@ __device__ __forceinline__ void rf_sweep(RrLine<NP, rf_vrows<NP>()> &P,
 #pragma unroll
     for (int q = 0; q < RF_PB; ++q) {
         const int j = NP - 2 - q;
-        gb[q] = j >= 0 ? *reinterpret_cast<const double *>(g0 + (size_t)j * ROW + (unsigned)voff)
-                       : 0.;
+        gb[q] = j >= 0 ? rf_rec1(g0 + (size_t)j * ROW, voff) : 0.;
     }
 #pragma unroll
     for (int j = NP - 2; j >= 0; --j) {
@@ -2519,8 +2549,7 @@ __device__ __forceinline__ void rf_sweep(RrLine<NP, rf_vrows<NP>()> &P,
         const double g = gb[s];
         if (j - RF_PB >= 0) {
             voff = rr_after(voff, u);
-            gb[s] = *reinterpret_cast<const double *>(g0 + (size_t)(j - RF_PB) * ROW +
-                                                      (unsigned)voff);
+            gb[s] = rf_rec1(g0 + (size_t)(j - RF_PB) * ROW, voff);
         }
         if (j < SURE || j <= n - 2) {
             const double v = P.get(j) - g * u;
