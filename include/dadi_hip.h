@@ -283,7 +283,7 @@ int dh_resident_rows(int n);
  * factorisation once (a_j, bet_j, its reciprocal, gamma_{j+1}) into the
  * workspace cfg->gamma_dev, so that a time step runs only the right-hand-side
  * recurrences: three quarters of dh_integrate_resident's step time for one
- * integration (20 against 27 us at 100 points), more memory traffic per step
+ * integration (18 against 27 us at 100 points), more memory traffic per step
  * (32 bytes of records per point and sweep, from the L2), so it is meant for
  * single integrations, batches too small to fill the GPU and grids above 104
  * points, where dh_integrate_resident has no kernel.  cfg->gamma_dev must hold
