@@ -328,6 +328,9 @@ def main():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-warmup", type=int, default=20,
+                    help="requests of a short untimed mix served first (kernel loading, "
+                         "allocator growth)")
     ap.add_argument("--seed", type=int, default=20250101)
     ap.add_argument("--calibrate-copy", type=float, default=0.0,
                     help="only run 3 device copies of this many GiB (rocprofv3 counter "
@@ -502,9 +505,14 @@ def main():
     serving = None
     if args.serve_requests > 0:
         from dadi_amd import serve
+        if args.serve_warmup > 0:   # untimed: the same mix with epochs 20x shorter
+            serve.Server().run(serve.mixed_requests(np.random.default_rng(args.seed + 3),
+                                                    args.serve_warmup, 200.0, scale=0.05),
+                               realtime=False)
         reqs = serve.mixed_requests(np.random.default_rng(args.seed + 1), args.serve_requests,
                                     args.serve_rate)
         serving = serve.Server().run(reqs)
+        serving.update({"warmup_requests": args.serve_warmup})
         serving.update({"rate_req_s": args.serve_rate, "mix": "55% 2D split_mig pts 40-120, "
                         "30% 1D three_epoch pts 100-600, 12% 3D chain pts 30-60, "
                         "3% 4D chain pts 16-24; Poisson arrivals, seeded"})
