@@ -502,6 +502,10 @@ def main():
                         "same_result": bool(np.array_equal(outs["auto"], outs["packed"]))})
         del rpb, rjobs, ar
 
+    workloads = model_workloads(args, np.random.default_rng(args.seed + 2))
+
+    # serving last: a Server run left the host-bound model workloads above
+    # slower when they ran after it (DFE Cache2D 7.6 -> 12.1 s after two runs)
     serving = None
     if args.serve_requests > 0:
         from dadi_amd import serve
@@ -516,8 +520,6 @@ def main():
         serving.update({"rate_req_s": args.serve_rate, "mix": "55% 2D split_mig pts 40-120, "
                         "30% 1D three_epoch pts 100-600, 12% 3D chain pts 30-60, "
                         "3% 4D chain pts 16-24; Poisson arrivals, seeded"})
-
-    workloads = model_workloads(args, np.random.default_rng(args.seed + 2))
 
     achieved = alg_bytes / kavg
     if resident:
